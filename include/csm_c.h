@@ -1,0 +1,159 @@
+/* C-ABI of the MI355X incremental match engine.
+ *
+ * One engine owns a device-resident dynamic labelled graph and one or more
+ * registered query graphs; every edge update is answered with the exact
+ * number of query embeddings it creates (insertion) or destroys (deletion).
+ * Plain pointers and sizes only; no exceptions cross this boundary and the
+ * library never exits the process.  Calls on one engine must be serialised by
+ * the caller (one host thread per engine); csm_classify is read-only.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository, ParaCOSM/CSM/...).
+ */
+#ifndef CSM_C_H
+#define CSM_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes.  0 = success, negative = error (details: csm_last_error). */
+#define CSM_OK 0
+#define CSM_E_ARG -1            /* bad argument / out-of-range id          */
+#define CSM_E_UNKNOWN_ALGO -2   /* main.cpp:760-764 "Unknown algorithm"    */
+#define CSM_E_DELETE_MISSING -3 /* graph.cpp:98-102 "deletion error"       */
+#define CSM_E_DEVICE -4         /* HIP runtime error or no device          */
+#define CSM_E_CAPACITY -5       /* batch/label/id exceeds a fixed capacity */
+#define CSM_E_UNSUPPORTED -6    /* not implemented on the device yet       */
+#define CSM_E_TIME_LIMIT -7     /* watchdog fired; counts are partial      */
+#define CSM_E_ADMISSION -8      /* work refused: device memory budget      */
+
+typedef struct csm_engine csm_engine;
+
+/* A labelled undirected graph given as arrays (vertex ids are 0..n-1).
+ * Mirrors what Graph::LoadFromFile builds (graph.cpp:217-249). */
+typedef struct csm_graph {
+  uint32_t num_vertices;
+  const uint32_t* vlabels;   /* [num_vertices]                      */
+  size_t num_edges;
+  const uint32_t* src;       /* [num_edges]                         */
+  const uint32_t* dst;       /* [num_edges]                         */
+  const uint32_t* elabels;   /* [num_edges]                         */
+} csm_graph;
+
+/* One stream record; mirrors InsertUnit (utils/types.h:34-42). */
+typedef struct csm_update {
+  uint8_t type;    /* 'e' or 'v'                                   */
+  uint8_t is_add;  /* 1 insertion, 0 deletion                      */
+  uint16_t pad;
+  uint32_t id1;    /* vertex id, or edge endpoint                  */
+  uint32_t id2;    /* edge endpoint (edges only)                   */
+  uint32_t label;  /* vertex label, or edge label (insertions)     */
+} csm_update;
+
+/* Options; mirror the reference CLI flags (main.cpp:681-703) plus device
+ * knobs.  Zero-initialise and set what you need. */
+typedef struct csm_opts {
+  uint64_t max_results;     /* --max-results; 0 = unbounded                */
+  uint32_t print_prep;      /* --print-prep                                */
+  uint32_t print_enum;      /* --print-enum (unsupported on device)        */
+  uint32_t homo;            /* --homo: drop the injectivity test           */
+  uint32_t threads;         /* -t (accepted; device sizes itself)          */
+  uint32_t auto_tuning;     /* --auto-tuning                               */
+  uint32_t max_batch;       /* max updates per csm_apply_batch; 0 = 65534  */
+  uint32_t want_hash;       /* compute per-update embedding-set hashes     */
+  double time_limit_s;      /* --time-limit per batch; 0 = none            */
+  uint64_t device_budget;   /* bytes of HBM the engine may use; 0 = 90%    */
+  uint32_t split_chunk;     /* candidates per depth-2 task; 0 = autotuned  */
+  uint32_t device;          /* HIP device ordinal                          */
+} csm_opts;
+
+/* Creates an engine and uploads the data graph.  `algo` is one of the
+ * reference plugin names: "parallel_graphflow", "parallel_symbi",
+ * "parallel_turboflux", "graphflow", "symbi", "turboflux"
+ * (main.cpp:740-764).  Replaces `new Parallel_Graphflow(...)` etc. and the
+ * matching ctor (matching.h:38-42).  On failure returns NULL and stores a
+ * CSM_E_* code in *err (if err is not NULL). */
+csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph* data,
+                       const csm_opts* opts, int* err);
+
+/* Registers one more standing query on the same data graph (serving); the
+ * first query is id 0.  No reference counterpart (the reference runs one
+ * query per process). */
+int csm_register_query(csm_engine* e, const csm_graph* query, uint32_t* qid);
+uint32_t csm_num_queries(const csm_engine* e);
+
+/* matching::Preprocessing (matching.h:45): matching orders / SymBi DAG and
+ * index build (graphflow.cpp:31-34, symbi.cpp:108-111). */
+int csm_preprocess(csm_engine* e);
+
+/* matching::InitialMatching (matching.h:46): embeddings of query 0 in the
+ * initial graph. */
+int csm_initial(csm_engine* e, uint64_t* num_results);
+
+/* matching::Classify (matching.h:56; parallel_graphflow.cpp:1990-2016):
+ * *safe = 1 iff no registered query edge matches (L(v1), L(v2), label) in
+ * either orientation. */
+int csm_classify(csm_engine* e, uint32_t v1, uint32_t v2, uint32_t label, int* safe);
+
+/* matching::AddEdge (matching.h:48; parallel_graphflow.cpp:2058-2138):
+ * inserts (v1, v2, label) and returns the new embeddings of query 0 in *pos
+ * (summed over queries when several are registered).  One update per launch
+ * sequence: the plain per-update GPU path. */
+int csm_add_edge(csm_engine* e, uint32_t v1, uint32_t v2, uint32_t label, uint64_t* pos);
+
+/* matching::RemoveEdge (matching.h:49; parallel_graphflow.cpp:2171-2250):
+ * counts the embeddings containing (v1, v2) before removing it.  Deleting an
+ * absent edge returns CSM_E_DELETE_MISSING (graph.cpp:98-102). */
+int csm_remove_edge(csm_engine* e, uint32_t v1, uint32_t v2, uint64_t* neg);
+
+/* matching::AddVertex / RemoveVertex (matching.h:50-51; graph.cpp:24-56). */
+int csm_add_vertex(csm_engine* e, uint32_t id, uint32_t label);
+int csm_remove_vertex(csm_engine* e, uint32_t id);
+
+/* Batched form of AddEdge/RemoveEdge over a stream slice (no reference
+ * counterpart: the reference applies updates one at a time in
+ * main.cpp:253-397).  Results are exact per update and per query, as if the
+ * updates were applied in order: row-major [update][query] arrays of
+ * n * num_queries entries.  pos/neg receive the count of insertions/deletions
+ * respectively (the other is 0); safe (n entries, nullable) is the Classify
+ * result; emb_hash (nullable, needs want_hash) the embedding-set hash.
+ * Host pointers. */
+int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t* pos,
+                    uint64_t* neg, uint8_t* safe, uint64_t* emb_hash);
+
+/* Same with every pointer in device memory (HBM) and no host round trip;
+ * `counts` is [n * num_queries] u64 (insertions and deletions together, the
+ * update's type says which).  Asynchronous on the engine's stream; call
+ * csm_sync before reading results. */
+int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
+                           uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash);
+int csm_sync(csm_engine* e);
+
+/* matching::GetNumInitialResults/Positive/Negative (matching.h:60-62). */
+int csm_totals(const csm_engine* e, uint64_t* initial, uint64_t* positive, uint64_t* negative);
+
+/* matching::GetMemoryCost (matching.h:53): edges and vertices held. */
+int csm_memory(const csm_engine* e, size_t* num_edges, size_t* num_vertices);
+
+/* Device bytes currently held by the engine (admission control). */
+int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
+
+/* Engine statistics: [0] DFS tasks run, [1] candidate entries tested,
+ * [2] algorithmic bytes of the search (4 B per candidate entry scanned plus
+ * 4 B per partner entry probed), [3] batches, [4] pool repacks. */
+int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
+
+/* The HIP stream the engine launches on (hipStream_t). */
+void* csm_stream(csm_engine* e);
+
+const char* csm_last_error(const csm_engine* e);
+void csm_destroy(csm_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CSM_C_H */

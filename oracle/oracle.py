@@ -1,0 +1,125 @@
+"""ctypes binding of oracle/liboracle.so (built by oracle/Makefile).
+
+Test infrastructure only: it is the checker, never the thing measured or
+shipped.  See csm_oracle.cpp for the reference files each function restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.orc_graph_create.restype = ctypes.c_void_p
+        L.orc_graph_create.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_size_t, _u32p, _u32p, _u32p]
+        L.orc_graph_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_graph_num_edges.restype = ctypes.c_uint64
+        L.orc_graph_num_edges.argtypes = [ctypes.c_void_p]
+        L.orc_classify.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_brute_count.restype = ctypes.c_uint64
+        L.orc_brute_count.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_run.restype = ctypes.c_int
+        L.orc_run.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                              _u8p, _u32p, _u32p, _u32p,
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                              ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_size_t)]
+        L.orc_graphflow_orders.restype = ctypes.c_int
+        L.orc_graphflow_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p]
+        _lib = L
+    return _lib
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+class OGraph:
+    """A reference-semantics graph (sorted neighbour vectors) held by the oracle."""
+
+    def __init__(self, vlabels, src, dst, elab):
+        vl, s, d, l = _u32(vlabels), _u32(src), _u32(dst), _u32(elab)
+        self._keep = (vl, s, d, l)
+        self.h = lib().orc_graph_create(len(vl), vl, len(s), s, d, l)
+
+    @classmethod
+    def from_query(cls, q):
+        return cls(q.vlabels, q.src, q.dst, q.elabels)
+
+    def num_edges(self) -> int:
+        return int(lib().orc_graph_num_edges(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_graph_destroy(self.h)
+            self.h = None
+
+
+def classify(query: OGraph, l1: int, l2: int, el: int) -> bool:
+    return bool(lib().orc_classify(query.h, l1, l2, el))
+
+
+def brute_count(data: OGraph, query: OGraph, homo: bool = False, want_hash: bool = False):
+    h = ctypes.c_uint64(0)
+    n = lib().orc_brute_count(data.h, query.h, int(homo), ctypes.byref(h) if want_hash else None)
+    return (int(n), int(h.value)) if want_hash else int(n)
+
+
+class RunResult:
+    def __init__(self, pos, neg, hsh, initial, seconds):
+        self.pos, self.neg, self.hash, self.initial, self.seconds = pos, neg, hsh, initial, seconds
+
+
+def run(algo: str, data: OGraph, query: OGraph, add, v1, v2, lab, *, threads: int = 1,
+        homo: bool = False, want_hash: bool = False, want_initial: bool = False,
+        check_index: bool = False, limit: int | None = None) -> RunResult:
+    add = np.ascontiguousarray(np.asarray(add, dtype=np.uint8))
+    v1, v2, lab = _u32(v1), _u32(v2), _u32(lab)
+    n = len(v1) if limit is None else min(limit, len(v1))
+    pos = np.zeros(n, dtype=np.uint64)
+    neg = np.zeros(n, dtype=np.uint64)
+    hsh = np.zeros(n, dtype=np.uint64) if want_hash else None
+    init = ctypes.c_uint64(0)
+    secs = ctypes.c_double(0.0)
+    fail = ctypes.c_size_t(0)
+    rc = lib().orc_run(algo.encode(), data.h, query.h, n, add, v1, v2, lab,
+                       pos.ctypes.data, neg.ctypes.data,
+                       hsh.ctypes.data if hsh is not None else None,
+                       threads, int(homo), ctypes.byref(init) if want_initial else None,
+                       int(check_index), ctypes.byref(secs), ctypes.byref(fail))
+    if rc == -1:
+        raise ValueError(f"unknown algorithm {algo!r}")
+    if rc == -2:
+        raise RuntimeError(f"deletion error at update {fail.value}")
+    if rc == -3:
+        raise AssertionError(f"SymBi index diverged from BuildDCS after update {fail.value}")
+    if rc != 0:
+        raise RuntimeError(f"oracle returned {rc}")
+    return RunResult(pos, neg, hsh, int(init.value) if want_initial else None, secs.value)
+
+
+def graphflow_orders(data: OGraph, query: OGraph, k: int, ne: int) -> np.ndarray:
+    out = np.zeros(max(ne, 1) * k, dtype=np.uint32)
+    n = lib().orc_graphflow_orders(data.h, query.h, out)
+    return out[: n * k].reshape(n, k)

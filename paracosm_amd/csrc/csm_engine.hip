@@ -1,0 +1,1053 @@
+// Host side of the MI355X incremental match engine and its C-ABI
+// (include/csm_c.h).  One engine = one device-resident data graph + registered
+// queries; all state lives in HBM and every launch goes to the engine's own
+// HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/csm_c.h"
+#include "csm_kernels.hpp"
+#include "csm_plan.hpp"
+#include "csm_symbi.hpp"
+
+using namespace csm;
+
+namespace {
+
+enum Algo { ALGO_GRAPHFLOW = 0, ALGO_SYMBI = 1, ALGO_TURBOFLUX = 2 };
+
+constexpr uint32_t DEFAULT_MAX_BATCH = 65534;
+constexpr int BUILD_BS = 256;
+constexpr int APPLY_BS = 256;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct QueryInfo {
+  QueryGraph g;                         // original labels
+  std::vector<uint32_t> dlab;           // dense vertex labels
+  std::vector<std::vector<uint32_t>> orders;
+  uint32_t plan_base = 0;               // first PlanRec index
+  Dag dag;
+  uint32_t d2_base = 0;                 // first d2 row
+  SymbiIndex sx;                        // device SymBi index (algo symbi)
+};
+
+inline uint32_t bits_for(uint64_t n) {  // bits to hold values in [0, n)
+  uint32_t b = 1;
+  while ((1ull << b) < n) ++b;
+  return b;
+}
+
+}  // namespace
+
+struct csm_engine {
+  int algo = ALGO_GRAPHFLOW;
+  std::string algo_name;
+  csm_opts opts{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  int search_blocks = 0;
+  std::string err;
+
+  // labels
+  std::vector<uint32_t> vl_map, el_map;  // sorted original labels; dense id = index
+  uint32_t nlab = 0, nel = 0, vbits = 0, vmask = 0;
+  uint32_t nv = 0, nv_cap = 0;
+  std::vector<uint32_t> h_vlab;          // dense labels (NO_LABEL = absent)
+  uint64_t num_edges = 0;
+
+  // device graph
+  uint2* d_vrec = nullptr;
+  uint32_t* d_vlab = nullptr;
+  uint32_t* d_adj[2] = {nullptr, nullptr};
+  uint32_t* d_ats[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint64_t pool_cap = 0;
+  Ctrl* d_ctrl = nullptr;
+  Ctrl* h_ctrl = nullptr;  // pinned
+  uint32_t* d_dirty[2] = {nullptr, nullptr};
+  int dcur = 0;            // d_dirty[dcur] = list compacted at the next batch start
+
+  // batch buffers
+  uint32_t max_batch = DEFAULT_MAX_BATCH;
+  UpdateRec* d_upd = nullptr;
+  uint8_t *d_type = nullptr, *d_add = nullptr;
+  uint32_t *d_v1 = nullptr, *d_v2 = nullptr, *d_lab = nullptr, *d_slab = nullptr;
+  uint64_t *d_keys = nullptr, *d_keys2 = nullptr;
+  uint32_t *d_vals = nullptr, *d_vals2 = nullptr, *d_runs = nullptr;
+  void* d_sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  uint32_t* d_elmap = nullptr;
+  Seed* d_seeds = nullptr;
+  uint32_t seed_cap = 0;
+  uint32_t* d_chunk_owner = nullptr;
+  uint32_t chunk_cap = 0;
+  unsigned long long* d_counts = nullptr;
+  unsigned long long* d_hashes = nullptr;
+  uint8_t* d_safe = nullptr;
+  unsigned long long* h_counts = nullptr;  // pinned
+  unsigned long long* h_hashes = nullptr;
+  uint8_t* h_safe = nullptr;
+  UpdateRec* h_upd = nullptr;              // pinned staging
+
+  // queries
+  std::vector<std::unique_ptr<QueryInfo>> queries;
+  std::vector<PlanRec> h_plans;
+  PlanRec* d_plans = nullptr;
+  size_t d_plans_cap = 0;
+  std::vector<uint32_t> h_tab_off, h_tab_ent;
+  uint32_t* d_tab_off = nullptr;
+  uint32_t* d_tab_ent = nullptr;
+  size_t d_tab_ent_cap = 0;
+  bool preprocessed = false;
+
+  // SymBi d2 bitmap rows (all queries), words per row
+  uint32_t* d_d2 = nullptr;
+  uint32_t d2_rows = 0, d2_words = 0;
+
+  // totals and stats
+  uint64_t tot_init = 0, tot_pos = 0, tot_neg = 0;
+  uint64_t stats[5] = {0, 0, 0, 0, 0};
+  uint64_t dev_bytes = 0;
+
+  std::vector<DevBuf> allocs;
+};
+
+namespace {
+
+int fail(csm_engine* e, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  return code;
+}
+
+#define HIP_TRY(e, call)                                                               \
+  do {                                                                                 \
+    hipError_t _st = (call);                                                           \
+    if (_st != hipSuccess)                                                             \
+      return fail((e), CSM_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(_st));  \
+  } while (0)
+
+template <class T>
+int dalloc(csm_engine* e, T** p, size_t count) {
+  size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+  if (e->opts.device_budget && e->dev_bytes + bytes > e->opts.device_budget)
+    return fail(e, CSM_E_ADMISSION, "device budget of %llu bytes exceeded (%llu + %zu)",
+                (unsigned long long)e->opts.device_budget, (unsigned long long)e->dev_bytes, bytes);
+  void* q = nullptr;
+  hipError_t st = hipMalloc(&q, bytes);
+  if (st != hipSuccess)
+    return fail(e, CSM_E_DEVICE, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st));
+  e->allocs.push_back({q, bytes});
+  e->dev_bytes += bytes;
+  *p = static_cast<T*>(q);
+  return CSM_OK;
+}
+
+template <class T>
+void dfree(csm_engine* e, T*& p) {
+  if (!p) return;
+  for (size_t i = 0; i < e->allocs.size(); ++i)
+    if (e->allocs[i].p == p) {
+      e->dev_bytes -= e->allocs[i].bytes;
+      e->allocs.erase(e->allocs.begin() + i);
+      break;
+    }
+  (void)hipFree(p);
+  p = nullptr;
+}
+
+inline int grid_for(uint64_t n, int bs, int cap = 4096) {
+  uint64_t g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  return (int)std::min<uint64_t>(g, (uint64_t)cap);
+}
+
+inline uint32_t dense_of(const std::vector<uint32_t>& map, uint32_t x) {
+  auto it = std::lower_bound(map.begin(), map.end(), x);
+  return (it != map.end() && *it == x) ? (uint32_t)(it - map.begin()) : NO_LABEL;
+}
+
+GraphView view(const csm_engine* e) {
+  GraphView g;
+  g.vrec = e->d_vrec;
+  g.vlab = e->d_vlab;
+  g.adj = e->d_adj[e->cur];
+  g.ats = e->d_ats[e->cur];
+  g.vbits = e->vbits;
+  g.vmask = e->vmask;
+  g.nel = e->nel;
+  g.nv = e->nv;
+  return g;
+}
+
+int parse_algo(const char* a, int* out) {
+  std::string s(a ? a : "");
+  if (s == "parallel_graphflow" || s == "graphflow") *out = ALGO_GRAPHFLOW;
+  else if (s == "parallel_symbi" || s == "symbi") *out = ALGO_SYMBI;
+  else if (s == "parallel_turboflux" || s == "turboflux") *out = ALGO_TURBOFLUX;
+  else return CSM_E_UNKNOWN_ALGO;
+  return CSM_OK;
+}
+
+// -------------------------------------------------------------- planning
+int build_query(csm_engine* e, const csm_graph* q, std::unique_ptr<QueryInfo>& out) {
+  if (!q || q->num_vertices == 0) return fail(e, CSM_E_ARG, "empty query graph");
+  if (q->num_vertices > (uint32_t)MAXK)
+    return fail(e, CSM_E_CAPACITY, "query has %u vertices; the search supports up to %d",
+                q->num_vertices, MAXK);
+  auto qi = std::make_unique<QueryInfo>();
+  qi->g.init(q->num_vertices, q->vlabels);
+  for (size_t i = 0; i < q->num_edges; ++i) qi->g.add_edge(q->src[i], q->dst[i], q->elabels[i]);
+  if (!qi->g.connected()) return fail(e, CSM_E_ARG, "query graph is not connected");
+  out = std::move(qi);
+  return CSM_OK;
+}
+
+// Dense label maps must cover the query's labels; data labels are fixed at
+// create time, so a query label absent from the data can never match: it is
+// mapped to a label no data vertex carries.
+int plan_query(csm_engine* e, QueryInfo& qi, uint32_t qid) {
+  const QueryGraph& g = qi.g;
+  qi.dlab.resize(g.k);
+  for (uint32_t u = 0; u < g.k; ++u) {
+    uint32_t d = dense_of(e->vl_map, g.vl[u]);
+    qi.dlab[u] = d;
+  }
+  qi.orders = graphflow_orders(g);
+  qi.plan_base = (uint32_t)e->h_plans.size();
+  for (size_t oi = 0; oi < qi.orders.size(); ++oi) {
+    const auto& o = qi.orders[oi];
+    PlanRec P;
+    std::memset(&P, 0, sizeof P);
+    P.k = g.k;
+    P.q = qid;
+    P.symbi = e->algo == ALGO_SYMBI ? 1u : 0u;
+    std::vector<uint32_t> posof(g.k, 0);
+    for (uint32_t p = 0; p < g.k; ++p) posof[o[p]] = p;
+    bool matchable = true;
+    for (uint32_t p = 0; p < g.k; ++p) {
+      uint32_t u = o[p];
+      P.qv[p] = u;
+      P.d2row[p] = qi.d2_base + u;
+      uint32_t nb = 0;
+      for (size_t j = 0; j < g.nb[u].size(); ++j) {
+        uint32_t w = g.nb[u][j];
+        if (posof[w] >= p) continue;
+        uint32_t el = dense_of(e->el_map, g.el[u][j]);
+        if (qi.dlab[u] == NO_LABEL || el == NO_LABEL) { matchable = false; el = 0; }
+        uint32_t key = (qi.dlab[u] == NO_LABEL ? 0 : qi.dlab[u]) * e->nel + el;
+        P.bw[p][nb++] = posof[w] | (key << 8);
+      }
+      P.nb[p] = nb;
+    }
+    if (!matchable) P.k = 0;  // never seeded (see the seed table below)
+    e->h_plans.push_back(P);
+  }
+  return CSM_OK;
+}
+
+// Seed table: bucket (L(v1), L(v2), l) -> plans whose first query edge
+// matches in that orientation (the per-edge checks of graphflow.cpp:300-352).
+void build_seed_table(csm_engine* e) {
+  const uint32_t nb = e->nlab * e->nlab * e->nel;
+  std::vector<std::vector<uint32_t>> buckets(nb);
+  for (auto& qp : e->queries) {
+    QueryInfo& qi = *qp;
+    for (size_t oi = 0; oi < qi.orders.size(); ++oi) {
+      uint32_t qo = qi.plan_base + (uint32_t)oi;
+      if (e->h_plans[qo].k == 0) continue;
+      uint32_t ua = qi.orders[oi][0], ub = qi.orders[oi][1];
+      uint32_t la = qi.dlab[ua], lb = qi.dlab[ub];
+      uint32_t el = dense_of(e->el_map, qi.g.elabel(ua, ub));
+      if (la == NO_LABEL || lb == NO_LABEL || el == NO_LABEL) continue;
+      buckets[(la * e->nlab + lb) * e->nel + el].push_back(qo << 1);
+      buckets[(lb * e->nlab + la) * e->nel + el].push_back((qo << 1) | 1u);
+    }
+  }
+  e->h_tab_off.assign(nb + 1, 0);
+  e->h_tab_ent.clear();
+  for (uint32_t b = 0; b < nb; ++b) {
+    e->h_tab_off[b] = (uint32_t)e->h_tab_ent.size();
+    e->h_tab_ent.insert(e->h_tab_ent.end(), buckets[b].begin(), buckets[b].end());
+  }
+  e->h_tab_off[nb] = (uint32_t)e->h_tab_ent.size();
+}
+
+int upload_plans(csm_engine* e) {
+  if (e->h_plans.size() > e->d_plans_cap) {
+    dfree(e, e->d_plans);
+    size_t cap = std::max<size_t>(64, e->h_plans.size() * 2);
+    int rc = dalloc(e, &e->d_plans, cap);
+    if (rc) return rc;
+    e->d_plans_cap = cap;
+  }
+  if (!e->h_plans.empty())
+    HIP_TRY(e, hipMemcpyAsync(e->d_plans, e->h_plans.data(), e->h_plans.size() * sizeof(PlanRec),
+                              hipMemcpyHostToDevice, e->stream));
+  if (!e->d_tab_off) {
+    int rc = dalloc(e, &e->d_tab_off, e->h_tab_off.size());
+    if (rc) return rc;
+  }
+  if (e->h_tab_ent.size() > e->d_tab_ent_cap) {
+    dfree(e, e->d_tab_ent);
+    size_t cap = std::max<size_t>(256, e->h_tab_ent.size() * 2);
+    int rc = dalloc(e, &e->d_tab_ent, cap);
+    if (rc) return rc;
+    e->d_tab_ent_cap = cap;
+  }
+  HIP_TRY(e, hipMemcpyAsync(e->d_tab_off, e->h_tab_off.data(), e->h_tab_off.size() * 4,
+                            hipMemcpyHostToDevice, e->stream));
+  if (!e->h_tab_ent.empty())
+    HIP_TRY(e, hipMemcpyAsync(e->d_tab_ent, e->h_tab_ent.data(), e->h_tab_ent.size() * 4,
+                              hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
+// --------------------------------------------------------------- sorting
+int sort_keys(csm_engine* e, uint64_t* in, uint64_t* out, uint32_t* vin, uint32_t* vout,
+              size_t n) {
+  size_t need = 0;
+  hipError_t st;
+  if (vin)
+    st = rocprim::radix_sort_pairs(nullptr, need, in, out, vin, vout, n, 0, 64, e->stream);
+  else
+    st = rocprim::radix_sort_keys(nullptr, need, in, out, n, 0, 64, e->stream);
+  if (st != hipSuccess) return fail(e, CSM_E_DEVICE, "radix sort sizing failed");
+  if (need > e->sort_tmp_bytes) {
+    if (e->d_sort_tmp) {
+      HIP_TRY(e, hipStreamSynchronize(e->stream));
+      dfree(e, e->d_sort_tmp);
+    }
+    uint8_t* p = nullptr;
+    int rc = dalloc(e, &p, need);
+    if (rc) return rc;
+    e->d_sort_tmp = p;
+    e->sort_tmp_bytes = need;
+  }
+  if (vin)
+    st = rocprim::radix_sort_pairs(e->d_sort_tmp, need, in, out, vin, vout, n, 0, 64, e->stream);
+  else
+    st = rocprim::radix_sort_keys(e->d_sort_tmp, need, in, out, n, 0, 64, e->stream);
+  if (st != hipSuccess) return fail(e, CSM_E_DEVICE, "radix sort failed");
+  return CSM_OK;
+}
+
+// ------------------------------------------------------------ graph build
+int build_graph(csm_engine* e, const csm_graph* d) {
+  const uint64_t m = d->num_edges;
+  const uint64_t n2 = 2 * m;
+  uint32_t *d_src = nullptr, *d_dst = nullptr, *d_el = nullptr;
+  uint64_t *k1 = nullptr, *k2 = nullptr;
+  uint32_t *v1 = nullptr, *v2 = nullptr, *vend = nullptr;
+  int rc;
+  if ((rc = dalloc(e, &d_src, m)) || (rc = dalloc(e, &d_dst, m)) || (rc = dalloc(e, &d_el, m)) ||
+      (rc = dalloc(e, &k1, n2)) || (rc = dalloc(e, &k2, n2)) || (rc = dalloc(e, &v1, n2)) ||
+      (rc = dalloc(e, &v2, n2)) || (rc = dalloc(e, &vend, e->nv_cap)))
+    return rc;
+  std::vector<uint32_t> del(m);
+  for (uint64_t i = 0; i < m; ++i) {
+    if (d->src[i] >= e->nv || d->dst[i] >= e->nv)
+      return fail(e, CSM_E_ARG, "edge %llu references a vertex >= %u", (unsigned long long)i, e->nv);
+    del[i] = dense_of(e->el_map, d->elabels[i]);
+  }
+  if (m) {
+    HIP_TRY(e, hipMemcpyAsync(d_src, d->src, m * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(d_dst, d->dst, m * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(d_el, del.data(), m * 4, hipMemcpyHostToDevice, e->stream));
+    k_build_keys<<<grid_for(m, BUILD_BS), BUILD_BS, 0, e->stream>>>(d_src, d_dst, m, k1, v1);
+    if ((rc = sort_keys(e, k1, k2, v1, v2, n2))) return rc;
+    k_build_pack<<<grid_for(n2, BUILD_BS), BUILD_BS, 0, e->stream>>>(k2, v2, n2, e->d_vlab, d_el,
+                                                                    e->nel, e->vbits, k1);
+    if ((rc = sort_keys(e, k1, k2, nullptr, nullptr, n2))) return rc;
+  }
+  HIP_TRY(e, hipMemsetAsync(e->d_vrec, 0, (size_t)e->nv_cap * sizeof(uint2), e->stream));
+  HIP_TRY(e, hipMemsetAsync(vend, 0, (size_t)e->nv_cap * 4, e->stream));
+  if (m) {
+    k_build_lists<<<grid_for(n2, BUILD_BS), BUILD_BS, 0, e->stream>>>(
+        k2, n2, e->d_adj[0], e->d_ats[0], e->d_vrec, vend);
+    k_build_lens<<<grid_for(e->nv_cap, BUILD_BS), BUILD_BS, 0, e->stream>>>(e->d_vrec, vend,
+                                                                          e->nv_cap);
+  }
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  // live entries = non-sentinel keys; count on the host from the list lengths
+  std::vector<uint2> rec(e->nv_cap);
+  HIP_TRY(e, hipMemcpy(rec.data(), e->d_vrec, rec.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+  uint64_t live = 0;
+  for (auto& r : rec) live += r.y & LEN_MASK;
+  e->num_edges = live / 2;
+  std::memset(e->h_ctrl, 0, sizeof(Ctrl));
+  e->h_ctrl->pool_top = (uint32_t)n2;
+  e->h_ctrl->live = live;
+  e->h_ctrl->err_at = 0xFFFFFFFFu;
+  HIP_TRY(e, hipMemcpy(e->d_ctrl, e->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice));
+  dfree(e, d_src); dfree(e, d_dst); dfree(e, d_el);
+  dfree(e, k1); dfree(e, k2); dfree(e, v1); dfree(e, v2); dfree(e, vend);
+  return CSM_OK;
+}
+
+// Repack every list into the other pool (drops garbage left by out-of-place
+// rewrites).  Lists keep their versions and DIRTY flags.
+int repack(csm_engine* e) {
+  int other = 1 - e->cur;
+  if (!e->d_adj[other]) {
+    int rc;
+    if ((rc = dalloc(e, &e->d_adj[other], e->pool_cap)) ||
+        (rc = dalloc(e, &e->d_ats[other], e->pool_cap)))
+      return rc;
+  }
+  uint32_t *lens = nullptr, *offs = nullptr;
+  int rc;
+  if ((rc = dalloc(e, &lens, e->nv_cap)) || (rc = dalloc(e, &offs, e->nv_cap))) return rc;
+  k_repack_lens<<<grid_for(e->nv_cap, 256), 256, 0, e->stream>>>(e->d_vrec, lens, e->nv_cap);
+  size_t need = 0;
+  HIP_TRY(e, rocprim::exclusive_scan(nullptr, need, lens, offs, 0u, e->nv_cap,
+                                     rocprim::plus<uint32_t>(), e->stream));
+  uint8_t* tmp = nullptr;
+  if ((rc = dalloc(e, &tmp, need))) return rc;
+  HIP_TRY(e, rocprim::exclusive_scan(tmp, need, lens, offs, 0u, e->nv_cap,
+                                     rocprim::plus<uint32_t>(), e->stream));
+  k_repack_copy<256><<<grid_for(e->nv_cap, 1, 8192), 256, 0, e->stream>>>(
+      e->d_vrec, offs, e->d_adj[e->cur], e->d_ats[e->cur], e->d_adj[other], e->d_ats[other],
+      e->nv_cap);
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  HIP_TRY(e, hipMemcpy(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  e->h_ctrl->pool_top = (uint32_t)e->h_ctrl->live;
+  HIP_TRY(e, hipMemcpy(e->d_ctrl, e->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice));
+  e->cur = other;
+  dfree(e, lens); dfree(e, offs); dfree(e, tmp);
+  ++e->stats[4];
+  return CSM_OK;
+}
+
+int alloc_batch(csm_engine* e) {
+  const uint32_t B = e->max_batch;
+  int rc;
+  if ((rc = dalloc(e, &e->d_upd, B)) || (rc = dalloc(e, &e->d_type, B)) ||
+      (rc = dalloc(e, &e->d_add, B)) || (rc = dalloc(e, &e->d_v1, B)) ||
+      (rc = dalloc(e, &e->d_v2, B)) || (rc = dalloc(e, &e->d_lab, B)) ||
+      (rc = dalloc(e, &e->d_slab, B)) || (rc = dalloc(e, &e->d_keys, 2 * (size_t)B)) ||
+      (rc = dalloc(e, &e->d_keys2, 2 * (size_t)B)) || (rc = dalloc(e, &e->d_vals, 2 * (size_t)B)) ||
+      (rc = dalloc(e, &e->d_vals2, 2 * (size_t)B)) || (rc = dalloc(e, &e->d_runs, 2 * (size_t)B)) ||
+      (rc = dalloc(e, &e->d_dirty[0], 2 * (size_t)B + 2)) ||
+      (rc = dalloc(e, &e->d_dirty[1], 2 * (size_t)B + 2)) || (rc = dalloc(e, &e->d_safe, B)))
+    return rc;
+  e->seed_cap = std::max<uint32_t>(1u << 16, std::min<uint64_t>(16ull * B * 8, 1u << 24));
+  e->chunk_cap = 1u << 26;
+  if ((rc = dalloc(e, &e->d_seeds, e->seed_cap)) ||
+      (rc = dalloc(e, &e->d_chunk_owner, e->chunk_cap)))
+    return rc;
+  HIP_TRY(e, hipHostMalloc((void**)&e->h_upd, (size_t)B * sizeof(UpdateRec)));
+  HIP_TRY(e, hipHostMalloc((void**)&e->h_safe, B));
+  return CSM_OK;
+}
+
+int ensure_results(csm_engine* e) {
+  size_t need = (size_t)e->max_batch * std::max<size_t>(1, e->queries.size());
+  static_assert(sizeof(unsigned long long) == 8, "u64");
+  if (e->d_counts) {
+    // capacity tracked via allocation size
+    for (auto& a : e->allocs)
+      if (a.p == e->d_counts && a.bytes >= need * 8) return CSM_OK;
+    dfree(e, e->d_counts);
+    dfree(e, e->d_hashes);
+    if (e->h_counts) (void)hipHostFree(e->h_counts);
+    if (e->h_hashes) (void)hipHostFree(e->h_hashes);
+    e->h_counts = e->h_hashes = nullptr;
+  }
+  int rc;
+  if ((rc = dalloc(e, &e->d_counts, need)) || (rc = dalloc(e, &e->d_hashes, need))) return rc;
+  HIP_TRY(e, hipHostMalloc((void**)&e->h_counts, need * 8));
+  HIP_TRY(e, hipHostMalloc((void**)&e->h_hashes, need * 8));
+  return CSM_OK;
+}
+
+int read_ctrl(csm_engine* e) {
+  HIP_TRY(e, hipMemcpyAsync(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
+BatchView batch_view(const csm_engine* e, uint32_t n) {
+  BatchView b;
+  b.type = e->d_type;
+  b.add = e->d_add;
+  b.v1 = e->d_v1;
+  b.v2 = e->d_v2;
+  b.label = e->d_lab;
+  b.n = n;
+  return b;
+}
+
+template <bool V, bool H, bool S>
+void launch_search(csm_engine* e, const SearchArgs& sa) {
+  k_search<V, H, S><<<e->search_blocks, SEARCH_WAVES * WAVE, 0, e->stream>>>(sa);
+}
+
+void search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned long long* hashes) {
+  SearchArgs sa;
+  sa.g = view(e);
+  sa.plans = e->d_plans;
+  sa.seeds = e->d_seeds;
+  sa.chunk_owner = e->d_chunk_owner;
+  sa.ctrl = e->d_ctrl;
+  sa.counts = counts;
+  sa.hashes = hashes;
+  sa.d2 = e->d_d2;
+  sa.d2_words = e->d2_words;
+  sa.nq = (uint32_t)e->queries.size();
+  sa.chunk = e->opts.split_chunk;
+  sa.homo = e->opts.homo;
+  sa.pos_offset = 0;
+  const bool h = hashes != nullptr;
+  const bool s = e->algo == ALGO_SYMBI && e->d_d2 != nullptr;
+  if (versioned) {
+    if (h) { if (s) launch_search<true, true, true>(e, sa); else launch_search<true, true, false>(e, sa); }
+    else { if (s) launch_search<true, false, true>(e, sa); else launch_search<true, false, false>(e, sa); }
+  } else {
+    if (h) { if (s) launch_search<false, true, true>(e, sa); else launch_search<false, true, false>(e, sa); }
+    else { if (s) launch_search<false, false, true>(e, sa); else launch_search<false, false, false>(e, sa); }
+  }
+}
+
+void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
+           uint8_t* safe, bool versioned) {
+  SeedArgs a;
+  a.g = view(e);
+  a.b = batch_view(e, n);
+  a.seed_label = e->d_slab;
+  a.tab_off = e->d_tab_off;
+  a.tab_ent = e->d_tab_ent;
+  a.plans = e->d_plans;
+  a.d2 = e->algo == ALGO_SYMBI ? e->d_d2 : nullptr;
+  a.d2_words = e->d2_words;
+  a.nlab = e->nlab;
+  a.nq = (uint32_t)e->queries.size();
+  a.chunk = e->opts.split_chunk;
+  a.versioned = versioned;
+  a.homo = e->opts.homo;
+  a.want_hash = hashes != nullptr;
+  a.seeds = e->d_seeds;
+  a.seed_cap = e->seed_cap;
+  a.chunk_owner = e->d_chunk_owner;
+  a.chunk_cap = e->chunk_cap;
+  a.ctrl = e->d_ctrl;
+  a.counts = counts;
+  a.hashes = hashes;
+  a.safe = safe;
+  k_seeds<<<grid_for(n, 256), 256, 0, e->stream>>>(a);
+}
+
+int check_errors(csm_engine* e) {
+  uint32_t err = e->h_ctrl->err;
+  if (!err) return CSM_OK;
+  if (err & ERR_DELETE_MISSING)
+    return fail(e, CSM_E_DELETE_MISSING, "deletion error (update %u)", e->h_ctrl->err_at);
+  if (err & ERR_BAD_UPDATE)
+    return fail(e, CSM_E_ARG, "invalid update (unknown vertex or label, or self-loop) at %u",
+                e->h_ctrl->err_at);
+  if (err & (ERR_SEED_OVERFLOW | ERR_CHUNK_OVERFLOW))
+    return fail(e, CSM_E_CAPACITY, "search task capacity exceeded; use smaller batches");
+  if (err & ERR_POOL_OVERFLOW) return fail(e, CSM_E_CAPACITY, "adjacency pool exhausted");
+  return fail(e, CSM_E_DEVICE, "device error bits 0x%x", err);
+}
+
+int symbi_prepare_query(csm_engine* e, QueryInfo& q);
+int symbi_batch_insert(csm_engine* e, uint32_t n);
+int symbi_batch_delete(csm_engine* e, uint32_t n);
+int symbi_add_vertex(csm_engine* e, uint32_t id);
+
+// The batch pipeline; updates already unpacked into d_type/d_add/d_v1/d_v2/d_lab.
+int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
+              uint8_t* safe) {
+  int rc;
+  // room for this batch's rewrites: at most every list twice plus the new entries
+  uint64_t live = e->h_ctrl->live;
+  if ((uint64_t)e->h_ctrl->pool_top + 2 * live + 2ull * n + 1024 > e->pool_cap) {
+    if ((rc = repack(e))) return rc;
+    if ((uint64_t)e->h_ctrl->pool_top + 2 * live + 2ull * n + 1024 > e->pool_cap)
+      return fail(e, CSM_E_CAPACITY, "adjacency pool too small");
+  }
+  const size_t nq = e->queries.size();
+  HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
+  if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
+  k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+  // 1. compact lists dirtied by the previous batch
+  {
+    GraphView g = view(e);
+    k_compact<APPLY_BS><<<2048, APPLY_BS, 0, e->stream>>>(
+        e->d_vrec, e->d_adj[e->cur], e->d_ats[e->cur], e->d_dirty[e->dcur], &e->d_ctrl->ndirty_cur,
+        e->d_ctrl, (uint32_t)e->pool_cap);
+    (void)g;
+  }
+  // 2. merge insertions
+  BatchView b = batch_view(e, n);
+  k_batch_keys<<<grid_for(n, 256), 256, 0, e->stream>>>(b, e->d_vlab, e->nel, e->vbits, e->nv,
+                                                        e->d_keys, e->d_vals, &e->d_ctrl->err,
+                                                        &e->d_ctrl->err_at);
+  if ((rc = sort_keys(e, e->d_keys, e->d_keys2, e->d_vals, e->d_vals2, 2 * (size_t)n))) return rc;
+  k_runs<<<grid_for(2 * n, 256), 256, 0, e->stream>>>(e->d_keys2, 2 * n, e->d_runs,
+                                                      &e->d_ctrl->nruns);
+  k_merge<APPLY_BS><<<2048, APPLY_BS, 0, e->stream>>>(
+      e->d_vrec, e->d_adj[e->cur], e->d_ats[e->cur], e->d_keys2, e->d_vals2, 2 * n, e->d_runs,
+      e->d_ctrl, e->d_dirty[1 - e->dcur], (uint32_t)e->pool_cap);
+  // 3. deletions and duplicate insertions
+  GraphView g = view(e);
+  k_delete_mark<<<grid_for(n, 256), 256, 0, e->stream>>>(b, g, e->d_ats[e->cur], e->d_vrec,
+                                                         e->d_slab, e->d_ctrl,
+                                                         e->d_dirty[1 - e->dcur]);
+  k_dedup<<<grid_for(n, 256), 256, 0, e->stream>>>(b, g, e->d_ats[e->cur]);
+  // 4. SymBi index: insertions (before the search) ...
+  if (e->algo == ALGO_SYMBI && e->d_d2)
+    if ((rc = symbi_batch_insert(e, n))) return rc;
+  // 5. seeds + search
+  seeds(e, n, counts, hashes, safe, true);
+  search(e, true, counts, hashes);
+  // 6. ... and deletions (after it)
+  if (e->algo == ALGO_SYMBI && e->d_d2)
+    if ((rc = symbi_batch_delete(e, n))) return rc;
+  // next batch compacts what this one dirtied
+  k_batch_end<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+  e->dcur = 1 - e->dcur;
+  ++e->stats[3];
+  return CSM_OK;
+}
+
+void accumulate_stats(csm_engine* e) {
+  e->stats[0] = e->h_ctrl->stat_tasks;
+  e->stats[1] = e->h_ctrl->stat_cands;
+  e->stats[2] = e->h_ctrl->stat_bytes;
+}
+
+// SymBi index maintenance (device).  Until the index is built the SymBi
+// plugin searches exactly like GraphFlow (same counts, no d2 pruning).
+int symbi_prepare_query(csm_engine* e, QueryInfo& q) { (void)e; (void)q; return CSM_OK; }
+int symbi_batch_insert(csm_engine* e, uint32_t n) { (void)e; (void)n; return CSM_OK; }
+int symbi_batch_delete(csm_engine* e, uint32_t n) { (void)e; (void)n; return CSM_OK; }
+int symbi_add_vertex(csm_engine* e, uint32_t id) { (void)e; (void)id; return CSM_OK; }
+
+// InitialMatching on the device: seeds are all directed data edges matching
+// the first edge of order 0, searched from depth 2 (graphflow.cpp:268-283
+// enumerates from single vertices; both count every embedding once).
+int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
+  const PlanRec& P = e->h_plans[q.plan_base];
+  if (P.k == 0) { *total = 0; return CSM_OK; }
+  uint32_t ua = q.orders[0][0], ub = q.orders[0][1];
+  uint32_t la = q.dlab[ua];
+  uint32_t key01 = (q.dlab[ub]) * e->nel + dense_of(e->el_map, q.g.elabel(ua, ub));
+  const size_t nq = e->queries.size();
+  HIP_TRY(e, hipMemsetAsync(e->d_counts, 0, nq * 8, e->stream));
+  uint32_t v0 = 0, step = e->nv;
+  while (v0 < e->nv) {
+    uint32_t v1 = (uint32_t)std::min<uint64_t>((uint64_t)v0 + step, e->nv);
+    k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+    GraphView g = view(e);
+    k_initial_seeds<<<grid_for(v1 - v0, 256), 256, 0, e->stream>>>(
+        g, e->d_plans, q.plan_base, la, key01, v0, v1, e->opts.split_chunk, e->d_seeds,
+        e->seed_cap, e->d_chunk_owner, e->chunk_cap, e->d_ctrl, e->d_d2, e->d2_words,
+        (e->algo == ALGO_SYMBI && e->d_d2) ? 1u : 0u, e->d_counts);
+    int rc = read_ctrl(e);
+    if (rc) return rc;
+    if (e->h_ctrl->err & (ERR_SEED_OVERFLOW | ERR_CHUNK_OVERFLOW)) {
+      if (step == 1) return fail(e, CSM_E_CAPACITY, "initial matching: one vertex overflows the task buffers");
+      step = std::max<uint32_t>(1, step / 4);
+      continue;
+    }
+    // search with the end-of-batch view (position 0xFFFF)
+    SearchArgs sa;
+    sa.g = g;
+    sa.plans = e->d_plans;
+    sa.seeds = e->d_seeds;
+    sa.chunk_owner = e->d_chunk_owner;
+    sa.ctrl = e->d_ctrl;
+    sa.counts = e->d_counts;
+    sa.hashes = nullptr;
+    sa.d2 = e->d_d2;
+    sa.d2_words = e->d2_words;
+    sa.nq = (uint32_t)nq;
+    sa.chunk = e->opts.split_chunk;
+    sa.homo = e->opts.homo;
+    sa.pos_offset = 0xFFFEu;
+    if (e->algo == ALGO_SYMBI && e->d_d2) launch_search<true, false, true>(e, sa);
+    else launch_search<true, false, false>(e, sa);
+    if ((rc = read_ctrl(e))) return rc;
+    v0 = v1;
+  }
+  unsigned long long c = 0;
+  HIP_TRY(e, hipMemcpy(&c, e->d_counts, 8, hipMemcpyDeviceToHost));  // update 0, query 0
+  *total = c;
+  return CSM_OK;
+}
+
+}  // namespace
+
+// ================================================================== C-ABI
+extern "C" {
+
+csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph* data,
+                       const csm_opts* opts, int* err) {
+  auto set = [&](int c) { if (err) *err = c; };
+  int a;
+  if (parse_algo(algo, &a)) { set(CSM_E_UNKNOWN_ALGO); return nullptr; }
+  if (!query || !data) { set(CSM_E_ARG); return nullptr; }
+  std::unique_ptr<csm_engine> e(new csm_engine());
+  e->algo = a;
+  e->algo_name = algo;
+  if (opts) e->opts = *opts;
+  if (e->opts.max_batch == 0 || e->opts.max_batch > DEFAULT_MAX_BATCH) e->opts.max_batch = DEFAULT_MAX_BATCH;
+  if (e->opts.split_chunk == 0) e->opts.split_chunk = 64;
+  e->max_batch = e->opts.max_batch;
+  e->dev = (int)e->opts.device;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= e->dev) { set(CSM_E_DEVICE); return nullptr; }
+  if (hipSetDevice(e->dev) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, e->dev) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  e->num_cus = prop.multiProcessorCount;
+  if (e->opts.device_budget == 0) e->opts.device_budget = (uint64_t)(prop.totalGlobalMem * 0.9);
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search<true, false, false>,
+                                                   SEARCH_WAVES * WAVE, 0) != hipSuccess || per_cu < 1)
+    per_cu = 2;
+  e->search_blocks = e->num_cus * per_cu;
+
+  // label maps from data and query labels
+  std::vector<uint32_t> vls(data->vlabels, data->vlabels + data->num_vertices);
+  std::vector<uint32_t> els(data->elabels, data->elabels + data->num_edges);
+  els.insert(els.end(), query->elabels, query->elabels + query->num_edges);
+  std::sort(vls.begin(), vls.end());
+  vls.erase(std::unique(vls.begin(), vls.end()), vls.end());
+  vls.erase(std::remove(vls.begin(), vls.end(), NO_LABEL), vls.end());
+  std::sort(els.begin(), els.end());
+  els.erase(std::unique(els.begin(), els.end()), els.end());
+  if (els.empty()) els.push_back(0);
+  if (vls.empty()) vls.push_back(0);
+  e->vl_map = vls;
+  e->el_map = els;
+  e->nlab = (uint32_t)vls.size();
+  e->nel = (uint32_t)els.size();
+  e->nv = data->num_vertices;
+  uint64_t keys = (uint64_t)e->nlab * e->nel;
+  uint32_t kbits = bits_for(keys);
+  uint64_t want_cap = (uint64_t)e->nv + e->nv / 8 + 1024;
+  uint32_t vb = bits_for(want_cap);
+  if (vb + kbits > 32) vb = bits_for(std::max<uint64_t>(e->nv, 2));
+  if (vb + kbits > 32) {
+    set(CSM_E_CAPACITY);
+    return nullptr;
+  }
+  e->vbits = vb;
+  e->vmask = vb >= 32 ? 0xFFFFFFFFu : ((1u << vb) - 1u);
+  e->nv_cap = (uint32_t)std::min<uint64_t>(want_cap, 1ull << vb);
+  if (e->nv_cap < e->nv) e->nv_cap = e->nv;
+  e->h_vlab.assign(e->nv_cap, NO_LABEL);
+  for (uint32_t v = 0; v < e->nv; ++v) e->h_vlab[v] = dense_of(e->vl_map, data->vlabels[v]);
+
+  int rc = CSM_OK;
+  e->pool_cap = std::max<uint64_t>(4 * 2 * data->num_edges + 8ull * e->max_batch, 1u << 20);
+  if (e->pool_cap > 0xFFFFFFF0ull) e->pool_cap = 0xFFFFFFF0ull;
+  if ((rc = dalloc(e.get(), &e->d_vrec, e->nv_cap)) || (rc = dalloc(e.get(), &e->d_vlab, e->nv_cap)) ||
+      (rc = dalloc(e.get(), &e->d_adj[0], e->pool_cap)) ||
+      (rc = dalloc(e.get(), &e->d_ats[0], e->pool_cap)) || (rc = dalloc(e.get(), &e->d_ctrl, 1)) ||
+      (rc = dalloc(e.get(), &e->d_elmap, e->el_map.size()))) {
+    set(rc);
+    return nullptr;
+  }
+  if (hipHostMalloc((void**)&e->h_ctrl, sizeof(Ctrl)) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  if (hipMemcpy(e->d_vlab, e->h_vlab.data(), (size_t)e->nv_cap * 4, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(e->d_elmap, e->el_map.data(), e->el_map.size() * 4, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    set(CSM_E_DEVICE);
+    return nullptr;
+  }
+  if ((rc = build_graph(e.get(), data)) || (rc = alloc_batch(e.get()))) { set(rc); return nullptr; }
+  std::unique_ptr<QueryInfo> qi;
+  if ((rc = build_query(e.get(), query, qi))) { set(rc); return nullptr; }
+  e->queries.push_back(std::move(qi));
+  if ((rc = ensure_results(e.get()))) { set(rc); return nullptr; }
+  set(CSM_OK);
+  return e.release();
+}
+
+int csm_register_query(csm_engine* e, const csm_graph* query, uint32_t* qid) {
+  if (!e) return CSM_E_ARG;
+  std::unique_ptr<QueryInfo> qi;
+  int rc = build_query(e, query, qi);
+  if (rc) return rc;
+  uint32_t id = (uint32_t)e->queries.size();
+  e->queries.push_back(std::move(qi));
+  if (qid) *qid = id;
+  if ((rc = ensure_results(e))) return rc;
+  if (e->preprocessed) {
+    QueryInfo& q = *e->queries.back();
+    if (e->algo == ALGO_SYMBI) {
+      q.d2_base = e->d2_rows;
+      if ((rc = symbi_prepare_query(e, q))) return rc;
+    }
+    if ((rc = plan_query(e, q, id))) return rc;
+    build_seed_table(e);
+    if ((rc = upload_plans(e))) return rc;
+  }
+  return CSM_OK;
+}
+
+uint32_t csm_num_queries(const csm_engine* e) { return e ? (uint32_t)e->queries.size() : 0; }
+
+int csm_preprocess(csm_engine* e) {
+  if (!e) return CSM_E_ARG;
+  if (e->preprocessed) return CSM_OK;
+  int rc;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  e->h_plans.clear();
+  for (size_t i = 0; i < e->queries.size(); ++i) {
+    QueryInfo& q = *e->queries[i];
+    if (e->algo == ALGO_SYMBI) {
+      q.d2_base = e->d2_rows;
+      if ((rc = symbi_prepare_query(e, q))) return rc;
+    }
+    if ((rc = plan_query(e, q, (uint32_t)i))) return rc;
+  }
+  build_seed_table(e);
+  if ((rc = upload_plans(e))) return rc;
+  e->preprocessed = true;
+  if (e->opts.print_prep) {
+    QueryInfo& q = *e->queries[0];
+    printf("matching order: \n-vertex(backward neighbors)-\n");
+    for (size_t i = 0; i < q.orders.size(); ++i) {
+      printf("#%zu: ", i);
+      for (uint32_t p = 0; p < q.g.k; ++p) printf("%u%s", q.orders[i][p], p + 1 < q.g.k ? "-" : "");
+      printf("\n");
+    }
+    printf("\n");
+  }
+  return CSM_OK;
+}
+
+int csm_initial(csm_engine* e, uint64_t* num_results) {
+  if (!e || !num_results) return CSM_E_ARG;
+  int rc = csm_preprocess(e);
+  if (rc) return rc;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  QueryInfo& q = *e->queries[0];
+  uint64_t total = 0;
+  if (q.g.ne == 0) {
+    for (uint32_t v = 0; v < e->nv; ++v) total += (e->h_vlab[v] != NO_LABEL && e->h_vlab[v] == q.dlab[0]);
+    *num_results = total;
+    e->tot_init = total;
+    return CSM_OK;
+  }
+  rc = initial_matching(e, q, &total);
+  if (rc) return rc;
+  *num_results = total;
+  e->tot_init = total;
+  return CSM_OK;
+}
+
+int csm_classify(csm_engine* e, uint32_t v1, uint32_t v2, uint32_t label, int* safe) {
+  if (!e || !safe) return CSM_E_ARG;
+  if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
+  if (v1 >= e->nv || v2 >= e->nv) return fail(e, CSM_E_ARG, "vertex out of range");
+  uint32_t l1 = e->h_vlab[v1], l2 = e->h_vlab[v2], el = dense_of(e->el_map, label);
+  if (l1 == NO_LABEL || l2 == NO_LABEL || el == NO_LABEL) { *safe = 1; return CSM_OK; }
+  uint32_t bucket = (l1 * e->nlab + l2) * e->nel + el;
+  *safe = e->h_tab_off[bucket] == e->h_tab_off[bucket + 1];
+  return CSM_OK;
+}
+
+// Plain per-update path: the batch pipeline run on one update, one launch
+// sequence and one host round trip per update.
+static int single_update(csm_engine* e, bool add, uint32_t v1, uint32_t v2, uint32_t label,
+                         uint64_t* out) {
+  if (!e) return CSM_E_ARG;
+  csm_update u;
+  std::memset(&u, 0, sizeof u);
+  u.type = 'e';
+  u.is_add = add ? 1 : 0;
+  u.id1 = v1;
+  u.id2 = v2;
+  u.label = label;
+  if (v1 >= e->nv || v2 >= e->nv || e->h_vlab[v1] == NO_LABEL || e->h_vlab[v2] == NO_LABEL)
+    return fail(e, add ? CSM_E_ARG : CSM_E_DELETE_MISSING, "vertex out of range");
+  std::vector<uint64_t> pos(e->queries.size()), neg(e->queries.size());
+  int rc = csm_apply_batch(e, &u, 1, pos.data(), neg.data(), nullptr, nullptr);
+  if (rc) return rc;
+  uint64_t tot = 0;
+  for (size_t q = 0; q < pos.size(); ++q) tot += add ? pos[q] : neg[q];
+  if (out) *out = tot;
+  return CSM_OK;
+}
+
+int csm_add_edge(csm_engine* e, uint32_t v1, uint32_t v2, uint32_t label, uint64_t* pos) {
+  return single_update(e, true, v1, v2, label, pos);
+}
+
+int csm_remove_edge(csm_engine* e, uint32_t v1, uint32_t v2, uint64_t* neg) {
+  return single_update(e, false, v1, v2, 0, neg);
+}
+
+int csm_add_vertex(csm_engine* e, uint32_t id, uint32_t label) {
+  if (!e) return CSM_E_ARG;
+  if (id >= e->nv_cap) return fail(e, CSM_E_CAPACITY, "vertex id %u beyond capacity %u", id, e->nv_cap);
+  uint32_t d = dense_of(e->vl_map, label);
+  if (d == NO_LABEL) return fail(e, CSM_E_CAPACITY, "vertex label %u unknown to the engine", label);
+  // Graph::AddVertex (graph.cpp:24-44): only a vertex without a label gets one
+  if (e->h_vlab[id] != NO_LABEL) return CSM_OK;
+  e->h_vlab[id] = d;
+  if (id >= e->nv) e->nv = id + 1;
+  HIP_TRY(e, hipMemcpyAsync(e->d_vlab + id, &e->h_vlab[id], 4, hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  if (e->algo == ALGO_SYMBI && e->d_d2) return symbi_add_vertex(e, id);
+  return CSM_OK;
+}
+
+int csm_remove_vertex(csm_engine* e, uint32_t id) {
+  (void)id;
+  return fail(e, CSM_E_UNSUPPORTED, "vertex deletion is not supported on the device");
+}
+
+int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
+                           uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash) {
+  if (!e || (n && (!d_updates || !d_counts))) return CSM_E_ARG;
+  if (n > e->max_batch) return fail(e, CSM_E_CAPACITY, "batch of %zu > max_batch %u", n, e->max_batch);
+  if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
+  if (d_hash && !e->opts.want_hash) d_hash = nullptr;
+  if (n == 0) return CSM_OK;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  k_unpack<<<grid_for(n, 256), 256, 0, e->stream>>>(
+      reinterpret_cast<const UpdateRec*>(d_updates), (uint32_t)n, e->d_type, e->d_add, e->d_v1,
+      e->d_v2, e->d_lab, e->d_elmap, (uint32_t)e->el_map.size());
+  int rc = run_batch(e, (uint32_t)n, reinterpret_cast<unsigned long long*>(d_counts),
+                     reinterpret_cast<unsigned long long*>(d_hash), d_safe);
+  if (rc) return rc;
+  if ((rc = read_ctrl(e))) return rc;
+  accumulate_stats(e);
+  return check_errors(e);
+}
+
+int csm_sync(csm_engine* e) {
+  if (!e) return CSM_E_ARG;
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
+int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t* pos,
+                    uint64_t* neg, uint8_t* safe, uint64_t* emb_hash) {
+  if (!e || (n && !updates)) return CSM_E_ARG;
+  if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
+  HIP_TRY(e, hipSetDevice(e->dev));
+  const size_t nq = e->queries.size();
+  size_t done = 0;
+  while (done < n) {
+    size_t m = std::min<size_t>(n - done, e->max_batch);
+    // vertex insertions take effect before the slice's edges
+    size_t j = 0;
+    for (size_t i = 0; i < m; ++i) {
+      const csm_update& u = updates[done + i];
+      if (u.type == 'v') {
+        if (u.is_add) { int rc = csm_add_vertex(e, u.id1, u.label); if (rc) return rc; }
+        else return fail(e, CSM_E_UNSUPPORTED, "vertex deletion is not supported on the device");
+      }
+      std::memcpy(&e->h_upd[j++], &u, sizeof(UpdateRec));
+    }
+    HIP_TRY(e, hipMemcpyAsync(e->d_upd, e->h_upd, m * sizeof(UpdateRec), hipMemcpyHostToDevice,
+                              e->stream));
+    bool hash = emb_hash && e->opts.want_hash;
+    int rc = csm_apply_batch_device(e, reinterpret_cast<const csm_update*>(e->d_upd), m,
+                                    reinterpret_cast<uint64_t*>(e->d_counts), e->d_safe,
+                                    hash ? reinterpret_cast<uint64_t*>(e->d_hashes) : nullptr);
+    if (rc) return rc;
+    HIP_TRY(e, hipMemcpyAsync(e->h_counts, e->d_counts, m * nq * 8, hipMemcpyDeviceToHost, e->stream));
+    if (hash)
+      HIP_TRY(e, hipMemcpyAsync(e->h_hashes, e->d_hashes, m * nq * 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(e->h_safe, e->d_safe, m, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i < m; ++i) {
+      const csm_update& u = updates[done + i];
+      bool is_edge = u.type == 'e';
+      for (size_t q = 0; q < nq; ++q) {
+        uint64_t c = is_edge ? e->h_counts[i * nq + q] : 0;
+        size_t o = (done + i) * nq + q;
+        if (pos) pos[o] = (is_edge && u.is_add) ? c : 0;
+        if (neg) neg[o] = (is_edge && !u.is_add) ? c : 0;
+        if (emb_hash) emb_hash[o] = hash ? e->h_hashes[i * nq + q] : 0;
+        if (u.is_add) e->tot_pos += (is_edge ? c : 0); else e->tot_neg += (is_edge ? c : 0);
+      }
+      if (safe) safe[done + i] = is_edge ? e->h_safe[i] : 1;
+      if (is_edge) e->num_edges += u.is_add ? 1 : -1;
+    }
+    done += m;
+  }
+  return CSM_OK;
+}
+
+int csm_totals(const csm_engine* e, uint64_t* initial, uint64_t* positive, uint64_t* negative) {
+  if (!e) return CSM_E_ARG;
+  if (initial) *initial = e->tot_init;
+  if (positive) *positive = e->tot_pos;
+  if (negative) *negative = e->tot_neg;
+  return CSM_OK;
+}
+
+int csm_memory(const csm_engine* e, size_t* num_edges, size_t* num_vertices) {
+  if (!e) return CSM_E_ARG;
+  if (num_edges) *num_edges = (size_t)e->num_edges;
+  if (num_vertices) *num_vertices = e->nv;
+  return CSM_OK;
+}
+
+int csm_device_bytes(const csm_engine* e, uint64_t* bytes) {
+  if (!e || !bytes) return CSM_E_ARG;
+  *bytes = e->dev_bytes;
+  return CSM_OK;
+}
+
+int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
+  if (!e || !out) return CSM_E_ARG;
+  for (size_t i = 0; i < n_out && i < 5; ++i) out[i] = e->stats[i];
+  return CSM_OK;
+}
+
+void* csm_stream(csm_engine* e) { return e ? (void*)e->stream : nullptr; }
+
+const char* csm_last_error(const csm_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+void csm_destroy(csm_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->dev);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (auto& a : e->allocs) (void)hipFree(a.p);
+  e->allocs.clear();
+  if (e->h_ctrl) (void)hipHostFree(e->h_ctrl);
+  if (e->h_counts) (void)hipHostFree(e->h_counts);
+  if (e->h_hashes) (void)hipHostFree(e->h_hashes);
+  if (e->h_safe) (void)hipHostFree(e->h_safe);
+  if (e->h_upd) (void)hipHostFree(e->h_upd);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+}  // extern "C"

@@ -1,0 +1,934 @@
+// gfx950 kernels of the incremental match engine.
+//
+// Device graph layout (HBM):
+//   vrec[v]  = {off, len | DIRTY}: the neighbour list of v is adj[off, off+len)
+//   adj[i]   = packed neighbour entry  (key << vbits) | nbr,  key = L(nbr)*nel + elabel
+//              so each list is sorted by (neighbour label, edge label, id) and
+//              the label-filtered neighbour set S(v, L, l) -- the reference's
+//              DCS list (ParaCOSM/CSM/matching/SymBi/symbi.cpp:404-529) and the
+//              label test of the GraphFlow scan (graphflow.cpp:212-218) -- is
+//              one contiguous segment of it.
+//   ats[i]   = version word of adj[i]: (ins << 16) | del, positions inside the
+//              current batch; an entry is visible to the search of the update
+//              at position p iff ins <= p <= del.  Untouched entries hold
+//              TS_BASE (always visible); only lists flagged DIRTY hold others.
+//   vlab[v]  = dense vertex label (NO_LABEL when the vertex does not exist).
+//
+// Lists are never edited in place: a list that changes is rewritten into
+// fresh space taken from the pool with one atomic bump, and the pool is
+// repacked when it runs low.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace csm {
+
+constexpr uint32_t DIRTY = 0x80000000u;
+constexpr uint32_t LEN_MASK = 0x7fffffffu;
+constexpr uint32_t TS_BASE = 0x0000FFFFu;   // ins 0, never deleted
+constexpr uint32_t TS_DEAD = 0xFFFF0000u;   // never visible
+constexpr uint32_t POS_MAX = 0xFFFEu;       // largest update position in a batch
+constexpr uint32_t NO_LABEL = 0xFFFFFFFFu;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int MAXK = 16;                    // query vertices supported by the search
+constexpr int MAXNB = MAXK - 1;
+constexpr int WAVE = 64;
+constexpr int SEARCH_WAVES = 4;             // waves per search workgroup
+
+// error bits in Ctrl::err
+constexpr uint32_t ERR_DELETE_MISSING = 1u;
+constexpr uint32_t ERR_SEED_OVERFLOW = 2u;
+constexpr uint32_t ERR_CHUNK_OVERFLOW = 4u;
+constexpr uint32_t ERR_POOL_OVERFLOW = 8u;
+constexpr uint32_t ERR_BAD_UPDATE = 16u;
+
+struct Ctrl {
+  uint32_t pool_top;      // bump pointer of the current pool (entries)
+  uint32_t nseeds;        // seeds emitted this batch
+  uint32_t nchunks;       // search tasks (seed chunks) this batch
+  uint32_t chunk_head;    // next chunk to run
+  uint32_t err;           // ERR_* bits
+  uint32_t err_at;        // first failing update index (min)
+  uint32_t nruns;         // vertices with inserted entries this batch
+  uint32_t ndirty_next;   // dirty vertices recorded for the next batch
+  uint32_t ndirty_cur;    // dirty vertices to compact at the start of this batch
+  uint32_t abort;         // host-set: stop searching
+  uint32_t pad[6];
+  unsigned long long live;        // live entries (sum of list lengths)
+  unsigned long long stat_tasks;
+  unsigned long long stat_cands;
+  unsigned long long stat_bytes;
+};
+
+// One search plan per (query, matching order): positions 0..k-1.
+struct PlanRec {
+  uint32_t k;
+  uint32_t q;                    // query id (result column)
+  uint32_t symbi;                // 1: filter candidates by the d2 bitmap
+  uint32_t pad;
+  uint32_t qv[MAXK];             // query vertex at position p
+  uint32_t nb[MAXK];             // backward neighbours of position p
+  uint32_t bw[MAXK][MAXNB];      // (backward position) | (segment key << 8)
+  uint32_t d2row[MAXK];          // row of position p's query vertex in the d2 bitmap
+};
+
+// A seed: one (update, query edge orientation) with m[0], m[1] fixed, and the
+// depth-2 candidate segment that its chunks split.
+struct Seed {
+  uint32_t t;        // update index in the batch
+  uint32_t qo;       // plan index
+  uint32_t m0, m1;
+  uint32_t lo, hi;   // depth-2 candidate segment (absolute pool offsets)
+  uint32_t base;     // first chunk id
+  uint32_t jmin;     // which backward neighbour supplied the segment
+};
+
+struct GraphView {
+  const uint2* __restrict__ vrec;
+  const uint32_t* __restrict__ vlab;
+  const uint32_t* __restrict__ adj;
+  const uint32_t* __restrict__ ats;
+  uint32_t vbits;
+  uint32_t vmask;
+  uint32_t nel;
+  uint32_t nv;
+};
+
+__device__ __forceinline__ bool ts_visible(uint32_t ts, uint32_t pos) {
+  return (ts >> 16) <= pos && pos <= (ts & 0xFFFFu);
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// first index i in [lo, hi) with a[i] >= x (hi if none)
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ a, uint32_t lo,
+                                                uint32_t hi, uint32_t x) {
+  uint32_t n = hi - lo;
+  while (n > 0) {
+    uint32_t half = n >> 1;
+    uint32_t mid = lo + half;
+    bool less = a[mid] < x;
+    lo = less ? mid + 1 : lo;
+    n = less ? n - half - 1 : half;
+  }
+  return lo;
+}
+
+// segment of key `key` inside the list [off, off+len)
+__device__ __forceinline__ uint2 segment(const uint32_t* __restrict__ adj, uint32_t off,
+                                         uint32_t len, uint32_t key, uint32_t vbits) {
+  uint32_t end = off + len;
+  uint32_t lo = lower_bound(adj, off, end, key << vbits);
+  uint64_t hk = (uint64_t)(key + 1) << vbits;
+  uint32_t hi = hk > 0xFFFFFFFFull ? end : lower_bound(adj, lo, end, (uint32_t)hk);
+  return make_uint2(lo, hi);
+}
+
+// is packed value x visible in [lo, hi) at update position pos
+__device__ __forceinline__ bool contains(const uint32_t* __restrict__ adj,
+                                         const uint32_t* __restrict__ ats, uint32_t lo,
+                                         uint32_t hi, uint32_t x, bool versioned, uint32_t pos) {
+  uint32_t i = lower_bound(adj, lo, hi, x);
+  if (!versioned) return i < hi && adj[i] == x;
+  for (; i < hi && adj[i] == x; ++i)
+    if (ts_visible(ats[i], pos)) return true;
+  return false;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t popc64(uint64_t x) { return (uint32_t)__popcll(x); }
+
+__device__ __forceinline__ uint32_t uniform(uint32_t x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint64_t y = __shfl_xor(x, o);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ bool d2_bit(const uint32_t* __restrict__ d2, uint32_t row,
+                                       uint32_t words_per_row, uint32_t v) {
+  return (d2[(size_t)row * words_per_row + (v >> 5)] >> (v & 31)) & 1u;
+}
+
+// ------------------------------------------------------------ graph build
+// Directed entries of an edge list, keyed (src << 32 | nbr) for the duplicate
+// pass.  Self-loops are dropped.
+__global__ void k_build_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                             uint64_t n, uint64_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t a = src[i], b = dst[i];
+    bool loop = a == b;
+    keys[2 * i] = loop ? ~0ull : ((uint64_t)a << 32) | b;
+    keys[2 * i + 1] = loop ? ~0ull : ((uint64_t)b << 32) | a;
+    vals[2 * i] = (uint32_t)i;
+    vals[2 * i + 1] = (uint32_t)i;
+  }
+}
+
+// After a stable sort by (src, nbr): keep the first copy of each pair (the
+// earliest input edge, as Graph::AddEdge ignores later duplicates,
+// graph.cpp:63-64) and re-key it by (src << 32 | packed entry).
+__global__ void k_build_pack(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                             uint64_t n, const uint32_t* __restrict__ vlab,
+                             const uint32_t* __restrict__ elab_dense, uint32_t nel,
+                             uint32_t vbits, uint64_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    bool dup = (k == ~0ull) || (i > 0 && keys[i - 1] == k);
+    if (dup) { out[i] = ~0ull; continue; }
+    uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+    uint32_t key = vlab[b] * nel + elab_dense[vals[i]];
+    out[i] = ((uint64_t)a << 32) | ((key << vbits) | b);
+  }
+}
+
+// Sorted (src, packed) entries -> pool + vertex records.
+__global__ void k_build_lists(const uint64_t* __restrict__ keys, uint64_t n,
+                              uint32_t* __restrict__ adj, uint32_t* __restrict__ ats,
+                              uint2* __restrict__ vrec, uint32_t* __restrict__ vend) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    if (k == ~0ull) continue;
+    uint32_t a = (uint32_t)(k >> 32);
+    adj[i] = (uint32_t)k;
+    ats[i] = TS_BASE;
+    if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != a) vrec[a].x = (uint32_t)i;
+    if (i + 1 == n || keys[i + 1] == ~0ull || (uint32_t)(keys[i + 1] >> 32) != a)
+      vend[a] = (uint32_t)(i + 1);
+  }
+}
+
+__global__ void k_build_lens(uint2* __restrict__ vrec, const uint32_t* __restrict__ vend,
+                             uint32_t nv) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+    uint32_t e = vend[v];
+    vrec[v].y = e ? e - vrec[v].x : 0u;
+    if (!e) vrec[v].x = 0u;
+  }
+}
+
+// ------------------------------------------------------------ batch apply
+struct BatchView {
+  const uint8_t* __restrict__ type;   // 'e' / 'v'
+  const uint8_t* __restrict__ add;
+  const uint32_t* __restrict__ v1;
+  const uint32_t* __restrict__ v2;
+  const uint32_t* __restrict__ label; // original edge labels (inserts)
+  uint32_t n;
+};
+
+// Unpack csm_update records (16 B) into structure-of-arrays.
+struct UpdateRec { uint8_t type, add; uint16_t pad; uint32_t id1, id2, label; };
+
+__global__ void k_unpack(const UpdateRec* __restrict__ u, uint32_t n, uint8_t* __restrict__ type,
+                         uint8_t* __restrict__ add, uint32_t* __restrict__ v1,
+                         uint32_t* __restrict__ v2, uint32_t* __restrict__ lab,
+                         const uint32_t* __restrict__ elab_map, uint32_t elab_map_n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    UpdateRec r = u[i];
+    type[i] = r.type;
+    add[i] = r.add;
+    v1[i] = r.id1;
+    v2[i] = r.id2;
+    // dense edge label for insertions (NO_LABEL if unknown)
+    uint32_t l = NO_LABEL;
+    if (r.type == 'e' && r.add) {
+      // elab_map: sorted original labels; dense id = index
+      uint32_t lo = 0, hi = elab_map_n;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (elab_map[mid] < r.label) lo = mid + 1; else hi = mid;
+      }
+      if (lo < elab_map_n && elab_map[lo] == r.label) l = lo;
+    }
+    lab[i] = l;
+  }
+}
+
+// Directed entries of the batch's insertions keyed (src << 32 | packed),
+// valued by update position; other records give a sentinel key.
+__global__ void k_batch_keys(BatchView b, const uint32_t* __restrict__ vlab, uint32_t nel,
+                             uint32_t vbits, uint32_t nv, uint64_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals, uint32_t* __restrict__ ctrl_err,
+                             uint32_t* __restrict__ ctrl_err_at) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+    uint64_t k0 = ~0ull, k1 = ~0ull;
+    if (b.type[i] == 'e' && b.add[i]) {
+      uint32_t a = b.v1[i], c = b.v2[i], l = b.label[i];
+      if (a < nv && c < nv && a != c && l != NO_LABEL && vlab[a] != NO_LABEL &&
+          vlab[c] != NO_LABEL) {
+        k0 = ((uint64_t)a << 32) | (((vlab[c] * nel + l) << vbits) | c);
+        k1 = ((uint64_t)c << 32) | (((vlab[a] * nel + l) << vbits) | a);
+      } else {
+        atomicOr(ctrl_err, ERR_BAD_UPDATE);
+        atomicMin(ctrl_err_at, i);
+      }
+    }
+    keys[2 * i] = k0;
+    keys[2 * i + 1] = k1;
+    vals[2 * i] = i + 1;
+    vals[2 * i + 1] = i + 1;
+  }
+}
+
+// Heads of the sorted insertion entries -> runs (one per touched source).
+__global__ void k_runs(const uint64_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ run_start,
+                       uint32_t* __restrict__ nruns) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    if (k == ~0ull) continue;
+    if (i == 0 || (keys[i - 1] >> 32) != (k >> 32)) run_start[atomicAdd(nruns, 1u)] = i;
+  }
+}
+
+// Compaction of lists dirtied by the previous batch: drop entries deleted in
+// that batch, reset the version of the rest to TS_BASE, clear DIRTY.  One
+// workgroup per vertex.  A list without tombstones is fixed in place.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_compact(uint2* __restrict__ vrec, uint32_t* __restrict__ adj,
+                                                uint32_t* __restrict__ ats,
+                                                const uint32_t* __restrict__ list,
+                                                const uint32_t* __restrict__ count,
+                                                Ctrl* __restrict__ ctrl, uint32_t pool_cap) {
+  __shared__ uint32_t s_cnt[BS / WAVE];
+  __shared__ uint32_t s_total, s_off;
+  const uint32_t n = *count;
+  for (uint32_t it = blockIdx.x; it < n; it += gridDim.x) {
+    uint32_t v = list[it];
+    uint2 r = vrec[v];
+    uint32_t off = r.x, len = r.y & LEN_MASK;
+    // pass 1: tombstones?
+    uint32_t dead = 0;
+    for (uint32_t i = threadIdx.x; i < len; i += BS) dead += (ats[off + i] & 0xFFFFu) != 0xFFFFu;
+    dead = wave_sum(dead);
+    if (lane_id() == 0) s_cnt[threadIdx.x / WAVE] = dead;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < BS / WAVE; ++w) t += s_cnt[w];
+      s_total = t;
+      uint32_t keep = len - t;
+      if (t) {
+        uint32_t o = atomicAdd(&ctrl->pool_top, keep);
+        if ((uint64_t)o + keep > pool_cap) { atomicOr(&ctrl->err, ERR_POOL_OVERFLOW); o = NONE; }
+        s_off = o;
+        atomicAdd(&ctrl->live, (unsigned long long)0 - (unsigned long long)t);
+      } else {
+        s_off = off;
+      }
+    }
+    __syncthreads();
+    uint32_t total_dead = s_total, noff = s_off;
+    if (noff == NONE) { __syncthreads(); continue; }
+    if (total_dead == 0) {
+      for (uint32_t i = threadIdx.x; i < len; i += BS) ats[off + i] = TS_BASE;
+    } else {
+      // pass 2: ordered stream compaction in chunks of BS
+      uint32_t written = 0;
+      for (uint32_t c = 0; c < len; c += BS) {
+        uint32_t i = c + threadIdx.x;
+        bool keep = i < len && (ats[off + i] & 0xFFFFu) == 0xFFFFu;
+        uint64_t m = ballot(keep);
+        uint32_t before = popc64(m & ((1ull << lane_id()) - 1ull));
+        __syncthreads();
+        if (lane_id() == 0) s_cnt[threadIdx.x / WAVE] = popc64(m);
+        __syncthreads();
+        uint32_t wbase = 0, ctot = 0;
+        for (int w = 0; w < BS / WAVE; ++w) {
+          if (w < (int)(threadIdx.x / WAVE)) wbase += s_cnt[w];
+          ctot += s_cnt[w];
+        }
+        if (keep) {
+          adj[noff + written + wbase + before] = adj[off + i];
+          ats[noff + written + wbase + before] = TS_BASE;
+        }
+        written += ctot;
+      }
+    }
+    if (threadIdx.x == 0) vrec[v] = make_uint2(noff, len - total_dead);
+    __syncthreads();
+  }
+}
+
+// Merge each touched vertex's sorted new entries into a fresh copy of its
+// (clean) list.  One workgroup per run.  New entries carry their update
+// position as `ins`; the list is flagged DIRTY and recorded for compaction at
+// the start of the next batch.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_merge(uint2* __restrict__ vrec, uint32_t* __restrict__ adj,
+                                              uint32_t* __restrict__ ats,
+                                              const uint64_t* __restrict__ keys,
+                                              const uint32_t* __restrict__ vals, uint32_t nkeys,
+                                              const uint32_t* __restrict__ run_start,
+                                              Ctrl* __restrict__ ctrl, uint32_t* __restrict__ dirty_next,
+                                              uint32_t pool_cap) {
+  __shared__ uint32_t s_off;
+  __shared__ uint32_t s_b;
+  const uint32_t nr = ctrl->nruns;
+  for (uint32_t it = blockIdx.x; it < nr; it += gridDim.x) {
+    uint32_t bs = run_start[it];
+    uint32_t v = (uint32_t)(keys[bs] >> 32);
+    if (threadIdx.x == 0) {
+      uint32_t e = bs + 1;
+      while (e < nkeys && keys[e] != ~0ull && (uint32_t)(keys[e] >> 32) == v) ++e;
+      s_b = e - bs;
+    }
+    __syncthreads();
+    uint32_t b = s_b;
+    uint2 r = vrec[v];
+    uint32_t off = r.x, a = r.y & LEN_MASK;
+    if (threadIdx.x == 0) {
+      uint32_t o = atomicAdd(&ctrl->pool_top, a + b);
+      if ((uint64_t)o + a + b > pool_cap) { atomicOr(&ctrl->err, ERR_POOL_OVERFLOW); o = NONE; }
+      s_off = o;
+      atomicAdd(&ctrl->live, (unsigned long long)b);
+    }
+    __syncthreads();
+    uint32_t noff = s_off;
+    if (noff != NONE) {
+      // old entries: out = i + #(new < old[i])
+      for (uint32_t i = threadIdx.x; i < a; i += BS) {
+        uint32_t x = adj[off + i];
+        uint32_t lo = 0, n = b;
+        while (n > 0) {
+          uint32_t h = n >> 1;
+          bool less = (uint32_t)keys[bs + lo + h] < x;
+          lo = less ? lo + h + 1 : lo;
+          n = less ? n - h - 1 : h;
+        }
+        adj[noff + i + lo] = x;
+        ats[noff + i + lo] = ats[off + i];
+      }
+      // new entries: out = j + #(old <= new[j])
+      for (uint32_t j = threadIdx.x; j < b; j += BS) {
+        uint32_t x = (uint32_t)keys[bs + j];
+        uint32_t lo = 0, n = a;
+        while (n > 0) {
+          uint32_t h = n >> 1;
+          bool le = adj[off + lo + h] <= x;
+          lo = le ? lo + h + 1 : lo;
+          n = le ? n - h - 1 : h;
+        }
+        adj[noff + j + lo] = x;
+        ats[noff + j + lo] = (vals[bs + j] << 16) | 0xFFFFu;
+      }
+      if (threadIdx.x == 0) {
+        uint32_t old = r.y;
+        vrec[v] = make_uint2(noff, (a + b) | DIRTY);
+        if (!(old & DIRTY)) dirty_next[atomicAdd(&ctrl->ndirty_next, 1u)] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Locate the live version of (a -> b) for the update at position pos: the
+// entry for neighbour b (any edge label) with the largest ins < pos
+// (mode 0), or any version visible just before pos (mode 1).  Returns the
+// pool index or NONE, and the dense edge label in *el.
+__device__ __forceinline__ uint32_t find_version(const GraphView& g, uint32_t a, uint32_t b,
+                                                 uint32_t pos, int mode, uint32_t* el) {
+  uint2 r = g.vrec[a];
+  uint32_t off = r.x, end = r.x + (r.y & LEN_MASK);
+  uint32_t lb_ = g.vlab[b];
+  uint32_t best = NONE, best_ins = 0;
+  for (uint32_t l = 0; l < g.nel; ++l) {
+    uint32_t x = ((lb_ * g.nel + l) << g.vbits) | b;
+    for (uint32_t i = lower_bound(g.adj, off, end, x); i < end && g.adj[i] == x; ++i) {
+      uint32_t ts = g.ats[i];
+      uint32_t ins = ts >> 16, del = ts & 0xFFFFu;
+      if (mode == 0) {
+        if (ins < pos && (best == NONE || ins >= best_ins)) { best = i; best_ins = ins; *el = l; }
+      } else {
+        if (ins < pos && del >= pos) { *el = l; return i; }
+      }
+    }
+  }
+  return best;
+}
+
+// Deletions: mark the live version of both directed entries deleted at the
+// update's position and record the edge label for the seeds.
+__global__ void k_delete_mark(BatchView b, GraphView g, uint32_t* __restrict__ ats,
+                              uint2* __restrict__ vrec, uint32_t* __restrict__ seed_label,
+                              Ctrl* __restrict__ ctrl, uint32_t* __restrict__ dirty_next) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+    if (b.type[i] != 'e') { seed_label[i] = NO_LABEL; continue; }
+    if (b.add[i]) { seed_label[i] = b.label[i]; continue; }
+    uint32_t pos = i + 1, x = b.v1[i], y = b.v2[i];
+    uint32_t el0 = NO_LABEL, el1 = NO_LABEL;
+    uint32_t i0 = NONE, i1 = NONE;
+    if (x < g.nv && y < g.nv && g.vlab[x] != NO_LABEL && g.vlab[y] != NO_LABEL) {
+      i0 = find_version(g, x, y, pos, 0, &el0);
+      i1 = find_version(g, y, x, pos, 0, &el1);
+    }
+    if (i0 == NONE || i1 == NONE) {
+      atomicOr(&ctrl->err, ERR_DELETE_MISSING);
+      atomicMin(&ctrl->err_at, i);
+      seed_label[i] = NO_LABEL;
+      continue;
+    }
+    // the chosen versions must still be alive; a second deletion of the same
+    // version is a deletion of an absent edge
+    uint32_t t0 = ats[i0], t1 = ats[i1];
+    uint32_t e0 = (t0 & 0xFFFF0000u) | 0xFFFFu, e1 = (t1 & 0xFFFF0000u) | 0xFFFFu;
+    bool ok0 = atomicCAS(&ats[i0], e0, (t0 & 0xFFFF0000u) | pos) == e0;
+    bool ok1 = atomicCAS(&ats[i1], e1, (t1 & 0xFFFF0000u) | pos) == e1;
+    if (!ok0 || !ok1) {
+      atomicOr(&ctrl->err, ERR_DELETE_MISSING);
+      atomicMin(&ctrl->err_at, i);
+      seed_label[i] = NO_LABEL;
+      continue;
+    }
+    seed_label[i] = el0;
+    uint32_t* y0 = reinterpret_cast<uint32_t*>(&vrec[x]) + 1;
+    uint32_t* y1 = reinterpret_cast<uint32_t*>(&vrec[y]) + 1;
+    if (!(atomicOr(y0, DIRTY) & DIRTY)) dirty_next[atomicAdd(&ctrl->ndirty_next, 1u)] = x;
+    if (!(atomicOr(y1, DIRTY) & DIRTY)) dirty_next[atomicAdd(&ctrl->ndirty_next, 1u)] = y;
+  }
+}
+
+// Insertions of an edge that is already present just before the insertion
+// leave the graph unchanged (graph.cpp:63-64): hide the new version.  The
+// update is still searched, as the reference's AddEdge does.
+__global__ void k_dedup(BatchView b, GraphView g, uint32_t* __restrict__ ats) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+    if (b.type[i] != 'e' || !b.add[i] || b.label[i] == NO_LABEL) continue;
+    uint32_t pos = i + 1;
+    for (int dir = 0; dir < 2; ++dir) {
+      uint32_t x = dir ? b.v2[i] : b.v1[i], y = dir ? b.v1[i] : b.v2[i];
+      if (x >= g.nv || y >= g.nv || x == y) continue;
+      uint32_t el;
+      if (find_version(g, x, y, pos, 1, &el) == NONE) continue;
+      // hide this update's own version(s) of (x -> y)
+      uint2 r = g.vrec[x];
+      uint32_t off = r.x, end = r.x + (r.y & LEN_MASK);
+      uint32_t xv = ((g.vlab[y] * g.nel + b.label[i]) << g.vbits) | y;
+      for (uint32_t j = lower_bound(g.adj, off, end, xv); j < end && g.adj[j] == xv; ++j)
+        if ((g.ats[j] >> 16) == pos) ats[j] = TS_DEAD;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ seeds
+struct SeedArgs {
+  GraphView g;
+  BatchView b;
+  const uint32_t* __restrict__ seed_label;   // dense edge label per update
+  const uint32_t* __restrict__ tab_off;      // [nlab*nlab*nel + 1]
+  const uint32_t* __restrict__ tab_ent;      // (qo << 1) | swap
+  const PlanRec* __restrict__ plans;
+  const uint32_t* __restrict__ d2;           // SymBi d2 bitmap (nullable)
+  uint32_t d2_words;
+  uint32_t nlab;
+  uint32_t nq;
+  uint32_t chunk;
+  uint32_t versioned;
+  uint32_t homo;
+  uint32_t want_hash;
+  Seed* seeds;
+  uint32_t seed_cap;
+  uint32_t* chunk_owner;
+  uint32_t chunk_cap;
+  Ctrl* ctrl;
+  unsigned long long* counts;
+  unsigned long long* hashes;
+  uint8_t* safe;
+};
+
+// One thread per (update, seed template): fix m[0], m[1], find the depth-2
+// candidate segment and emit ceil(|segment| / chunk) search tasks.  Updates
+// with no template are "safe" (the label-triple filter of
+// parallel_graphflow.cpp:1990-2016).
+__global__ void k_seeds(SeedArgs a) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.b.n;
+       i += gridDim.x * blockDim.x) {
+    uint32_t l = a.seed_label[i];
+    uint32_t x = a.b.v1[i], y = a.b.v2[i];
+    bool edge = a.b.type[i] == 'e' && l != NO_LABEL && x < a.g.nv && y < a.g.nv;
+    uint32_t s = 0, e = 0;
+    if (edge) {
+      uint32_t lx = a.g.vlab[x], ly = a.g.vlab[y];
+      if (lx != NO_LABEL && ly != NO_LABEL && lx < a.nlab && ly < a.nlab) {
+        uint32_t bucket = (lx * a.nlab + ly) * a.g.nel + l;
+        s = a.tab_off[bucket];
+        e = a.tab_off[bucket + 1];
+      }
+    }
+    if (a.safe) a.safe[i] = (s == e);
+    uint32_t pos = i + 1;
+    for (uint32_t j = s; j < e; ++j) {
+      uint32_t ent = a.tab_ent[j];
+      uint32_t qo = ent >> 1;
+      const PlanRec* P = a.plans + qo;
+      uint32_t m0 = (ent & 1) ? y : x, m1 = (ent & 1) ? x : y;
+      if (P->symbi && a.d2 &&
+          !(d2_bit(a.d2, P->d2row[0], a.d2_words, m0) && d2_bit(a.d2, P->d2row[1], a.d2_words, m1)))
+        continue;
+      if (P->k == 2) {
+        atomicAdd(&a.counts[(size_t)i * a.nq + P->q], 1ull);
+        if (a.want_hash) {
+          uint32_t mq[2];
+          mq[P->qv[0]] = m0;
+          mq[P->qv[1]] = m1;
+          uint64_t h = mix64(mix64((uint64_t)mq[0] + 1) + (uint64_t)mq[1] + 1);
+          atomicAdd(&a.hashes[(size_t)i * a.nq + P->q], (unsigned long long)h);
+        }
+        continue;
+      }
+      // depth-2 segment: smallest label-filtered partner segment
+      uint32_t best = NONE, blo = 0, bhi = 0, bj = 0;
+      for (uint32_t jj = 0; jj < P->nb[2]; ++jj) {
+        uint32_t w = P->bw[2][jj];
+        uint32_t vx = (w & 0xFFu) == 0 ? m0 : m1;
+        uint2 r = a.g.vrec[vx];
+        uint2 sg = segment(a.g.adj, r.x, r.y & LEN_MASK, w >> 8, a.g.vbits);
+        uint32_t sz = sg.y - sg.x;
+        if (best == NONE || sz < best) { best = sz; blo = sg.x; bhi = sg.y; bj = jj; }
+      }
+      if (best == 0 || best == NONE) continue;
+      uint32_t nch = (best + a.chunk - 1) / a.chunk;
+      uint32_t sidx = atomicAdd(&a.ctrl->nseeds, 1u);
+      if (sidx >= a.seed_cap) { atomicOr(&a.ctrl->err, ERR_SEED_OVERFLOW); continue; }
+      uint32_t base = atomicAdd(&a.ctrl->nchunks, nch);
+      if ((uint64_t)base + nch > a.chunk_cap) {
+        atomicOr(&a.ctrl->err, ERR_CHUNK_OVERFLOW);
+        continue;
+      }
+      Seed sd;
+      sd.t = i; sd.qo = qo; sd.m0 = m0; sd.m1 = m1; sd.lo = blo; sd.hi = bhi;
+      sd.base = base; sd.jmin = bj;
+      a.seeds[sidx] = sd;
+      for (uint32_t c = 0; c < nch; ++c) a.chunk_owner[base + c] = sidx;
+    }
+    (void)pos;
+  }
+}
+
+// ----------------------------------------------------------------- search
+struct SearchArgs {
+  GraphView g;
+  const PlanRec* __restrict__ plans;
+  const Seed* __restrict__ seeds;
+  const uint32_t* __restrict__ chunk_owner;
+  Ctrl* ctrl;
+  unsigned long long* counts;
+  unsigned long long* hashes;
+  const uint32_t* __restrict__ d2;
+  uint32_t d2_words;
+  uint32_t nq;
+  uint32_t chunk;
+  uint32_t homo;
+  uint32_t pos_offset;   // position of update index 0 minus 1 (0 in a batch)
+};
+
+// Per-wave DFS state in LDS.
+struct WaveState {
+  uint32_t m[MAXK];           // data vertex at each position
+  uint32_t mq[MAXK];          // data vertex of each query vertex (hash)
+  uint32_t cur[MAXK], end[MAXK];
+  uint32_t bn[MAXK], bi[MAXK];
+  uint32_t jmin[MAXK], pdirty[MAXK], udirty[MAXK];
+  uint32_t plo[MAXK][MAXNB], phi[MAXK][MAXNB];
+  uint32_t cand[MAXK][WAVE];
+};
+
+// One wavefront runs one depth-2 chunk of a seed at a time: an iterative DFS
+// over positions 2..k-1 whose per-position candidate buffers (one wave-width
+// each) live in LDS.  Each refill tests 64 consecutive entries of the u_min
+// segment in parallel: version visibility, injectivity against m[], the SymBi
+// d2 bit, and a binary search of every other backward neighbour's segment
+// (the join of graphflow.cpp:222-238).  The last position is never buffered:
+// its survivors are counted with one ballot.
+template <bool VERSIONED, bool HASH, bool SYMBI>
+__global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
+  __shared__ WaveState S_all[SEARCH_WAVES];
+  const uint32_t lane = lane_id();
+  WaveState& S = S_all[threadIdx.x / WAVE];
+  const uint32_t* __restrict__ adj = a.g.adj;
+  const uint32_t* __restrict__ ats = a.g.ats;
+  const uint32_t vbits = a.g.vbits, vmask = a.g.vmask;
+  unsigned long long n_tasks = 0, n_cands = 0, n_bytes = 0;
+
+  while (true) {
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(&a.ctrl->chunk_head, 1u);
+    c = uniform(__shfl(c, 0));
+    uint32_t nch = uniform(__atomic_load_n(&a.ctrl->nchunks, __ATOMIC_RELAXED));
+    if (c >= nch) break;
+    if (__atomic_load_n(&a.ctrl->abort, __ATOMIC_RELAXED)) break;
+    const uint32_t sidx = uniform(a.chunk_owner[c]);
+    const Seed sd = a.seeds[sidx];
+    const uint32_t t = uniform(sd.t), qo = uniform(sd.qo);
+    const PlanRec* __restrict__ P = a.plans + qo;
+    const uint32_t k = uniform(P->k);
+    const uint32_t pos = t + 1 + a.pos_offset;
+    const uint32_t q = uniform(P->q);
+    ++n_tasks;
+    S.m[0] = sd.m0;
+    S.m[1] = sd.m1;
+    if (HASH) {
+      S.mq[P->qv[0]] = sd.m0;
+      S.mq[P->qv[1]] = sd.m1;
+    }
+    unsigned long long cnt = 0, hsum = 0;
+
+    // enter position `lv`: partner segments, u_min, candidate range
+    auto enter = [&](uint32_t lv, uint32_t rlo, uint32_t rhi, int forced_j) -> bool {
+      const uint32_t nb = uniform(P->nb[lv]);
+      uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu;
+      if (lane < nb) {
+        uint32_t w = P->bw[lv][lane];
+        uint32_t vx = S.m[w & 0xFFu];
+        uint2 r = a.g.vrec[vx];
+        uint2 sg = segment(adj, r.x, r.y & LEN_MASK, w >> 8, vbits);
+        my_lo = sg.x;
+        my_hi = sg.y;
+        my_size = sg.y - sg.x;
+        my_dirty = VERSIONED ? (r.y >> 31) : 0u;
+        S.plo[lv][lane] = my_lo;
+        S.phi[lv][lane] = my_hi;
+      }
+      uint64_t mk = ((uint64_t)my_size << 32) | lane;
+      uint64_t mn = (forced_j >= 0) ? (((uint64_t)__shfl(my_size, forced_j) << 32) | (uint32_t)forced_j)
+                                    : wave_min_u64(mk);
+      uint32_t jm = uniform((uint32_t)mn);
+      uint32_t msz = uniform((uint32_t)(mn >> 32));
+      uint32_t dmask = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
+      unsigned long long seg_total = wave_sum((unsigned long long)(lane < nb ? my_size : 0u));
+      n_bytes += 4ull * seg_total;
+      wave_sync();
+      if (msz == 0) return false;
+      uint32_t lo = uniform(__shfl(my_lo, jm)), hi = uniform(__shfl(my_hi, jm));
+      lo = lo > rlo ? lo : rlo;
+      hi = hi < rhi ? hi : rhi;
+      S.cur[lv] = lo;
+      S.end[lv] = hi;
+      S.bn[lv] = 0;
+      S.bi[lv] = 0;
+      S.jmin[lv] = jm;
+      S.pdirty[lv] = dmask;
+      S.udirty[lv] = (dmask >> jm) & 1u;
+      return lo < hi;
+    };
+
+    // test entry e of position lv's u_min segment
+    auto test = [&](uint32_t lv, uint32_t e, uint32_t end, uint32_t& vout) -> bool {
+      bool ok = e < end;
+      uint32_t v = 0;
+      if (ok) {
+        v = adj[e] & vmask;
+        if (VERSIONED && S.udirty[lv]) ok = ts_visible(ats[e], pos);
+      }
+      if (ok && !a.homo) {
+        for (uint32_t i = 0; i < lv; ++i) ok = ok && (S.m[i] != v);
+      }
+      if (SYMBI && ok) ok = d2_bit(a.d2, P->d2row[lv], a.d2_words, v);
+      const uint32_t nb = uniform(P->nb[lv]);
+      const uint32_t jm = S.jmin[lv];
+      for (uint32_t j = 0; j < nb; ++j) {
+        if (j == jm) continue;
+        if (!ok) continue;
+        uint32_t w = P->bw[lv][j];
+        uint32_t x = ((w >> 8) << vbits) | v;
+        bool vd = VERSIONED && ((S.pdirty[lv] >> j) & 1u);
+        ok = contains(adj, ats, S.plo[lv][j], S.phi[lv][j], x, vd, pos);
+      }
+      vout = v;
+      return ok;
+    };
+
+    const uint32_t off = c - sd.base;
+    uint32_t rlo = sd.lo + off * a.chunk;
+    uint32_t rhi = rlo + a.chunk;
+    rhi = rhi < sd.hi ? rhi : sd.hi;
+    int lv = 2;
+    bool live = enter(2, rlo, rhi, (int)sd.jmin);
+    while (live) {
+      const uint32_t L = (uint32_t)lv;
+      if (L == k - 1) {
+        uint32_t cur = S.cur[L], end = S.end[L];
+        for (; cur < end; cur += WAVE) {
+          uint32_t v;
+          bool ok = test(L, cur + lane, end, v);
+          uint64_t m = ballot(ok);
+          cnt += popc64(m);
+          n_cands += (end - cur) < WAVE ? (end - cur) : WAVE;
+          if (HASH) {
+            uint64_t h = 0;
+            if (ok) {
+              for (uint32_t i = 0; i < k; ++i) {
+                uint32_t mv = (i == P->qv[L]) ? v : S.mq[i];
+                h = mix64(h + (uint64_t)mv + 1);
+              }
+            }
+            hsum += wave_sum(h);
+          }
+        }
+        n_bytes += 0;
+        --lv;
+        if (lv < 2) break;
+        continue;
+      }
+      uint32_t bi = S.bi[L], bn = S.bn[L];
+      if (bi < bn) {
+        uint32_t v = S.cand[L][bi];
+        S.bi[L] = bi + 1;
+        S.m[L] = v;
+        if (HASH) S.mq[P->qv[L]] = v;
+        ++lv;
+        if (!enter(L + 1, 0u, 0xFFFFFFFFu, -1)) --lv;
+        continue;
+      }
+      uint32_t cur = S.cur[L], end = S.end[L];
+      if (cur < end) {
+        uint32_t v;
+        bool ok = test(L, cur + lane, end, v);
+        uint64_t m = ballot(ok);
+        n_cands += (end - cur) < WAVE ? (end - cur) : WAVE;
+        uint32_t slot = popc64(m & ((1ull << lane) - 1ull));
+        if (ok) S.cand[L][slot] = v;
+        S.bn[L] = popc64(m);
+        S.bi[L] = 0;
+        S.cur[L] = cur + WAVE;
+        wave_sync();
+        continue;
+      }
+      --lv;
+      if (lv < 2) break;
+    }
+    if (lane == 0) {
+      if (cnt) atomicAdd(&a.counts[(size_t)t * a.nq + q], cnt);
+      if (HASH && hsum) atomicAdd(&a.hashes[(size_t)t * a.nq + q], hsum);
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&a.ctrl->stat_tasks, n_tasks);
+    atomicAdd(&a.ctrl->stat_cands, n_cands);
+    atomicAdd(&a.ctrl->stat_bytes, n_bytes + 4ull * n_cands);
+  }
+}
+
+
+// ------------------------------------------------------------ batch control
+__global__ void k_batch_begin(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    c->nseeds = 0;
+    c->nchunks = 0;
+    c->chunk_head = 0;
+    c->nruns = 0;
+    c->ndirty_next = 0;
+    c->err = 0;
+    c->err_at = 0xFFFFFFFFu;
+  }
+}
+
+__global__ void k_batch_end(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    c->ndirty_cur = c->ndirty_next;
+    c->ndirty_next = 0;
+  }
+}
+
+// Initial matching seeds: every directed data edge (v -> w) matching the first
+// query edge of plan `qo` (each embedding maps that query edge onto exactly
+// one directed data edge).  Vertices [v0, v1).
+__global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, uint32_t qo,
+                                uint32_t la, uint32_t key01, uint32_t v0, uint32_t v1,
+                                uint32_t chunk, Seed* seeds, uint32_t seed_cap,
+                                uint32_t* chunk_owner, uint32_t chunk_cap, Ctrl* ctrl,
+                                const uint32_t* __restrict__ d2, uint32_t d2_words, uint32_t symbi,
+                                unsigned long long* counts) {
+  const PlanRec* P = plans + qo;
+  for (uint32_t v = v0 + blockIdx.x * blockDim.x + threadIdx.x; v < v1;
+       v += gridDim.x * blockDim.x) {
+    if (g.vlab[v] != la) continue;
+    if (symbi && !d2_bit(d2, P->d2row[0], d2_words, v)) continue;
+    uint2 r = g.vrec[v];
+    uint2 sg = segment(g.adj, r.x, r.y & LEN_MASK, key01, g.vbits);
+    bool dirty = (r.y & DIRTY) != 0;
+    for (uint32_t i = sg.x; i < sg.y; ++i) {
+      if (dirty && !ts_visible(g.ats[i], 0xFFFFu)) continue;
+      uint32_t w = g.adj[i] & g.vmask;
+      if (symbi && !d2_bit(d2, P->d2row[1], d2_words, w)) continue;
+      if (P->k == 2) { atomicAdd(&counts[P->q], 1ull); continue; }
+      uint32_t best = NONE, blo = 0, bhi = 0, bj = 0;
+      for (uint32_t jj = 0; jj < P->nb[2]; ++jj) {
+        uint32_t bw = P->bw[2][jj];
+        uint32_t vx = (bw & 0xFFu) == 0 ? v : w;
+        uint2 rr = g.vrec[vx];
+        uint2 s2 = segment(g.adj, rr.x, rr.y & LEN_MASK, bw >> 8, g.vbits);
+        uint32_t sz = s2.y - s2.x;
+        if (best == NONE || sz < best) { best = sz; blo = s2.x; bhi = s2.y; bj = jj; }
+      }
+      if (best == 0 || best == NONE) continue;
+      uint32_t nch = (best + chunk - 1) / chunk;
+      uint32_t sidx = atomicAdd(&ctrl->nseeds, 1u);
+      if (sidx >= seed_cap) { atomicOr(&ctrl->err, ERR_SEED_OVERFLOW); continue; }
+      uint32_t base = atomicAdd(&ctrl->nchunks, nch);
+      if ((uint64_t)base + nch > chunk_cap) { atomicOr(&ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
+      Seed sd;
+      sd.t = 0; sd.qo = qo; sd.m0 = v; sd.m1 = w; sd.lo = blo; sd.hi = bhi; sd.base = base; sd.jmin = bj;
+      seeds[sidx] = sd;
+      for (uint32_t c = 0; c < nch; ++c) chunk_owner[base + c] = sidx;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- repack
+__global__ void k_repack_lens(const uint2* __restrict__ vrec, uint32_t* __restrict__ lens,
+                              uint32_t nv) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x)
+    lens[v] = vrec[v].y & LEN_MASK;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_repack_copy(uint2* __restrict__ vrec,
+                                                    const uint32_t* __restrict__ offs,
+                                                    const uint32_t* __restrict__ adj_in,
+                                                    const uint32_t* __restrict__ ats_in,
+                                                    uint32_t* __restrict__ adj_out,
+                                                    uint32_t* __restrict__ ats_out, uint32_t nv) {
+  for (uint32_t v = blockIdx.x; v < nv; v += gridDim.x) {
+    uint2 r = vrec[v];
+    uint32_t len = r.y & LEN_MASK, no = offs[v];
+    for (uint32_t i = threadIdx.x; i < len; i += BS) {
+      adj_out[no + i] = adj_in[r.x + i];
+      ats_out[no + i] = ats_in[r.x + i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) vrec[v].x = no;
+  }
+}
+
+}  // namespace csm

@@ -1,0 +1,162 @@
+// Host-side query planning: matching orders (GraphFlow) and the BFS DAG
+// (SymBi).  Plans stay on the host; the device receives flat PlanRec tables.
+#pragma once
+#include <stdint.h>
+
+#include <algorithm>
+#include <deque>
+#include <vector>
+
+namespace csm {
+
+// A small labelled query graph with sorted adjacency (like the reference
+// Graph, ParaCOSM/CSM/graph/graph.h:63-100, restricted to what planning needs).
+struct QueryGraph {
+  uint32_t k = 0;
+  size_t ne = 0;
+  std::vector<uint32_t> vl;                    // original labels
+  std::vector<std::vector<uint32_t>> nb, el;   // sorted neighbours, edge labels
+
+  void init(uint32_t n, const uint32_t* labels) {
+    k = n;
+    vl.assign(labels, labels + n);
+    nb.assign(n, {});
+    el.assign(n, {});
+    ne = 0;
+  }
+  bool add_edge(uint32_t a, uint32_t b, uint32_t l) {
+    if (a >= k || b >= k || a == b) return false;
+    auto it = std::lower_bound(nb[a].begin(), nb[a].end(), b);
+    if (it != nb[a].end() && *it == b) return false;
+    el[a].insert(el[a].begin() + (it - nb[a].begin()), l);
+    nb[a].insert(it, b);
+    it = std::lower_bound(nb[b].begin(), nb[b].end(), a);
+    el[b].insert(el[b].begin() + (it - nb[b].begin()), l);
+    nb[b].insert(it, a);
+    ++ne;
+    return true;
+  }
+  uint32_t deg(uint32_t u) const { return (uint32_t)nb[u].size(); }
+  uint32_t elabel(uint32_t a, uint32_t b) const {
+    auto it = std::lower_bound(nb[a].begin(), nb[a].end(), b);
+    if (it == nb[a].end() || *it != b) return UINT32_MAX;
+    return el[a][it - nb[a].begin()];
+  }
+  bool connected() const {
+    if (k == 0) return true;
+    std::vector<uint8_t> seen(k, 0);
+    std::deque<uint32_t> q{0};
+    seen[0] = 1;
+    uint32_t n = 1;
+    while (!q.empty()) {
+      uint32_t u = q.front();
+      q.pop_front();
+      for (uint32_t w : nb[u])
+        if (!seen[w]) { seen[w] = 1; ++n; q.push_back(w); }
+    }
+    return n == k;
+  }
+};
+
+// GraphFlow matching orders, one per query edge.  Same rule as the
+// reference (ParaCOSM/CSM/matching/GraphFlow/graphflow.cpp:36-160): order 0
+// starts at the highest-degree vertex; order i (i >= 1) starts with the i-th
+// backward edge of order 0; each next vertex has the most already-ordered
+// neighbours, ties going to the higher degree, then the lower id.
+inline std::vector<std::vector<uint32_t>> graphflow_orders(const QueryGraph& q) {
+  const uint32_t k = q.k;
+  std::vector<std::vector<uint32_t>> orders;
+  if (q.ne == 0) return orders;
+  auto next_vertex = [&](const std::vector<uint8_t>& placed) {
+    uint32_t best = UINT32_MAX, best_adj = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+      if (placed[j]) continue;
+      uint32_t adj = 0;
+      for (uint32_t w : q.nb[j]) adj += placed[w];
+      if (adj == 0) continue;
+      bool better = best == UINT32_MAX || adj > best_adj ||
+                    (adj == best_adj && q.deg(j) > q.deg(best));
+      if (better) { best = j; best_adj = adj; }
+    }
+    return best;
+  };
+  // order 0 and its backward-edge list (the query edges, enumerated)
+  std::vector<uint32_t> o0(k);
+  std::vector<std::pair<uint32_t, uint32_t>> edges;  // (vertex, backward neighbour)
+  {
+    std::vector<uint8_t> placed(k, 0);
+    uint32_t first = 0, maxd = 0;
+    for (uint32_t i = 0; i < k; ++i)
+      if (q.deg(i) > maxd) { maxd = q.deg(i); first = i; }
+    o0[0] = first;
+    placed[first] = 1;
+    for (uint32_t p = 1; p < k; ++p) {
+      uint32_t u = next_vertex(placed);
+      o0[p] = u;
+      placed[u] = 1;
+      for (uint32_t w : q.nb[u])
+        if (placed[w] && w != u) edges.push_back({u, w});
+    }
+  }
+  orders.push_back(o0);
+  for (size_t i = 1; i < q.ne; ++i) {
+    std::vector<uint32_t> o(k);
+    std::vector<uint8_t> placed(k, 0);
+    o[0] = edges[i].first;
+    o[1] = edges[i].second;
+    placed[o[0]] = placed[o[1]] = 1;
+    for (uint32_t p = 2; p < k; ++p) {
+      uint32_t u = next_vertex(placed);
+      o[p] = u;
+      placed[u] = 1;
+    }
+    orders.push_back(o);
+  }
+  return orders;
+}
+
+// SymBi's DAG: BFS from the root chosen by edge frequency (the rarest query
+// edge's less frequent endpoint; ParaCOSM/CSM/matching/SymBi/symbi.cpp:113-244).
+// `edge_freq(u, u2)` counts directed data edges whose labels match (u -> u2).
+struct Dag {
+  uint32_t root = 0;
+  std::vector<uint32_t> bfs;                       // serialized tree
+  std::vector<std::vector<uint32_t>> parents, children;
+};
+
+template <class EdgeFreq, class VertexFreq>
+Dag symbi_dag(const QueryGraph& q, EdgeFreq edge_freq, VertexFreq vertex_freq) {
+  Dag d;
+  const uint32_t k = q.k;
+  uint64_t min_f = UINT64_MAX;
+  uint32_t a0 = 0, a1 = 0;
+  for (uint32_t u = 0; u < k; ++u)
+    for (uint32_t w : q.nb[u]) {
+      if (u > w) continue;
+      uint64_t f = edge_freq(u, w);
+      if (f < min_f) { min_f = f; a0 = u; a1 = w; }
+    }
+  uint64_t f0 = vertex_freq(a0), f1 = vertex_freq(a1);
+  if (f0 > f1) d.root = a1;
+  else if (f0 < f1) d.root = a0;
+  else d.root = q.deg(a0) > q.deg(a1) ? a0 : a1;
+  d.parents.assign(k, {});
+  d.children.assign(k, {});
+  std::vector<uint8_t> done(k, 0), queued(k, 0);
+  std::deque<uint32_t> fifo{d.root};
+  queued[d.root] = 1;
+  d.bfs.push_back(d.root);
+  while (!fifo.empty()) {
+    uint32_t u = fifo.front();
+    fifo.pop_front();
+    done[u] = 1;
+    for (uint32_t w : q.nb[u]) {
+      if (!queued[w]) { queued[w] = 1; fifo.push_back(w); d.bfs.push_back(w); }
+      if (!done[w]) d.children[u].push_back(w);
+      else d.parents[u].push_back(w);
+    }
+  }
+  return d;
+}
+
+}  // namespace csm

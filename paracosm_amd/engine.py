@@ -1,0 +1,306 @@
+"""Python host mirror of the reference plugin interface over the C-ABI.
+
+`Matching` mirrors ``class matching`` (ParaCOSM/CSM/matching/matching.h:9-66):
+Preprocessing / InitialMatching / AddEdge / RemoveEdge / AddVertex /
+RemoveVertex / Classify / GetNum*Results / GetMemoryCost, with the same
+argument meaning.  Errors that make the reference print and ``exit(-1)``
+(deletion of an absent edge, unknown algorithm) raise :class:`CsmError`
+instead.  Everything runs in ``libcsm_mi355x.so`` on the GPU; there is no CPU
+fallback, and importing without the built library fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcsm_mi355x.so")
+
+CSM_OK = 0
+CSM_E_ARG = -1
+CSM_E_UNKNOWN_ALGO = -2
+CSM_E_DELETE_MISSING = -3
+CSM_E_DEVICE = -4
+CSM_E_CAPACITY = -5
+CSM_E_UNSUPPORTED = -6
+CSM_E_TIME_LIMIT = -7
+CSM_E_ADMISSION = -8
+
+ALGORITHMS = ("parallel_graphflow", "parallel_symbi", "parallel_turboflux",
+              "graphflow", "symbi", "turboflux")
+
+
+class CsmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class _Graph(ctypes.Structure):
+    _fields_ = [("num_vertices", ctypes.c_uint32), ("vlabels", ctypes.c_void_p),
+                ("num_edges", ctypes.c_size_t), ("src", ctypes.c_void_p),
+                ("dst", ctypes.c_void_p), ("elabels", ctypes.c_void_p)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("max_results", ctypes.c_uint64), ("print_prep", ctypes.c_uint32),
+                ("print_enum", ctypes.c_uint32), ("homo", ctypes.c_uint32),
+                ("threads", ctypes.c_uint32), ("auto_tuning", ctypes.c_uint32),
+                ("max_batch", ctypes.c_uint32), ("want_hash", ctypes.c_uint32),
+                ("time_limit_s", ctypes.c_double), ("device_budget", ctypes.c_uint64),
+                ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32)]
+
+
+UPDATE_DTYPE = np.dtype([("type", np.uint8), ("is_add", np.uint8), ("pad", np.uint16),
+                         ("id1", np.uint32), ("id2", np.uint32), ("label", np.uint32)])
+
+_lib = None
+
+
+def lib():
+    """The engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m paracosm_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t
+        P = ctypes.POINTER
+        sig = {
+            "csm_create": (vp, [ctypes.c_char_p, P(_Graph), P(_Graph), P(_Opts), P(ctypes.c_int)]),
+            "csm_register_query": (ctypes.c_int, [vp, P(_Graph), P(u32)]),
+            "csm_num_queries": (u32, [vp]),
+            "csm_preprocess": (ctypes.c_int, [vp]),
+            "csm_initial": (ctypes.c_int, [vp, P(u64)]),
+            "csm_classify": (ctypes.c_int, [vp, u32, u32, u32, P(ctypes.c_int)]),
+            "csm_add_edge": (ctypes.c_int, [vp, u32, u32, u32, P(u64)]),
+            "csm_remove_edge": (ctypes.c_int, [vp, u32, u32, P(u64)]),
+            "csm_add_vertex": (ctypes.c_int, [vp, u32, u32]),
+            "csm_remove_vertex": (ctypes.c_int, [vp, u32]),
+            "csm_apply_batch": (ctypes.c_int, [vp, vp, sz, vp, vp, vp, vp]),
+            "csm_apply_batch_device": (ctypes.c_int, [vp, vp, sz, vp, vp, vp]),
+            "csm_sync": (ctypes.c_int, [vp]),
+            "csm_totals": (ctypes.c_int, [vp, P(u64), P(u64), P(u64)]),
+            "csm_memory": (ctypes.c_int, [vp, P(sz), P(sz)]),
+            "csm_device_bytes": (ctypes.c_int, [vp, P(u64)]),
+            "csm_stats": (ctypes.c_int, [vp, P(u64), sz]),
+            "csm_stream": (vp, [vp]),
+            "csm_last_error": (ctypes.c_char_p, [vp]),
+            "csm_destroy": (None, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> Sequence[str]:
+    return ("csm_create", "csm_register_query", "csm_num_queries", "csm_preprocess",
+            "csm_initial", "csm_classify", "csm_add_edge", "csm_remove_edge",
+            "csm_add_vertex", "csm_remove_vertex", "csm_apply_batch",
+            "csm_apply_batch_device", "csm_sync", "csm_totals", "csm_memory",
+            "csm_device_bytes", "csm_stats", "csm_stream", "csm_last_error", "csm_destroy")
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+class _GraphArgs:
+    """Keeps the arrays alive while the C side reads them."""
+
+    def __init__(self, vlabels, src, dst, elabels):
+        self.vl, self.s, self.d, self.l = _u32(vlabels), _u32(src), _u32(dst), _u32(elabels)
+        self.g = _Graph(len(self.vl), self.vl.ctypes.data, len(self.s), self.s.ctypes.data,
+                        self.d.ctypes.data, self.l.ctypes.data)
+
+
+def make_updates(add, v1, v2, label, vertex=None) -> np.ndarray:
+    """Structured array of csm_update records (InsertUnit layout)."""
+    n = len(v1)
+    u = np.zeros(n, dtype=UPDATE_DTYPE)
+    u["type"] = ord("e")
+    if vertex is not None:
+        u["type"][np.asarray(vertex, bool)] = ord("v")
+    u["is_add"] = np.asarray(add, np.uint8)
+    u["id1"] = _u32(v1)
+    u["id2"] = _u32(v2)
+    u["label"] = _u32(label)
+    return u
+
+
+class Matching:
+    """One device engine: a data graph plus standing queries."""
+
+    def __init__(self, algo: str, query, vlabels, src, dst, elabels, *,
+                 max_results: int = 0, print_prep: bool = False, print_enum: bool = False,
+                 homo: bool = False, threads: int = 0, auto_tuning: int = 1,
+                 max_batch: int = 0, want_hash: bool = False, device_budget: int = 0,
+                 split_chunk: int = 0, device: int = 0):
+        L = lib()
+        self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
+        d = _GraphArgs(vlabels, src, dst, elabels)
+        o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
+                  auto_tuning, max_batch, int(want_hash), 0.0, device_budget, split_chunk,
+                  device)
+        err = ctypes.c_int(0)
+        self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
+                              ctypes.byref(o), ctypes.byref(err))
+        if not self.h:
+            if err.value == CSM_E_UNKNOWN_ALGO:
+                raise CsmError(err.value, "Unknown algorithm")
+            raise CsmError(err.value, f"csm_create failed for {algo!r}")
+        self.algo = algo
+        self.want_hash = want_hash
+
+    # -- helpers
+    def _check(self, rc: int) -> None:
+        if rc != CSM_OK:
+            raise CsmError(rc, lib().csm_last_error(self.h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().csm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- reference interface (matching.h:44-62)
+    def Preprocessing(self) -> None:
+        self._check(lib().csm_preprocess(self.h))
+
+    def InitialMatching(self) -> int:
+        n = ctypes.c_uint64(0)
+        self._check(lib().csm_initial(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def AddEdge(self, v1: int, v2: int, label: int) -> int:
+        n = ctypes.c_uint64(0)
+        self._check(lib().csm_add_edge(self.h, v1, v2, label, ctypes.byref(n)))
+        return int(n.value)
+
+    def RemoveEdge(self, v1: int, v2: int) -> int:
+        n = ctypes.c_uint64(0)
+        self._check(lib().csm_remove_edge(self.h, v1, v2, ctypes.byref(n)))
+        return int(n.value)
+
+    def AddVertex(self, vid: int, label: int) -> None:
+        self._check(lib().csm_add_vertex(self.h, vid, label))
+
+    def RemoveVertex(self, vid: int) -> None:
+        self._check(lib().csm_remove_vertex(self.h, vid))
+
+    def Classify(self, v1: int, v2: int, label: int) -> bool:
+        s = ctypes.c_int(0)
+        self._check(lib().csm_classify(self.h, v1, v2, label, ctypes.byref(s)))
+        return bool(s.value)
+
+    def GetNumInitialResults(self) -> int:
+        return self.totals()[0]
+
+    def GetNumPositiveResults(self) -> int:
+        return self.totals()[1]
+
+    def GetNumNegativeResults(self) -> int:
+        return self.totals()[2]
+
+    def GetMemoryCost(self):
+        e, v = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        self._check(lib().csm_memory(self.h, ctypes.byref(e), ctypes.byref(v)))
+        return int(e.value), int(v.value)
+
+    # -- extensions
+    def register_query(self, query) -> int:
+        g = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
+        qid = ctypes.c_uint32(0)
+        self._check(lib().csm_register_query(self.h, ctypes.byref(g.g), ctypes.byref(qid)))
+        return int(qid.value)
+
+    @property
+    def num_queries(self) -> int:
+        return int(lib().csm_num_queries(self.h))
+
+    def totals(self):
+        a, b, c = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._check(lib().csm_totals(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    def stats(self):
+        out = (ctypes.c_uint64 * 5)()
+        self._check(lib().csm_stats(self.h, out, 5))
+        return {"tasks": out[0], "candidates": out[1], "alg_bytes": out[2], "batches": out[3],
+                "repacks": out[4]}
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_uint64(0)
+        self._check(lib().csm_device_bytes(self.h, ctypes.byref(b)))
+        return int(b.value)
+
+    def stream(self) -> int:
+        return int(lib().csm_stream(self.h) or 0)
+
+    def apply_batch(self, updates: np.ndarray):
+        """Host arrays in, (pos, neg, safe, hash) out; pos/neg/hash are
+        [n, num_queries] uint64, safe is [n] uint8."""
+        u = np.ascontiguousarray(updates, dtype=UPDATE_DTYPE)
+        n, nq = len(u), self.num_queries
+        pos = np.zeros((n, nq), np.uint64)
+        neg = np.zeros((n, nq), np.uint64)
+        safe = np.zeros(n, np.uint8)
+        hsh = np.zeros((n, nq), np.uint64)
+        self._check(lib().csm_apply_batch(self.h, u.ctypes.data, n, pos.ctypes.data,
+                                          neg.ctypes.data, safe.ctypes.data,
+                                          hsh.ctypes.data if self.want_hash else None))
+        return pos, neg, safe, (hsh if self.want_hash else None)
+
+    def apply_batch_device(self, d_updates_ptr: int, n: int, d_counts_ptr: int,
+                           d_safe_ptr: int = 0, d_hash_ptr: int = 0) -> None:
+        """Device pointers (e.g. torch tensors' data_ptr()) in HBM; results
+        stay on the device.  Synchronous with respect to the engine's stream's
+        control read-back."""
+        self._check(lib().csm_apply_batch_device(self.h, d_updates_ptr, n, d_counts_ptr,
+                                                 d_safe_ptr or None, d_hash_ptr or None))
+
+    def sync(self) -> None:
+        self._check(lib().csm_sync(self.h))
+
+
+def run_stream(m: Matching, add, v1, v2, label, batch: Optional[int] = None):
+    """Apply a whole stream, batched (default: the engine's max batch) or one
+    update at a time (batch=1 through AddEdge/RemoveEdge).  Returns per-update
+    (pos, neg) summed over queries when batch == 1, else [n, nq] arrays."""
+    n = len(v1)
+    if batch == 1:
+        pos = np.zeros(n, np.uint64)
+        neg = np.zeros(n, np.uint64)
+        for i in range(n):
+            if add[i]:
+                pos[i] = m.AddEdge(int(v1[i]), int(v2[i]), int(label[i]))
+            else:
+                neg[i] = m.RemoveEdge(int(v1[i]), int(v2[i]))
+        return pos, neg
+    u = make_updates(add, v1, v2, label)
+    if batch is None:
+        pos, neg, _, _ = m.apply_batch(u)
+        return pos, neg
+    outs_p, outs_n = [], []
+    for s in range(0, n, batch):
+        p, q, _, _ = m.apply_batch(u[s:s + batch])
+        outs_p.append(p)
+        outs_n.append(q)
+    return np.concatenate(outs_p), np.concatenate(outs_n)

@@ -1,0 +1,211 @@
+"""GPU parity: the HIP engine through the C-ABI against the CPU oracle.
+
+Bit-exact per update and per query (counts and embedding-set hashes) on
+seeded inputs small enough for the oracle to finish in seconds.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import (mixed_stream_small, query_path2, query_star, query_triangle,
+                           random_connected_query, random_graph)
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(algo, q, vl, init, **kw):
+    from paracosm_amd.engine import Matching
+    m = Matching(algo, q, vl, init[0], init[1], init[2], **kw)
+    m.Preprocessing()
+    return m
+
+
+def _oracle(O, q, vl, init, ups, algo="graphflow", **kw):
+    return O.run(algo, O.OGraph(vl, *init), O.OGraph.from_query(q), *ups, **kw)
+
+
+def _case(seed, nv=40, p=0.25, n_vl=3, n_el=2, n_upd=120, k=None, extra=None):
+    rng = np.random.default_rng(seed)
+    vl, s, d, el = random_graph(nv, p, n_vl, n_el, rng)
+    init, ups = mixed_stream_small(s, d, el, nv, n_upd, 0.65, n_el, rng)
+    k = k or int(3 + seed % 4)
+    extra = seed % 3 if extra is None else extra
+    q = random_connected_query(k, extra, n_vl, n_el, rng)
+    return vl, init, ups, q
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi", "parallel_turboflux"])
+@pytest.mark.parametrize("seed", range(4))
+def test_batch_matches_oracle(oracle_lib, algo, seed):
+    vl, init, ups, q = _case(seed)
+    ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+    from paracosm_amd.engine import make_updates
+    with _engine(algo, q, vl, init, want_hash=True) as m:
+        pos, neg, safe, hsh = m.apply_batch(make_updates(*ups))
+    print("case", algo, seed, "k", q.k, "total", int(ref.pos.sum() + ref.neg.sum()), flush=True)
+    np.testing.assert_array_equal(pos[:, 0], ref.pos)
+    np.testing.assert_array_equal(neg[:, 0], ref.neg)
+    np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+
+
+@pytest.mark.parametrize("batch", [1, 5, 33])
+def test_batch_sizes_agree(oracle_lib, batch):
+    vl, init, ups, q = _case(11, nv=50, p=0.3, n_upd=150, k=5, extra=2)
+    ref = _oracle(oracle_lib, q, vl, init, ups)
+    from paracosm_amd.engine import run_stream
+    with _engine("parallel_graphflow", q, vl, init) as m:
+        pos, neg = run_stream(m, *ups, batch=batch)
+    pos = pos if pos.ndim == 1 else pos[:, 0]
+    neg = neg if neg.ndim == 1 else neg[:, 0]
+    np.testing.assert_array_equal(pos, ref.pos)
+    np.testing.assert_array_equal(neg, ref.neg)
+
+
+def test_plain_per_update_path(oracle_lib):
+    vl, init, ups, q = _case(5, nv=30, p=0.3, n_upd=60, k=4, extra=1)
+    ref = _oracle(oracle_lib, q, vl, init, ups)
+    with _engine("parallel_graphflow", q, vl, init) as m:
+        add, v1, v2, lab = ups
+        for i in range(len(v1)):
+            if add[i]:
+                assert m.AddEdge(int(v1[i]), int(v2[i]), int(lab[i])) == ref.pos[i], i
+            else:
+                assert m.RemoveEdge(int(v1[i]), int(v2[i])) == ref.neg[i], i
+        init_n, p, n = m.totals()
+        assert p == int(ref.pos.sum()) and n == int(ref.neg.sum())
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_initial_matching(oracle_lib, algo):
+    O = oracle_lib
+    for seed in range(3):
+        vl, init, ups, q = _case(20 + seed, nv=35, p=0.3, k=4 + seed % 2)
+        want = O.brute_count(O.OGraph(vl, *init), O.OGraph.from_query(q))
+        with _engine(algo, q, vl, init) as m:
+            assert m.InitialMatching() == want
+
+
+def test_closed_forms_on_complete_graph():
+    """Triangle query on a clique built up by insertions: pos = 6*common."""
+    from paracosm_amd.engine import make_updates
+    nv = 24
+    vl = np.zeros(nv, np.uint32)
+    pairs = [(a, b) for a in range(nv) for b in range(a + 1, nv)]
+    rng = np.random.default_rng(0)
+    rng.shuffle(pairs)
+    ups = make_updates(np.ones(len(pairs), np.uint8), [p[0] for p in pairs],
+                       [p[1] for p in pairs], np.zeros(len(pairs), np.uint32))
+    empty = (np.zeros(0, np.uint32),) * 3
+    with _engine("parallel_graphflow", query_triangle(), vl, empty) as m:
+        pos, _, _, _ = m.apply_batch(ups)
+    adj = [set() for _ in range(nv)]
+    for i, (a, b) in enumerate(pairs):
+        adj[a].add(b); adj[b].add(a)
+        assert pos[i, 0] == 6 * len(adj[a] & adj[b]), i
+
+
+def test_churn_within_one_batch(oracle_lib):
+    """The same edges deleted and re-inserted many times inside one batch:
+    each update must see exactly the graph before it."""
+    rng = np.random.default_rng(3)
+    nv = 20
+    vl, s, d, el = random_graph(nv, 0.5, 2, 1, rng)
+    init = (s, d, el)
+    hot = [(int(s[i]), int(d[i]), int(el[i])) for i in rng.choice(len(s), 6, replace=False)]
+    add, v1, v2, lab = [], [], [], []
+    present = {(a, b): True for a, b, _ in hot}
+    for _ in range(200):
+        a, b, l = hot[int(rng.integers(0, len(hot)))]
+        if present[(a, b)]:
+            add.append(0); present[(a, b)] = False
+        else:
+            add.append(1); present[(a, b)] = True
+        if rng.random() < 0.5:
+            a, b = b, a
+        v1.append(a); v2.append(b); lab.append(l)
+    ups = (np.asarray(add, np.uint8), np.asarray(v1, np.uint32), np.asarray(v2, np.uint32),
+           np.asarray(lab, np.uint32))
+    q = query_star(3)
+    q.vlabels[:] = 0
+    vl[:] = 0
+    ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+    from paracosm_amd.engine import make_updates
+    with _engine("parallel_graphflow", q, vl, init, want_hash=True) as m:
+        pos, neg, _, hsh = m.apply_batch(make_updates(*ups))
+    np.testing.assert_array_equal(pos[:, 0], ref.pos)
+    np.testing.assert_array_equal(neg[:, 0], ref.neg)
+    np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+
+
+def test_duplicate_insert_follows_reference(oracle_lib):
+    """Inserting an edge that exists leaves the graph unchanged but is still
+    searched (graph.cpp:63-64, graphflow.cpp:296-352)."""
+    vl = np.zeros(6, np.uint32)
+    init = (np.asarray([0, 1, 2, 0], np.uint32), np.asarray([1, 2, 3, 2], np.uint32),
+            np.zeros(4, np.uint32))
+    ups = (np.asarray([1, 1, 0, 1, 1], np.uint8), np.asarray([0, 1, 0, 2, 0], np.uint32),
+           np.asarray([1, 0, 1, 3, 1], np.uint32), np.zeros(5, np.uint32))
+    q = query_path2()
+    ref = _oracle(oracle_lib, q, vl, init, ups)
+    from paracosm_amd.engine import make_updates
+    with _engine("parallel_graphflow", q, vl, init) as m:
+        pos, neg, _, _ = m.apply_batch(make_updates(*ups))
+    np.testing.assert_array_equal(pos[:, 0], ref.pos)
+    np.testing.assert_array_equal(neg[:, 0], ref.neg)
+
+
+def test_delete_missing_edge_is_an_error():
+    from paracosm_amd.engine import CSM_E_DELETE_MISSING, CsmError, make_updates
+    vl = np.zeros(5, np.uint32)
+    init = (np.asarray([0, 1], np.uint32), np.asarray([1, 2], np.uint32), np.zeros(2, np.uint32))
+    with _engine("parallel_graphflow", query_path2(), vl, init) as m:
+        with pytest.raises(CsmError) as ei:
+            m.RemoveEdge(3, 4)
+        assert ei.value.code == CSM_E_DELETE_MISSING
+    with _engine("parallel_graphflow", query_path2(), vl, init) as m:
+        with pytest.raises(CsmError) as ei:
+            m.apply_batch(make_updates([1, 0, 0], [2, 0, 0], [3, 1, 1], [0, 0, 0]))
+        assert ei.value.code == CSM_E_DELETE_MISSING
+
+
+def test_multi_query_columns(oracle_lib):
+    vl, init, ups, q0 = _case(7, nv=40, p=0.3, n_upd=100, k=4)
+    rng = np.random.default_rng(77)
+    qs = [q0] + [random_connected_query(k, 1, 3, 2, rng) for k in (3, 5, 6)]
+    from paracosm_amd.engine import make_updates
+    with _engine("parallel_graphflow", q0, vl, init) as m:
+        for q in qs[1:]:
+            m.register_query(q)
+        pos, neg, _, _ = m.apply_batch(make_updates(*ups))
+    for j, q in enumerate(qs):
+        ref = _oracle(oracle_lib, q, vl, init, ups)
+        np.testing.assert_array_equal(pos[:, j], ref.pos, err_msg=f"query {j}")
+        np.testing.assert_array_equal(neg[:, j], ref.neg, err_msg=f"query {j}")
+
+
+def test_classify_matches_oracle(oracle_lib):
+    O = oracle_lib
+    vl, init, ups, q = _case(9)
+    qg = O.OGraph.from_query(q)
+    with _engine("parallel_graphflow", q, vl, init) as m:
+        add, v1, v2, lab = ups
+        for i in range(len(v1)):
+            want = O.classify(qg, int(vl[v1[i]]), int(vl[v2[i]]), int(lab[i]))
+            assert m.Classify(int(v1[i]), int(v2[i]), int(lab[i])) == want
+
+
+def test_rmat_config1_slice(oracle_lib):
+    """Config-1 shape (R-MAT, 16 uniform labels, 1 edge label, 6-vertex
+    random-walk queries, insertion stream) at scale 12."""
+    from paracosm_amd import gen
+    from paracosm_amd.engine import make_updates
+    w = gen.make_workload(12, 8, 16, "uniform", 1, stream="insert", frac_stream=0.1,
+                          query_sizes=(6,), queries_per_size=3, seed=4)
+    ups = (w.upd_add, w.upd_v1, w.upd_v2, w.upd_label)
+    init = (w.init_src, w.init_dst, w.init_elab)
+    for qi, q in enumerate(w.queries):
+        ref = _oracle(oracle_lib, q, w.vlabels, init, ups)
+        for algo in ("parallel_graphflow", "parallel_symbi"):
+            with _engine(algo, q, w.vlabels, init) as m:
+                pos, neg, _, _ = m.apply_batch(make_updates(*ups))
+            print("rmat12 query", qi, algo, int(ref.pos.sum()), flush=True)
+            np.testing.assert_array_equal(pos[:, 0], ref.pos)

@@ -1,0 +1,209 @@
+"""Pins the CPU oracle against what the mathematics fixes.
+
+The reference ships no tests, fixtures or golden vectors for this path, and it
+cannot be compiled here without stand-in headers, so the oracle is pinned by:
+closed forms (triangles, 2-paths, stars), brute-force enumeration on tiny
+graphs (pos = |Emb(G+e)| - |Emb(G)|, neg = |Emb(G)| - |Emb(G-e)|, and the same
+for the embedding-set hash), the telescoping identity against
+InitialMatching, agreement of three independent search formulations
+(GraphFlow, SymBi, the parallel GraphFlow shape), the SymBi index fix-point
+(incremental maintenance == from-scratch BuildDCS after every update), and the
+label-triple filter checked exhaustively.
+"""
+import itertools
+from math import perm
+
+import numpy as np
+import pytest
+
+from tests.helpers import (mixed_stream_small, query_path2, query_star, query_triangle,
+                           random_connected_query, random_graph)
+
+
+def _adj_after(nv, init, ups, t):
+    """Adjacency sets after applying updates [0, t] (inclusive)."""
+    adj = [set() for _ in range(nv)]
+    for a, b in zip(init[0], init[1]):
+        adj[a].add(int(b)); adj[b].add(int(a))
+    add, v1, v2, _ = ups
+    for i in range(t + 1):
+        a, b = int(v1[i]), int(v2[i])
+        if add[i]:
+            adj[a].add(b); adj[b].add(a)
+        else:
+            adj[a].discard(b); adj[b].discard(a)
+    return adj
+
+
+def _stream_on_complete_graph(nv, n_upd, seed):
+    rng = np.random.default_rng(seed)
+    vl, s, d, el = random_graph(nv, 1.0, 1, 1, rng)
+    return vl, mixed_stream_small(s, d, el, nv, n_upd, 0.6, 1, rng)
+
+
+@pytest.mark.parametrize("algo", ["graphflow", "symbi", "parallel_graphflow"])
+def test_closed_form_triangle(oracle_lib, algo):
+    O = oracle_lib
+    nv = 14
+    vl, (init, ups) = _stream_on_complete_graph(nv, 60, 1)
+    res = O.run(algo, O.OGraph(vl, *init), O.OGraph.from_query(query_triangle()), *ups, threads=3)
+    add, v1, v2, _ = ups
+    for t in range(len(v1)):
+        # state the search sees: G+e for inserts, G (before removal) for deletes
+        adj = _adj_after(nv, init, ups, t if add[t] else t - 1)
+        common = len(adj[int(v1[t])] & adj[int(v2[t])])
+        got = res.pos[t] if add[t] else res.neg[t]
+        assert got == 6 * common, (t, algo)
+
+
+@pytest.mark.parametrize("algo", ["graphflow", "symbi", "parallel_graphflow"])
+def test_closed_form_path_and_star(oracle_lib, algo):
+    O = oracle_lib
+    nv = 12
+    vl, (init, ups) = _stream_on_complete_graph(nv, 50, 2)
+    add, v1, v2, _ = ups
+    for q, s in ((query_path2(), 2), (query_star(3), 3), (query_star(4), 4)):
+        res = O.run(algo, O.OGraph(vl, *init), O.OGraph.from_query(q), *ups, threads=2)
+        for t in range(len(v1)):
+            adj = _adj_after(nv, init, ups, t if add[t] else t - 1)
+            da, db = len(adj[int(v1[t])]), len(adj[int(v2[t])])
+            # a star with s leaves: the updated edge is a centre-leaf edge,
+            # centre on either endpoint, remaining s-1 leaves arranged injectively
+            want = s * (perm(da - 1, s - 1) + perm(db - 1, s - 1))
+            got = res.pos[t] if add[t] else res.neg[t]
+            assert got == want, (t, algo, s)
+
+
+def _brute_series(O, nv, vl, init, ups, q, homo=False):
+    """|Emb(G_t)| and hash for t = -1 (initial) .. n-1 by brute force."""
+    add, v1, v2, lab = ups
+    edges = {(min(a, b), max(a, b)): int(l) for a, b, l in zip(*init)}
+    qg = O.OGraph.from_query(q)
+
+    def count():
+        ks = sorted(edges)
+        g = O.OGraph(vl, [k[0] for k in ks], [k[1] for k in ks], [edges[k] for k in ks])
+        return O.brute_count(g, qg, homo=homo, want_hash=True)
+
+    out = [count()]
+    for i in range(len(v1)):
+        key = (min(int(v1[i]), int(v2[i])), max(int(v1[i]), int(v2[i])))
+        if add[i]:
+            edges[key] = int(lab[i])
+        else:
+            del edges[key]
+        out.append(count())
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_brute_force_random(oracle_lib, seed):
+    O = oracle_lib
+    rng = np.random.default_rng(100 + seed)
+    nv = 16
+    n_vl, n_el = 2, 2
+    vl, s, d, el = random_graph(nv, 0.45, n_vl, n_el, rng)
+    init, ups = mixed_stream_small(s, d, el, nv, 30, 0.6, n_el, rng)
+    k = 3 + seed % 3
+    q = random_connected_query(k, seed % 3, n_vl, n_el, rng)
+    series = _brute_series(O, nv, vl, init, ups, q)
+    add = ups[0]
+    results = {}
+    for algo in ("graphflow", "symbi", "parallel_graphflow"):
+        r = O.run(algo, O.OGraph(vl, *init), O.OGraph.from_query(q), *ups, threads=3,
+                  want_hash=algo != "parallel_graphflow", want_initial=True,
+                  check_index=algo == "symbi")
+        results[algo] = r
+        assert r.initial == series[0][0], algo
+        for t in range(len(add)):
+            before, after = series[t], series[t + 1]
+            if add[t]:
+                assert r.pos[t] == after[0] - before[0] and r.neg[t] == 0, (algo, t)
+                if r.hash is not None:
+                    assert int(r.hash[t]) == (after[1] - before[1]) % (1 << 64), (algo, t)
+            else:
+                assert r.neg[t] == before[0] - after[0] and r.pos[t] == 0, (algo, t)
+                if r.hash is not None:
+                    assert int(r.hash[t]) == (before[1] - after[1]) % (1 << 64), (algo, t)
+    # telescoping: sum(pos) - sum(neg) == final - initial
+    r = results["graphflow"]
+    assert int(r.pos.sum()) - int(r.neg.sum()) == series[-1][0] - series[0][0]
+
+
+def test_homomorphism_closed_form(oracle_lib):
+    """--homo drops the visited check (graphflow.cpp:241).  Each seed (query
+    edge, orientation) then extends over every neighbour, so for a 2-path the
+    per-update sum over seeds is 2*deg(a) + 2*deg(b) in closed form."""
+    O = oracle_lib
+    nv = 10
+    vl = np.zeros(nv, np.uint32)
+    src, dst = [], []
+    for a in range(0, nv, 2):
+        for b in range(1, nv, 2):
+            src.append(a); dst.append(b)
+    el = np.zeros(len(src), np.uint32)
+    rng = np.random.default_rng(7)
+    init, ups = mixed_stream_small(np.asarray(src, np.uint32), np.asarray(dst, np.uint32),
+                                   el, nv, 20, 0.6, 1, rng)
+    q = query_path2()
+    r = O.run("graphflow", O.OGraph(vl, *init), O.OGraph.from_query(q), *ups, homo=True)
+    add, v1, v2, _ = ups
+    for t in range(len(v1)):
+        adj = _adj_after(nv, init, ups, t if add[t] else t - 1)
+        da, db = len(adj[int(v1[t])]), len(adj[int(v2[t])])
+        # homomorphisms of a 2-path using e: the other edge may reuse e's endpoints
+        want = 2 * da + 2 * db
+        got = r.pos[t] if add[t] else r.neg[t]
+        assert got == want, t
+
+
+def test_classify_exhaustive(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        q = random_connected_query(5, 2, 3, 2, rng)
+        qg = O.OGraph.from_query(q)
+        qedges = {(int(q.vlabels[a]), int(q.vlabels[b]), int(l)) for a, b, l in
+                  zip(q.src, q.dst, q.elabels)}
+        qedges |= {(b, a, l) for a, b, l in qedges}
+        for l1, l2, el in itertools.product(range(4), range(4), range(3)):
+            assert O.classify(qg, l1, l2, el) == ((l1, l2, el) not in qedges)
+
+
+def test_safe_updates_have_no_matches(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(11)
+    nv = 18
+    vl, s, d, el = random_graph(nv, 0.4, 3, 2, rng)
+    init, ups = mixed_stream_small(s, d, el, nv, 40, 0.7, 2, rng)
+    q = random_connected_query(4, 1, 3, 2, rng)
+    qg = O.OGraph.from_query(q)
+    r = O.run("graphflow", O.OGraph(vl, *init), qg, *ups)
+    add, v1, v2, lab = ups
+    for t in range(len(v1)):
+        if O.classify(qg, int(vl[v1[t]]), int(vl[v2[t]]), int(lab[t])):
+            assert r.pos[t] == 0 and r.neg[t] == 0
+
+
+def test_deletion_of_absent_edge_is_an_error(oracle_lib):
+    O = oracle_lib
+    vl = np.zeros(4, np.uint32)
+    g = O.OGraph(vl, [0, 1], [1, 2], [0, 0])
+    q = O.OGraph.from_query(query_path2())
+    with pytest.raises(RuntimeError, match="deletion error at update 1"):
+        O.run("graphflow", g, q, [1, 0], [2, 0], [3, 3], [0, 0])
+
+
+def test_graphflow_orders_cover_every_query_edge(oracle_lib):
+    """Order i starts on query edge i (graphflow.cpp:91-112): the first two
+    vertices of the |E_q| orders enumerate every query edge exactly once."""
+    O = oracle_lib
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        q = random_connected_query(6, 3, 2, 1, rng)
+        g = O.OGraph(np.zeros(2, np.uint32), [0], [1], [0])
+        orders = O.graphflow_orders(g, O.OGraph.from_query(q), q.k, q.num_edges)
+        firsts = sorted(tuple(sorted((int(o[0]), int(o[1])))) for o in orders)
+        assert firsts == sorted(zip(q.src.tolist(), q.dst.tolist()))
+        for o in orders:
+            assert sorted(o.tolist()) == list(range(q.k))
