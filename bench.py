@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark of the incremental match search on one MI355X.
+
+Workload: scale-20 R-MAT (1,048,576 vertices, edge factor 16), Zipf(1.0)
+vertex labels over 32, 2 uniform edge labels; 1,000,000 updates (80 %
+insertions of held-out edges, 20 % deletions of present edges); 10 standing
+8-vertex random-walk queries (5 sparse, 5 dense) admitted by the exact-count
+rule of paracosm_amd.workloads.  One step = one batch of updates through the
+whole device pipeline (graph update, seeds, search) for all queries; the
+update records are resident in HBM before the timed region starts.
+
+Prints one JSON line: batched updates/s (value) beside the plain per-update
+GPU path (one update per launch sequence), per-update latency percentiles,
+the search kernel's roofline figures, and the CPU oracle's parallel GraphFlow
+restatement timed on a bounded sample on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "updates/s + p99 update latency, served vs per-update GPU path, exact counts"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(xs, p):
+    return float(np.percentile(np.asarray(xs, np.float64), p)) if len(xs) else None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=15)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=62500)
+    ap.add_argument("--algo", default="parallel_graphflow")
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=None, help="override the R-MAT scale (testing)")
+    ap.add_argument("--plain-sample", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--split-chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        return 2
+
+    import torch
+    from paracosm_amd import workloads
+    from paracosm_amd.engine import Matching, make_updates, lib
+
+    lib()  # fail loudly if the engine is not built
+    t0 = time.time()
+    d = workloads.build(args.config, seed=args.seed, scale=args.scale)
+    w = d.w
+    log(f"workload {args.config}: {w.nv} vertices, {len(w.init_src)} initial edges, "
+        f"{w.num_updates} updates ({int(w.upd_add.sum())} insertions) in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    queries = workloads.admit(d, log=log)
+    log(f"admitted {len(queries)} queries in {time.time() - t0:.1f}s: "
+        f"k={[q.k for q in queries]} edges={[q.num_edges for q in queries]}")
+
+    n_total = min(w.num_updates, (args.warmup + args.steps) * args.batch)
+    ups = make_updates(w.upd_add[:n_total], w.upd_v1[:n_total], w.upd_v2[:n_total],
+                       w.upd_label[:n_total])
+    dev = torch.device("cuda:0")
+    d_ups = torch.from_numpy(ups.view(np.uint8).copy()).to(dev)
+    nq = len(queries)
+    d_counts = torch.zeros(n_total * nq, dtype=torch.int64, device=dev)
+
+    m = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                 max_batch=args.batch, split_chunk=args.split_chunk)
+    for q in queries[1:]:
+        m.register_query(q)
+    m.Preprocessing()
+    rec = 16
+
+    def step(i):
+        s = i * args.batch
+        n = min(args.batch, n_total - s)
+        m.apply_batch_device(d_ups.data_ptr() + s * rec, n, d_counts.data_ptr() + s * nq * 8)
+        return n
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    m.sync()
+    st0 = m.stats()
+    lat = []
+    n_done = 0
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.warmup, args.warmup + args.steps):
+        tb = time.perf_counter()
+        n_done += step(i)
+        lat.append(time.perf_counter() - tb)
+    m.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    st1 = m.stats()
+    counts = d_counts.cpu().numpy().astype(np.uint64).reshape(n_total, nq)
+    timed = counts[args.warmup * args.batch: args.warmup * args.batch + n_done]
+    matches = int(timed.sum())
+    value = n_done / elapsed
+    log(f"batched: {n_done} updates in {elapsed:.3f}s = {value:.0f} updates/s, "
+        f"{matches} matches ({matches / elapsed:.3e}/s)")
+    m.close()
+
+    # plain per-update GPU path on the stream's first updates (fresh engine)
+    P = min(args.plain_sample, n_total)
+    mp = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    for q in queries[1:]:
+        mp.register_query(q)
+    mp.Preprocessing()
+    plat = []
+    pcounts = np.zeros(P, np.uint64)
+    tp0 = time.perf_counter()
+    for i in range(P):
+        ta = time.perf_counter()
+        if w.upd_add[i]:
+            pcounts[i] = mp.AddEdge(int(w.upd_v1[i]), int(w.upd_v2[i]), int(w.upd_label[i]))
+        else:
+            pcounts[i] = mp.RemoveEdge(int(w.upd_v1[i]), int(w.upd_v2[i]))
+        plat.append(time.perf_counter() - ta)
+    plain_elapsed = time.perf_counter() - tp0
+    mp.close()
+    plain_ups = P / plain_elapsed
+    exact = bool(np.array_equal(pcounts, counts[:P].sum(axis=1)))
+    log(f"plain per-update: {P} updates in {plain_elapsed:.3f}s = {plain_ups:.0f} updates/s; "
+        f"counts equal to batched: {exact}")
+
+    # roofline of the search kernel: algorithmic bytes per launch over its
+    # average launch duration (HIP events on the engine stream)
+    launches = st1["search_launches"] - st0["search_launches"]
+    k_ns = st1["search_ns"] - st0["search_ns"]
+    alg = st1["alg_bytes"] - st0["alg_bytes"]
+    per_launch_bytes = alg / max(launches, 1)
+    avg_ns = k_ns / max(launches, 1)
+    achieved = (per_launch_bytes / avg_ns) if avg_ns else 0.0  # bytes/ns == GB/s
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "kernel": "k_search", "avg_launch_us": round(avg_ns / 1e3, 2),
+            "alg_bytes_per_launch": int(per_launch_bytes), "launches": int(launches),
+            "kernel_share_of_step": round(k_ns / 1e9 / elapsed, 4) if elapsed else None}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, queries, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": d.cfg.description, "config": args.config, "seed": args.seed,
+                   "algo": args.algo, "batch": args.batch, "queries": nq,
+                   "query_fingerprint": workloads.query_fingerprint(queries),
+                   "parallelism": "single-gpu"},
+        "p50_update_latency_ms": round(1e3 * pct(lat, 50), 3),
+        "p99_update_latency_ms": round(1e3 * pct(lat, 99), 3),
+        "latency_note": "closed loop: each update's latency is its batch's latency",
+        "plain_updates_per_s": round(plain_ups, 1),
+        "plain_p50_update_latency_ms": round(1e3 * pct(plat, 50), 4),
+        "plain_p99_update_latency_ms": round(1e3 * pct(plat, 99), 4),
+        "speedup_vs_plain": round(value / plain_ups, 2),
+        "matches_per_s": round(matches / elapsed, 1),
+        "matches": matches,
+        "exact_counts_plain_vs_batched": exact,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(w, queries, seconds):
+    """The oracle's restatement of the reference's parallel GraphFlow
+    (serial depth-2 expansion, depth-3 task vector, OpenMP loop), all cores,
+    on the first S updates of the same stream for every query."""
+    from oracle import oracle as O
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    t0 = time.time()
+    dg = O.OGraph(w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    log(f"cpu baseline: oracle graph built in {time.time() - t0:.1f}s")
+    qgs = [O.OGraph.from_query(q) for q in queries]
+    args = (w.upd_add, w.upd_v1, w.upd_v2, w.upd_label)
+    s = 100
+    while True:
+        tot = 0.0
+        for qg in qgs:
+            r = O.run("parallel_graphflow", dg, qg, *args, threads=cores, limit=s)
+            tot += r.seconds
+        log(f"cpu baseline: {s} updates x {len(qgs)} queries took {tot:.2f}s")
+        if tot * 3 >= seconds or s >= w.num_updates:
+            break
+        s = int(min(w.num_updates, s * max(2.0, 0.8 * seconds / max(tot, 1e-3))))
+    log(f"cpu baseline: {s} updates x {len(qgs)} queries in {tot:.2f}s on {cores} threads")
+    return {"value": round(s / tot, 1), "unit": "updates/s", "cores": cores, "kind": "port",
+            "sample": f"first {s} updates of the stream, all {len(qgs)} queries, "
+                      f"oracle parallel_graphflow, {cores} OpenMP threads"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -142,9 +142,11 @@ int csm_memory(const csm_engine* e, size_t* num_edges, size_t* num_vertices);
 /* Device bytes currently held by the engine (admission control). */
 int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
 
-/* Engine statistics: [0] DFS tasks run, [1] candidate entries tested,
- * [2] algorithmic bytes of the search (4 B per candidate entry scanned plus
- * 4 B per partner entry probed), [3] batches, [4] pool repacks. */
+/* Engine statistics (cumulative): [0] search tasks run, [1] candidate
+ * entries tested, [2] algorithmic bytes of the search (4 B per candidate
+ * entry scanned plus 4 B per entry of every partner segment, merge model),
+ * [3] batches, [4] pool repacks, [5] search-kernel time in ns (HIP events on
+ * the engine stream), [6] search launches timed. */
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
 
 /* The HIP stream the engine launches on (hipStream_t). */

@@ -850,11 +850,42 @@ struct SymBi {
   }
 };
 
+// Same result as calling Graph::AddEdge for every edge in order
+// (graph.cpp:217-249): sorted lists, the first copy of a duplicate pair wins.
 Graph make_graph(uint32_t nv, const uint32_t* vl, size_t ne, const uint32_t* s, const uint32_t* t,
                  const uint32_t* l) {
   Graph g;
   for (uint32_t i = 0; i < nv; ++i) g.add_vertex(i, vl[i]);
-  for (size_t i = 0; i < ne; ++i) g.add_edge(s[i], t[i], l[i]);
+  std::vector<uint64_t> cnt(nv + 1, 0);
+  for (size_t i = 0; i < ne; ++i) {
+    if (s[i] == t[i]) continue;
+    ++cnt[s[i]];
+    ++cnt[t[i]];
+  }
+  std::vector<std::vector<std::pair<uint32_t, size_t>>> tmp(nv);
+  for (uint32_t v = 0; v < nv; ++v) tmp[v].reserve(cnt[v]);
+  for (size_t i = 0; i < ne; ++i) {
+    if (s[i] == t[i]) continue;
+    tmp[s[i]].push_back({t[i], i});
+    tmp[t[i]].push_back({s[i], i});
+  }
+  size_t m2 = 0;
+  for (uint32_t v = 0; v < nv; ++v) {
+    auto& a = tmp[v];
+    std::sort(a.begin(), a.end());
+    auto& nb = g.nb[v];
+    auto& el = g.el[v];
+    nb.reserve(a.size());
+    el.reserve(a.size());
+    for (size_t j = 0; j < a.size(); ++j) {
+      if (j > 0 && a[j].first == a[j - 1].first) continue;
+      nb.push_back(a[j].first);
+      el.push_back(l[a[j].second]);
+    }
+    m2 += nb.size();
+    std::vector<std::pair<uint32_t, size_t>>().swap(a);
+  }
+  g.ne = m2 / 2;
   return g;
 }
 

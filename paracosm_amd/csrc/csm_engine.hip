@@ -124,7 +124,10 @@ struct csm_engine {
 
   // totals and stats
   uint64_t tot_init = 0, tot_pos = 0, tot_neg = 0;
-  uint64_t stats[5] = {0, 0, 0, 0, 0};
+  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // HIP events around search launches on the engine stream (kernel time)
+  hipEvent_t ev_search[2] = {nullptr, nullptr};
+  bool ev_pending = false;
   uint64_t dev_bytes = 0;
 
   std::vector<DevBuf> allocs;
@@ -523,6 +526,7 @@ void search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned 
   sa.pos_offset = 0;
   const bool h = hashes != nullptr;
   const bool s = e->algo == ALGO_SYMBI && e->d_d2 != nullptr;
+  if (e->ev_search[0]) (void)hipEventRecord(e->ev_search[0], e->stream);
   if (versioned) {
     if (h) { if (s) launch_search<true, true, true>(e, sa); else launch_search<true, true, false>(e, sa); }
     else { if (s) launch_search<true, false, true>(e, sa); else launch_search<true, false, false>(e, sa); }
@@ -530,6 +534,21 @@ void search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned 
     if (h) { if (s) launch_search<false, true, true>(e, sa); else launch_search<false, true, false>(e, sa); }
     else { if (s) launch_search<false, false, true>(e, sa); else launch_search<false, false, false>(e, sa); }
   }
+  if (e->ev_search[1]) {
+    (void)hipEventRecord(e->ev_search[1], e->stream);
+    e->ev_pending = true;
+  }
+}
+
+// after a stream sync: fold the last search launch's duration into the stats
+void collect_search_time(csm_engine* e) {
+  if (!e->ev_pending) return;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, e->ev_search[0], e->ev_search[1]) == hipSuccess) {
+    e->stats[5] += (uint64_t)(ms * 1e6);
+    e->stats[6] += 1;
+  }
+  e->ev_pending = false;
 }
 
 void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
@@ -636,6 +655,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
 }
 
 void accumulate_stats(csm_engine* e) {
+  collect_search_time(e);
   e->stats[0] = e->h_ctrl->stat_tasks;
   e->stats[1] = e->h_ctrl->stat_cands;
   e->stats[2] = e->h_ctrl->stat_bytes;
@@ -728,6 +748,10 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   e->num_cus = prop.multiProcessorCount;
   if (e->opts.device_budget == 0) e->opts.device_budget = (uint64_t)(prop.totalGlobalMem * 0.9);
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  if (hipEventCreate(&e->ev_search[0]) != hipSuccess || hipEventCreate(&e->ev_search[1]) != hipSuccess) {
+    set(CSM_E_DEVICE);
+    return nullptr;
+  }
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search<true, false, false>,
                                                    SEARCH_WAVES * WAVE, 0) != hipSuccess || per_cu < 1)
@@ -1027,7 +1051,7 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes) {
 
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
   if (!e || !out) return CSM_E_ARG;
-  for (size_t i = 0; i < n_out && i < 5; ++i) out[i] = e->stats[i];
+  for (size_t i = 0; i < n_out && i < 8; ++i) out[i] = e->stats[i];
   return CSM_OK;
 }
 
@@ -1046,6 +1070,8 @@ void csm_destroy(csm_engine* e) {
   if (e->h_hashes) (void)hipHostFree(e->h_hashes);
   if (e->h_safe) (void)hipHostFree(e->h_safe);
   if (e->h_upd) (void)hipHostFree(e->h_upd);
+  for (auto& ev : e->ev_search)
+    if (ev) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

@@ -241,10 +241,10 @@ class Matching:
         return int(a.value), int(b.value), int(c.value)
 
     def stats(self):
-        out = (ctypes.c_uint64 * 5)()
-        self._check(lib().csm_stats(self.h, out, 5))
+        out = (ctypes.c_uint64 * 7)()
+        self._check(lib().csm_stats(self.h, out, 7))
         return {"tasks": out[0], "candidates": out[1], "alg_bytes": out[2], "batches": out[3],
-                "repacks": out[4]}
+                "repacks": out[4], "search_ns": out[5], "search_launches": out[6]}
 
     def device_bytes(self) -> int:
         b = ctypes.c_uint64(0)

@@ -118,14 +118,13 @@ class Workload:
 
 def build_csr(nv: int, src: np.ndarray, dst: np.ndarray, elab: np.ndarray):
     """Symmetric CSR (offsets, neighbours sorted by id, parallel edge labels)."""
-    s = np.concatenate([src, dst]).astype(np.int64)
-    d = np.concatenate([dst, src]).astype(np.int64)
+    s = np.concatenate([src, dst]).astype(np.uint64)
+    d = np.concatenate([dst, src]).astype(np.uint64)
     e = np.concatenate([elab, elab])
-    order = np.lexsort((d, s))
+    order = np.argsort((s << np.uint64(32)) | d)
     s, d, e = s[order], d[order], e[order]
     off = np.zeros(nv + 1, dtype=np.int64)
-    np.add.at(off, s + 1, 1)
-    off = np.cumsum(off)
+    off[1:] = np.cumsum(np.bincount(s.astype(np.int64), minlength=nv))
     return off, d.astype(np.uint32), e.astype(np.uint32)
 
 
@@ -224,25 +223,20 @@ def mixed_stream(src, dst, elab, n_updates: int, p_insert: float, seed: int,
     init = perm[: m - n_holdout]
     coins = rng.random(n_updates) < p_insert
     picks = rng.random(n_updates)
-    present = list(init.tolist())          # edge indices present now
-    where = {e: i for i, e in enumerate(present)}
+    present = init.tolist()                # edge indices present now
     nxt = 0
     add = np.empty(n_updates, dtype=np.uint8)
     eidx = np.empty(n_updates, dtype=np.int64)
     for t in range(n_updates):
         if (coins[t] and nxt < n_holdout) or not present:
             e = int(hold[nxt]); nxt += 1
-            where[e] = len(present)
             present.append(e)
             add[t] = 1
         else:
             j = int(picks[t] * len(present))
             e = present[j]
-            last = present[-1]
-            present[j] = last
-            where[last] = j
+            present[j] = present[-1]
             present.pop()
-            del where[e]
             add[t] = 0
         eidx[t] = e
     return ((src[init], dst[init], elab[init]),
