@@ -69,6 +69,9 @@ typedef struct csm_opts {
   uint64_t device_budget;   /* bytes of HBM the engine may use; 0 = 90%    */
   uint32_t split_chunk;     /* candidates per depth-2 task; 0 = autotuned  */
   uint32_t device;          /* HIP device ordinal                          */
+  uint64_t match_budget;    /* per query and batch: stop searching a query
+                               once its matches in the batch exceed this
+                               (0 = unbounded); see csm_query_status       */
 } csm_opts;
 
 /* Creates an engine and uploads the data graph.  `algo` is one of the
@@ -133,6 +136,11 @@ int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
                            uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash);
 int csm_sync(csm_engine* e);
 
+/* Per query: 1 if the last batch stopped that query's search because its
+ * match budget was exceeded (its counts for that batch are then partial).
+ * No reference counterpart (the reference's --max-results caps one update). */
+int csm_query_status(const csm_engine* e, uint8_t* aborted, size_t n);
+
 /* matching::GetNumInitialResults/Positive/Negative (matching.h:60-62). */
 int csm_totals(const csm_engine* e, uint64_t* initial, uint64_t* positive, uint64_t* negative);
 
@@ -148,6 +156,14 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
  * [3] batches, [4] pool repacks, [5] search-kernel time in ns (HIP events on
  * the engine stream), [6] search launches timed. */
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
+
+/* Diagnostic read-back of the SymBi index of one query (parallel_symbi
+ * engines), the state behind Parrllel_SymBi's counters
+ * (Parallel_SymBi/parallel.h:67-76; symbi.cpp:246-377): what = 0 DAG root
+ * (out[0]); 1 d1, 2 d2 (0/1 per query vertex and data vertex, [k][nv]);
+ * 3 np1, 4 nc2 ([k][nv]); 5 n1 (per query vertex, per parent, [nv] each);
+ * 6 n2 (per query vertex, per neighbour in id order, [nv] each). */
+int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, size_t n);
 
 /* The HIP stream the engine launches on (hipStream_t). */
 void* csm_stream(csm_engine* e);

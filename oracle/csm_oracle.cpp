@@ -1043,6 +1043,38 @@ int orc_run(const char* algo, const orc_graph* data, const orc_graph* query, siz
   return -1;
 }
 
+// SymBi index built from scratch on `data` (symbi.cpp:246-377), dumped in the
+// layout of csm_index_dump: what = 0 root, 1 d1, 2 d2, 3 np1, 4 nc2 ([k][nv]),
+// 5 n1 (per vertex u, per DAG parent in BFS order), 6 n2 (per u, per
+// neighbour in id order).  Returns the number of words written, or -1.
+long orc_symbi_dump(const orc_graph* data, const orc_graph* query, uint32_t what, uint32_t* out,
+                    size_t n) {
+  Graph d = data->g;
+  SymBi sb(query->g, d, Opts());
+  const uint32_t k = query->g.nv(), V = d.nv();
+  size_t w = 0;
+  auto put_row = [&](const auto& row) {
+    for (uint32_t v = 0; v < V; ++v) {
+      if (w >= n) return false;
+      out[w++] = (uint32_t)row[v];
+    }
+    return true;
+  };
+  if (what == 0) { if (n < 1) return -1; out[0] = sb.root; return 1; }
+  for (uint32_t u = 0; u < k; ++u) {
+    bool ok = true;
+    if (what == 1) ok = put_row(sb.d1[u]);
+    else if (what == 2) ok = put_row(sb.d2[u]);
+    else if (what == 3) ok = put_row(sb.np1[u]);
+    else if (what == 4) ok = put_row(sb.nc2[u]);
+    else if (what == 5) { for (uint32_t p : sb.tn[u].bw) ok = ok && put_row(sb.n1[sb.eidx[u][p]]); }
+    else if (what == 6) { for (uint32_t x : query->g.nb[u]) ok = ok && put_row(sb.n2[sb.eidx[u][x]]); }
+    else return -1;
+    if (!ok) return -1;
+  }
+  return (long)w;
+}
+
 // Matching orders of the GraphFlow plugin (graphflow.cpp:36-160), for
 // checking the engine's host-side planner: out[i*k + j] = order_vs_[i][j].
 int orc_graphflow_orders(const orc_graph* data, const orc_graph* query, uint32_t* out) {

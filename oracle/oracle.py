@@ -45,6 +45,9 @@ def lib():
                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                               ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_size_t)]
+        L.orc_symbi_dump.restype = ctypes.c_long
+        L.orc_symbi_dump.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, _u32p,
+                                     ctypes.c_size_t]
         L.orc_graphflow_orders.restype = ctypes.c_int
         L.orc_graphflow_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p]
         _lib = L
@@ -123,3 +126,12 @@ def graphflow_orders(data: OGraph, query: OGraph, k: int, ne: int) -> np.ndarray
     out = np.zeros(max(ne, 1) * k, dtype=np.uint32)
     n = lib().orc_graphflow_orders(data.h, query.h, out)
     return out[: n * k].reshape(n, k)
+
+
+def symbi_dump(data: OGraph, query: OGraph, what: int, nv: int, rows: int) -> np.ndarray:
+    """From-scratch SymBi index of `data` (see orc_symbi_dump)."""
+    out = np.zeros(max(1, rows * nv), dtype=np.uint32)
+    n = lib().orc_symbi_dump(data.h, query.h, what, out, out.size)
+    if n < 0:
+        raise ValueError("orc_symbi_dump failed")
+    return out if what == 0 else out.reshape(rows, nv)

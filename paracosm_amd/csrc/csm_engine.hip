@@ -44,8 +44,8 @@ struct QueryInfo {
   std::vector<std::vector<uint32_t>> orders;
   uint32_t plan_base = 0;               // first PlanRec index
   Dag dag;
-  uint32_t d2_base = 0;                 // first d2 row
-  SymbiIndex sx;                        // device SymBi index (algo symbi)
+  uint32_t d2_base = 0;                 // first SymBi row
+  bool sx_built = false;
 };
 
 inline uint32_t bits_for(uint64_t n) {  // bits to hold values in [0, n)
@@ -101,6 +101,10 @@ struct csm_engine {
   uint32_t chunk_cap = 0;
   unsigned long long* d_counts = nullptr;
   unsigned long long* d_hashes = nullptr;
+  unsigned long long* d_qcount = nullptr;  // per-query running matches (budget)
+  uint32_t* d_qabort = nullptr;
+  std::vector<uint32_t> h_qabort;
+  size_t q_cap = 0;
   uint8_t* d_safe = nullptr;
   unsigned long long* h_counts = nullptr;  // pinned
   unsigned long long* h_hashes = nullptr;
@@ -118,9 +122,24 @@ struct csm_engine {
   size_t d_tab_ent_cap = 0;
   bool preprocessed = false;
 
-  // SymBi d2 bitmap rows (all queries), words per row
+  // SymBi index: rows of all queries; d_d2 aliases the d2 bitmap for the search
   uint32_t* d_d2 = nullptr;
   uint32_t d2_rows = 0, d2_words = 0;
+  uint32_t sx_n1rows = 0, sx_n2rows = 0;
+  uint32_t sx_cap_rows = 0, sx_cap_n1 = 0, sx_cap_n2 = 0;
+  std::vector<SxRow> h_sxrows;
+  SxRow* d_sxrows = nullptr;
+  uint32_t *d_sx_d1 = nullptr, *d_sx_d2 = nullptr, *d_sx_np1 = nullptr, *d_sx_nc2 = nullptr;
+  uint32_t *d_sx_n1 = nullptr, *d_sx_n2 = nullptr;
+  std::vector<uint32_t> h_sxtab_off, h_sxtab_ent;
+  uint32_t *d_sxtab_off = nullptr, *d_sxtab_ent = nullptr;
+  uint8_t* d_sx_mark = nullptr;
+  unsigned long long* d_sx_lists[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t sx_list_cap = 0;
+  uint32_t* d_sx_slots = nullptr;
+  uint32_t sx_maxk = 0;
+  // label histograms of the initial graph (SymBi root choice, symbi.cpp:113-175)
+  std::vector<uint64_t> h_triple_freq, h_vfreq;
 
   // totals and stats
   uint64_t tot_init = 0, tot_pos = 0, tot_neg = 0;
@@ -472,8 +491,13 @@ int ensure_results(csm_engine* e) {
   static_assert(sizeof(unsigned long long) == 8, "u64");
   if (e->d_counts) {
     // capacity tracked via allocation size
+    bool big = false;
     for (auto& a : e->allocs)
-      if (a.p == e->d_counts && a.bytes >= need * 8) return CSM_OK;
+      if (a.p == e->d_counts && a.bytes >= need * 8) big = true;
+    if (big && e->queries.size() <= e->q_cap) {
+      e->h_qabort.assign(e->queries.size(), 0);
+      return CSM_OK;
+    }
     dfree(e, e->d_counts);
     dfree(e, e->d_hashes);
     if (e->h_counts) (void)hipHostFree(e->h_counts);
@@ -482,6 +506,16 @@ int ensure_results(csm_engine* e) {
   }
   int rc;
   if ((rc = dalloc(e, &e->d_counts, need)) || (rc = dalloc(e, &e->d_hashes, need))) return rc;
+  if (e->queries.size() > e->q_cap) {
+    dfree(e, e->d_qcount);
+    dfree(e, e->d_qabort);
+    e->q_cap = std::max<size_t>(64, e->queries.size() * 2);
+    if ((rc = dalloc(e, &e->d_qcount, e->q_cap)) || (rc = dalloc(e, &e->d_qabort, e->q_cap)))
+      return rc;
+    HIP_TRY(e, hipMemset(e->d_qcount, 0, e->q_cap * 8));
+    HIP_TRY(e, hipMemset(e->d_qabort, 0, e->q_cap * 4));
+  }
+  e->h_qabort.assign(e->queries.size(), 0);
   HIP_TRY(e, hipHostMalloc((void**)&e->h_counts, need * 8));
   HIP_TRY(e, hipHostMalloc((void**)&e->h_hashes, need * 8));
   return CSM_OK;
@@ -524,6 +558,9 @@ void search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned 
   sa.chunk = e->opts.split_chunk;
   sa.homo = e->opts.homo;
   sa.pos_offset = 0;
+  sa.qcount = e->opts.match_budget ? e->d_qcount : nullptr;
+  sa.qabort = e->opts.match_budget ? e->d_qabort : nullptr;
+  sa.budget = e->opts.match_budget;
   const bool h = hashes != nullptr;
   const bool s = e->algo == ALGO_SYMBI && e->d_d2 != nullptr;
   if (e->ev_search[0]) (void)hipEventRecord(e->ev_search[0], e->stream);
@@ -590,6 +627,8 @@ int check_errors(csm_engine* e) {
   if (err & (ERR_SEED_OVERFLOW | ERR_CHUNK_OVERFLOW))
     return fail(e, CSM_E_CAPACITY, "search task capacity exceeded; use smaller batches");
   if (err & ERR_POOL_OVERFLOW) return fail(e, CSM_E_CAPACITY, "adjacency pool exhausted");
+  if (err & ERR_INDEX_OVERFLOW)
+    return fail(e, CSM_E_CAPACITY, "SymBi index propagation exceeded its frontier capacity");
   return fail(e, CSM_E_DEVICE, "device error bits 0x%x", err);
 }
 
@@ -613,6 +652,10 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
   if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
   k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+  if (e->opts.match_budget) {
+    HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
+    HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
+  }
   // 1. compact lists dirtied by the previous batch
   {
     GraphView g = view(e);
@@ -661,12 +704,268 @@ void accumulate_stats(csm_engine* e) {
   e->stats[2] = e->h_ctrl->stat_bytes;
 }
 
-// SymBi index maintenance (device).  Until the index is built the SymBi
-// plugin searches exactly like GraphFlow (same counts, no d2 pruning).
-int symbi_prepare_query(csm_engine* e, QueryInfo& q) { (void)e; (void)q; return CSM_OK; }
-int symbi_batch_insert(csm_engine* e, uint32_t n) { (void)e; (void)n; return CSM_OK; }
-int symbi_batch_delete(csm_engine* e, uint32_t n) { (void)e; (void)n; return CSM_OK; }
-int symbi_add_vertex(csm_engine* e, uint32_t id) { (void)e; (void)id; return CSM_OK; }
+// ------------------------------------------------------------- SymBi index
+constexpr uint32_t SX_UNMATCHABLE = 0xFFFFFFFEu;
+constexpr int SX_SLOTS = 256;
+
+SxView sx_view(csm_engine* e) {
+  SxView x;
+  x.rows = e->d_sxrows;
+  x.d1 = e->d_sx_d1;
+  x.d2 = e->d_sx_d2;
+  x.np1 = e->d_sx_np1;
+  x.nc2 = e->d_sx_nc2;
+  x.n1 = e->d_sx_n1;
+  x.n2 = e->d_sx_n2;
+  x.words = e->d2_words;
+  x.nvc = e->nv_cap;
+  return x;
+}
+
+// grow a row-major array of `rows_old` rows to `rows_new` rows of `per_row` words
+int grow_rows(csm_engine* e, uint32_t*& p, uint64_t per_row, uint32_t rows_old, uint32_t rows_new) {
+  uint32_t* q = nullptr;
+  int rc = dalloc(e, &q, per_row * std::max<uint32_t>(rows_new, 1));
+  if (rc) return rc;
+  HIP_TRY(e, hipMemsetAsync(q, 0, per_row * std::max<uint32_t>(rows_new, 1) * 4, e->stream));
+  if (p && rows_old)
+    HIP_TRY(e, hipMemcpyAsync(q, p, per_row * rows_old * 4, hipMemcpyDeviceToDevice, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  dfree(e, p);
+  p = q;
+  return CSM_OK;
+}
+
+int sx_reserve(csm_engine* e, uint32_t rows, uint32_t n1rows, uint32_t n2rows) {
+  int rc;
+  const uint64_t words = e->d2_words, nvc = e->nv_cap;
+  if (rows > e->sx_cap_rows) {
+    uint32_t cap = std::max<uint32_t>(rows, e->sx_cap_rows * 2);
+    if ((rc = grow_rows(e, e->d_sx_d1, words, e->d2_rows, cap)) ||
+        (rc = grow_rows(e, e->d_sx_d2, words, e->d2_rows, cap)) ||
+        (rc = grow_rows(e, e->d_sx_np1, nvc, e->d2_rows, cap)) ||
+        (rc = grow_rows(e, e->d_sx_nc2, nvc, e->d2_rows, cap)))
+      return rc;
+    e->sx_cap_rows = cap;
+  }
+  if (n1rows > e->sx_cap_n1) {
+    uint32_t cap = std::max<uint32_t>(n1rows, e->sx_cap_n1 * 2);
+    if ((rc = grow_rows(e, e->d_sx_n1, nvc, e->sx_n1rows, cap))) return rc;
+    e->sx_cap_n1 = cap;
+  }
+  if (n2rows > e->sx_cap_n2) {
+    uint32_t cap = std::max<uint32_t>(n2rows, e->sx_cap_n2 * 2);
+    if ((rc = grow_rows(e, e->d_sx_n2, nvc, e->sx_n2rows, cap))) return rc;
+    e->sx_cap_n2 = cap;
+  }
+  e->d_d2 = e->d_sx_d2;
+  return CSM_OK;
+}
+
+// DAG-edge table: bucket (L(p), L(c), l) -> (parent row << 8) | child index
+int sx_upload_tables(csm_engine* e) {
+  const uint32_t nb = e->nlab * e->nlab * e->nel;
+  std::vector<std::vector<uint32_t>> buckets(nb);
+  for (uint32_t r = 0; r < e->h_sxrows.size(); ++r) {
+    const SxRow& R = e->h_sxrows[r];
+    if (R.label == SX_UNMATCHABLE) continue;
+    for (uint32_t j = 0; j < R.nc; ++j) {
+      const SxRow& C = e->h_sxrows[R.ch_row[j]];
+      if (C.label == SX_UNMATCHABLE) continue;
+      uint32_t l = R.ch_key[j] % e->nel;
+      buckets[(R.label * e->nlab + C.label) * e->nel + l].push_back((r << 8) | j);
+    }
+  }
+  e->h_sxtab_off.assign(nb + 1, 0);
+  e->h_sxtab_ent.clear();
+  for (uint32_t b = 0; b < nb; ++b) {
+    e->h_sxtab_off[b] = (uint32_t)e->h_sxtab_ent.size();
+    e->h_sxtab_ent.insert(e->h_sxtab_ent.end(), buckets[b].begin(), buckets[b].end());
+  }
+  e->h_sxtab_off[nb] = (uint32_t)e->h_sxtab_ent.size();
+  dfree(e, e->d_sxtab_off);
+  dfree(e, e->d_sxtab_ent);
+  dfree(e, e->d_sxrows);
+  int rc;
+  if ((rc = dalloc(e, &e->d_sxtab_off, e->h_sxtab_off.size())) ||
+      (rc = dalloc(e, &e->d_sxtab_ent, std::max<size_t>(1, e->h_sxtab_ent.size()))) ||
+      (rc = dalloc(e, &e->d_sxrows, e->h_sxrows.size())))
+    return rc;
+  HIP_TRY(e, hipMemcpyAsync(e->d_sxtab_off, e->h_sxtab_off.data(), e->h_sxtab_off.size() * 4,
+                            hipMemcpyHostToDevice, e->stream));
+  if (!e->h_sxtab_ent.empty())
+    HIP_TRY(e, hipMemcpyAsync(e->d_sxtab_ent, e->h_sxtab_ent.data(), e->h_sxtab_ent.size() * 4,
+                              hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(e->d_sxrows, e->h_sxrows.data(), e->h_sxrows.size() * sizeof(SxRow),
+                            hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
+int symbi_prepare_query(csm_engine* e, QueryInfo& q) {
+  const QueryGraph& g = q.g;
+  const uint32_t k = g.k;
+  std::vector<uint32_t> dl(k);
+  for (uint32_t u = 0; u < k; ++u) dl[u] = dense_of(e->vl_map, g.vl[u]);
+  auto el_of = [&](uint32_t a, uint32_t b) { return dense_of(e->el_map, g.elabel(a, b)); };
+  auto efreq = [&](uint32_t a, uint32_t b) -> uint64_t {
+    uint32_t l = el_of(a, b);
+    if (dl[a] == NO_LABEL || dl[b] == NO_LABEL || l == NO_LABEL) return 0;
+    return e->h_triple_freq[(dl[a] * e->nlab + dl[b]) * e->nel + l];
+  };
+  auto vfreq = [&](uint32_t a) -> uint64_t { return dl[a] == NO_LABEL ? 0 : e->h_vfreq[dl[a]]; };
+  q.dag = symbi_dag(g, efreq, vfreq);
+  // rows, n1 rows (one per DAG edge, at the child), n2 rows (one per directed edge)
+  const uint32_t base = e->d2_rows;
+  std::vector<std::vector<uint32_t>> n2id(k, std::vector<uint32_t>(k, NONE));
+  uint32_t n2next = e->sx_n2rows;
+  for (uint32_t u = 0; u < k; ++u)
+    for (uint32_t w : g.nb[u]) n2id[u][w] = n2next++;
+  std::vector<std::vector<uint32_t>> n1id(k, std::vector<uint32_t>(k, NONE));
+  uint32_t n1next = e->sx_n1rows;
+  for (uint32_t u = 0; u < k; ++u)
+    for (uint32_t p : q.dag.parents[u]) n1id[u][p] = n1next++;
+  int rc = sx_reserve(e, base + k, n1next, n2next);
+  if (rc) return rc;
+  bool all_known = true;
+  for (uint32_t u = 0; u < k; ++u) {
+    SxRow R;
+    std::memset(&R, 0, sizeof R);
+    R.label = dl[u] == NO_LABEL ? SX_UNMATCHABLE : dl[u];
+    all_known = all_known && dl[u] != NO_LABEL;
+    R.np = (uint32_t)q.dag.parents[u].size();
+    R.nc = (uint32_t)q.dag.children[u].size();
+    for (uint32_t j = 0; j < R.np; ++j) {
+      uint32_t p = q.dag.parents[u][j];
+      uint32_t l = el_of(u, p);
+      all_known = all_known && l != NO_LABEL;
+      R.par_row[j] = base + p;
+      R.par_key[j] = (dl[p] == NO_LABEL ? 0 : dl[p]) * e->nel + (l == NO_LABEL ? 0 : l);
+      R.par_n1[j] = n1id[u][p];
+      R.par_n2up[j] = n2id[u][p];
+      R.par_n2dn[j] = n2id[p][u];
+    }
+    for (uint32_t j = 0; j < R.nc; ++j) {
+      uint32_t c = q.dag.children[u][j];
+      uint32_t l = el_of(u, c);
+      R.ch_row[j] = base + c;
+      R.ch_key[j] = (dl[c] == NO_LABEL ? 0 : dl[c]) * e->nel + (l == NO_LABEL ? 0 : l);
+      R.ch_n1[j] = n1id[c][u];
+      R.ch_n2dn[j] = n2id[u][c];
+      R.ch_n2up[j] = n2id[c][u];
+    }
+    e->h_sxrows.push_back(R);
+  }
+  e->d2_rows = base + k;
+  e->sx_n1rows = n1next;
+  e->sx_n2rows = n2next;
+  e->sx_maxk = std::max(e->sx_maxk, k);
+  q.d2_base = base;
+  if ((rc = sx_upload_tables(e))) return rc;
+  // BuildDCS (symbi.cpp:246-377): top-down, bottom-up, then parent-side n2
+  if (all_known) {
+    GraphView gv = view(e);
+    SxView x = sx_view(e);
+    const int grid = grid_for(e->nv, 256);
+    for (uint32_t i = 0; i < k; ++i)
+      k_sx_build<<<grid, 256, 0, e->stream>>>(gv, x, base + q.dag.bfs[i], 0);
+    for (uint32_t i = 0; i < k; ++i)
+      k_sx_build<<<grid, 256, 0, e->stream>>>(gv, x, base + q.dag.bfs[k - 1 - i], 1);
+    for (uint32_t i = 0; i < k; ++i) k_sx_build<<<grid, 256, 0, e->stream>>>(gv, x, base + i, 2);
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+  }
+  q.sx_built = true;
+  return CSM_OK;
+}
+
+int sx_buffers(csm_engine* e) {
+  if (e->d_sx_mark) return CSM_OK;
+  int rc;
+  e->sx_list_cap = 1u << 24;
+  if ((rc = dalloc(e, &e->d_sx_mark, e->max_batch)) || (rc = dalloc(e, &e->d_sx_slots, SX_SLOTS)))
+    return rc;
+  for (auto& l : e->d_sx_lists)
+    if ((rc = dalloc(e, &l, e->sx_list_cap))) return rc;
+  return CSM_OK;
+}
+
+SxList sx_list(csm_engine* e, int buf, int slot) {
+  SxList l;
+  l.buf = e->d_sx_lists[buf];
+  l.count = e->d_sx_slots + slot;
+  l.cap = e->sx_list_cap;
+  return l;
+}
+
+// One phase: snapshot step, d1 propagation, then d2 propagation (insert: set,
+// union view; delete: clear, end view).  Lists: 0 candidates, 1 frontier,
+// 2 changed-d1 record, 3 spare.  Slots: a fresh counter per list use.
+int sx_phase(csm_engine* e, uint32_t n, bool ins) {
+  int rc;
+  if ((rc = sx_buffers(e))) return rc;
+  GraphView g = view(e);
+  SxView x = sx_view(e);
+  BatchView b = batch_view(e, n);
+  uint32_t* err = &e->d_ctrl->err;
+  k_sx_zero<<<1, 256, 0, e->stream>>>(e->d_sx_slots, SX_SLOTS);
+  if (ins) k_sx_mark<<<grid_for(n, 256), 256, 0, e->stream>>>(b, g, e->d_slab, e->d_sx_mark);
+  SxSeedArgs a;
+  a.b = b;
+  a.g = g;
+  a.x = x;
+  a.slab = e->d_slab;
+  a.mark = e->d_sx_mark;
+  a.tab_off = e->d_sxtab_off;
+  a.tab_ent = e->d_sxtab_ent;
+  a.nlab = e->nlab;
+  a.want = ins ? 1u : 2u;
+  a.err = err;
+  int slot = 0;
+  const int iters = (int)e->sx_maxk + 1;
+  SxList none;
+  none.buf = nullptr;
+  none.count = e->d_sx_slots + SX_SLOTS - 1;
+  none.cap = 0;
+  const int pgrid = std::max(1, e->num_cus * 8);
+  // d1
+  SxList cand = sx_list(e, 0, slot++);
+  SxList rec = sx_list(e, 2, slot++);
+  a.stage = 0;
+  a.out = cand;
+  k_sx_seed<<<grid_for(n, 256), 256, 0, e->stream>>>(a);
+  for (int it = 0; it < iters; ++it) {
+    SxList front = sx_list(e, 1, slot++);
+    k_sx_flags<<<grid_for(e->sx_list_cap, 256, 2048), 256, 0, e->stream>>>(cand, e->d_sx_d1, e->d2_words, ins, front, rec, err);
+    cand = sx_list(e, 0, slot++);
+    k_sx_propagate<<<pgrid, 256, 0, e->stream>>>(g, x, front, ins, !ins, 0, cand, err);
+  }
+  // d2
+  SxList c2 = sx_list(e, 3, slot++);
+  a.stage = 1;
+  a.out = c2;
+  k_sx_seed<<<grid_for(n, 256), 256, 0, e->stream>>>(a);
+  k_sx_from_d1<<<grid_for(e->sx_list_cap, 256, 2048), 256, 0, e->stream>>>(rec, x, ins, c2, err);
+  for (int it = 0; it < iters; ++it) {
+    SxList front = sx_list(e, 1, slot++);
+    k_sx_flags<<<grid_for(e->sx_list_cap, 256, 2048), 256, 0, e->stream>>>(c2, e->d_sx_d2, e->d2_words, ins, front, none, err);
+    c2 = sx_list(e, it % 2 ? 3 : 0, slot++);
+    k_sx_propagate<<<pgrid, 256, 0, e->stream>>>(g, x, front, ins, !ins, 1, c2, err);
+  }
+  k_sx_check<<<1, 64, 0, e->stream>>>(cand.count, err);
+  k_sx_check<<<1, 64, 0, e->stream>>>(c2.count, err);
+  if (slot >= SX_SLOTS - 1) return fail(e, CSM_E_CAPACITY, "SymBi propagation slots exhausted");
+  return CSM_OK;
+}
+
+int symbi_batch_insert(csm_engine* e, uint32_t n) { return sx_phase(e, n, true); }
+int symbi_batch_delete(csm_engine* e, uint32_t n) { return sx_phase(e, n, false); }
+
+int symbi_add_vertex(csm_engine* e, uint32_t id) {
+  if (!e->d_sxrows) return CSM_OK;
+  k_sx_add_vertex<<<1, 256, 0, e->stream>>>(sx_view(e), e->d2_rows, id, e->h_vlab[id]);
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
 
 // InitialMatching on the device: seeds are all directed data edges matching
 // the first edge of order 0, searched from depth 2 (graphflow.cpp:268-283
@@ -710,6 +1009,9 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
     sa.chunk = e->opts.split_chunk;
     sa.homo = e->opts.homo;
     sa.pos_offset = 0xFFFEu;
+    sa.qcount = nullptr;
+    sa.qabort = nullptr;
+    sa.budget = 0;
     if (e->algo == ALGO_SYMBI && e->d_d2) launch_search<true, false, true>(e, sa);
     else launch_search<true, false, false>(e, sa);
     if ((rc = read_ctrl(e))) return rc;
@@ -787,8 +1089,23 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   e->vmask = vb >= 32 ? 0xFFFFFFFFu : ((1u << vb) - 1u);
   e->nv_cap = (uint32_t)std::min<uint64_t>(want_cap, 1ull << vb);
   if (e->nv_cap < e->nv) e->nv_cap = e->nv;
+  e->d2_words = (e->nv_cap + 31) / 32;
   e->h_vlab.assign(e->nv_cap, NO_LABEL);
   for (uint32_t v = 0; v < e->nv; ++v) e->h_vlab[v] = dense_of(e->vl_map, data->vlabels[v]);
+
+  // label histograms of the initial graph for the SymBi root choice
+  e->h_triple_freq.assign((size_t)e->nlab * e->nlab * e->nel, 0);
+  e->h_vfreq.assign(e->nlab, 0);
+  for (uint32_t v = 0; v < e->nv; ++v)
+    if (e->h_vlab[v] != NO_LABEL) ++e->h_vfreq[e->h_vlab[v]];
+  for (size_t i = 0; i < data->num_edges; ++i) {
+    uint32_t a = data->src[i], b = data->dst[i];
+    if (a >= e->nv || b >= e->nv || a == b) continue;
+    uint32_t la = e->h_vlab[a], lb = e->h_vlab[b], l = dense_of(e->el_map, data->elabels[i]);
+    if (la == NO_LABEL || lb == NO_LABEL || l == NO_LABEL) continue;
+    ++e->h_triple_freq[(la * e->nlab + lb) * e->nel + l];
+    ++e->h_triple_freq[(lb * e->nlab + la) * e->nel + l];
+  }
 
   int rc = CSM_OK;
   e->pool_cap = std::max<uint64_t>(4 * 2 * data->num_edges + 8ull * e->max_batch, 1u << 20);
@@ -967,9 +1284,18 @@ int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
   int rc = run_batch(e, (uint32_t)n, reinterpret_cast<unsigned long long*>(d_counts),
                      reinterpret_cast<unsigned long long*>(d_hash), d_safe);
   if (rc) return rc;
+  if (e->opts.match_budget)
+    HIP_TRY(e, hipMemcpyAsync(e->h_qabort.data(), e->d_qabort, e->queries.size() * 4,
+                              hipMemcpyDeviceToHost, e->stream));
   if ((rc = read_ctrl(e))) return rc;
   accumulate_stats(e);
   return check_errors(e);
+}
+
+int csm_query_status(const csm_engine* e, uint8_t* aborted, size_t n) {
+  if (!e || !aborted) return CSM_E_ARG;
+  for (size_t i = 0; i < n && i < e->h_qabort.size(); ++i) aborted[i] = e->h_qabort[i] ? 1 : 0;
+  return CSM_OK;
 }
 
 int csm_sync(csm_engine* e) {
@@ -1052,6 +1378,71 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes) {
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
   if (!e || !out) return CSM_E_ARG;
   for (size_t i = 0; i < n_out && i < 8; ++i) out[i] = e->stats[i];
+  return CSM_OK;
+}
+
+int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, size_t n) {
+  if (!e || !out) return CSM_E_ARG;
+  if (e->algo != ALGO_SYMBI || qid >= e->queries.size() || !e->preprocessed)
+    return fail(e, CSM_E_ARG, "no SymBi index for query %u", qid);
+  HIP_TRY(e, hipSetDevice(e->dev));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  QueryInfo& q = *e->queries[qid];
+  const uint32_t k = q.g.k, nv = e->nv, base = q.d2_base;
+  std::vector<uint32_t> rows;  // device rows to copy, in canonical order
+  uint32_t* arr = nullptr;
+  bool bits = false;
+  switch (what) {
+    case 0:
+      if (n < 1) return CSM_E_ARG;
+      out[0] = q.dag.root;
+      return CSM_OK;
+    case 1: case 2:
+      bits = true;
+      arr = what == 1 ? e->d_sx_d1 : e->d_sx_d2;
+      for (uint32_t u = 0; u < k; ++u) rows.push_back(base + u);
+      break;
+    case 3: case 4:
+      arr = what == 3 ? e->d_sx_np1 : e->d_sx_nc2;
+      for (uint32_t u = 0; u < k; ++u) rows.push_back(base + u);
+      break;
+    case 5:
+      arr = e->d_sx_n1;
+      for (uint32_t u = 0; u < k; ++u) {
+        const SxRow& R = e->h_sxrows[base + u];
+        for (uint32_t j = 0; j < R.np; ++j) rows.push_back(R.par_n1[j]);
+      }
+      break;
+    case 6:
+      arr = e->d_sx_n2;
+      for (uint32_t u = 0; u < k; ++u) {
+        const SxRow& R = e->h_sxrows[base + u];
+        for (uint32_t w : q.g.nb[u]) {
+          uint32_t row = NONE;
+          for (uint32_t j = 0; j < R.np; ++j)
+            if (R.par_row[j] == base + w) row = R.par_n2up[j];
+          for (uint32_t j = 0; j < R.nc; ++j)
+            if (R.ch_row[j] == base + w) row = R.ch_n2dn[j];
+          rows.push_back(row);
+        }
+      }
+      break;
+    default:
+      return CSM_E_ARG;
+  }
+  if (n < rows.size() * (size_t)nv) return fail(e, CSM_E_ARG, "dump buffer too small");
+  for (size_t i = 0; i < rows.size(); ++i) {
+    uint32_t* dst = out + i * nv;
+    if (bits) {
+      std::vector<uint32_t> w(e->d2_words);
+      HIP_TRY(e, hipMemcpy(w.data(), arr + (size_t)rows[i] * e->d2_words, e->d2_words * 4,
+                           hipMemcpyDeviceToHost));
+      for (uint32_t v = 0; v < nv; ++v) dst[v] = (w[v >> 5] >> (v & 31)) & 1u;
+    } else {
+      HIP_TRY(e, hipMemcpy(dst, arr + (size_t)rows[i] * e->nv_cap, (size_t)nv * 4,
+                           hipMemcpyDeviceToHost));
+    }
+  }
   return CSM_OK;
 }
 

@@ -649,7 +649,10 @@ struct SearchArgs {
   uint32_t nq;
   uint32_t chunk;
   uint32_t homo;
-  uint32_t pos_offset;   // position of update index 0 minus 1 (0 in a batch)
+  uint32_t pos_offset;        // position of update index 0 minus 1 (0 in a batch)
+  unsigned long long* qcount; // per-query running match count (nullable)
+  uint32_t* qabort;           // per-query abort flags (nullable)
+  unsigned long long budget;  // per-query match budget in this launch (0: none)
 };
 
 // Per-wave DFS state in LDS.
@@ -661,19 +664,34 @@ struct WaveState {
   uint32_t jmin[MAXK], pdirty[MAXK], udirty[MAXK];
   uint32_t plo[MAXK][MAXNB], phi[MAXK][MAXNB];
   uint32_t cand[MAXK][WAVE];
+  // fused last position: per-parent segments of the current parent buffer
+  uint32_t olo[WAVE], ohi[WAVE], pref[WAVE + 1];
+  uint32_t src[WAVE];
+  uint32_t ljf, ljp, lfsize, lC;
 };
 
 // One wavefront runs one depth-2 chunk of a seed at a time: an iterative DFS
-// over positions 2..k-1 whose per-position candidate buffers (one wave-width
+// over positions 2..k-2 whose per-position candidate buffers (one wave-width
 // each) live in LDS.  Each refill tests 64 consecutive entries of the u_min
 // segment in parallel: version visibility, injectivity against m[], the SymBi
 // d2 bit, and a binary search of every other backward neighbour's segment
-// (the join of graphflow.cpp:222-238).  The last position is never buffered:
-// its survivors are counted with one ballot.
+// (the join of graphflow.cpp:222-238).
+//
+// The last position is never descended into.  Its backward neighbours split
+// into "fixed" ones (positions < k-2, constant while position k-2 iterates)
+// and at most one "parent" (position k-2).  Fixed segments are found once per
+// entry of position k-2; then every refill of position k-2 (up to 64 parents)
+// is finished in one go:
+//   * no parent neighbour: the leaf set S is the same for all parents, so its
+//     valid size C is counted once and each parent adds C - [parent in S];
+//   * otherwise each lane finds its parent's own segment, the smaller of the
+//     own and the fixed segment is the candidate source, and the wave walks
+//     the concatenation of all parents' sources 64 entries at a time.
 template <bool VERSIONED, bool HASH, bool SYMBI>
 __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
   __shared__ WaveState S_all[SEARCH_WAVES];
   const uint32_t lane = lane_id();
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
   WaveState& S = S_all[threadIdx.x / WAVE];
   const uint32_t* __restrict__ adj = a.g.adj;
   const uint32_t* __restrict__ ats = a.g.ats;
@@ -694,6 +712,7 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
     const uint32_t k = uniform(P->k);
     const uint32_t pos = t + 1 + a.pos_offset;
     const uint32_t q = uniform(P->q);
+    if (a.qabort && __atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) continue;
     ++n_tasks;
     S.m[0] = sd.m0;
     S.m[1] = sd.m1;
@@ -701,9 +720,127 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
       S.mq[P->qv[0]] = sd.m0;
       S.mq[P->qv[1]] = sd.m1;
     }
-    unsigned long long cnt = 0, hsum = 0;
+    unsigned long long cnt = 0, hsum = 0, flushed = 0;
+    uint32_t ticks = 0;
+    bool stop = false;
+    const uint32_t L1 = k - 1, L2 = k - 2;   // last and second-to-last positions
 
-    // enter position `lv`: partner segments, u_min, candidate range
+    // publish the running count; stop if the query's budget is exhausted
+    auto flush = [&](bool final_) {
+      unsigned long long d = cnt - flushed;
+      if (lane == 0 && d) {
+        atomicAdd(&a.counts[(size_t)t * a.nq + q], d);
+        if (a.qcount) {
+          unsigned long long tot = atomicAdd(&a.qcount[q], d) + d;
+          if (a.budget && tot > a.budget) atomicOr(&a.qabort[q], 1u);
+        }
+      }
+      flushed = cnt;
+      if (!final_ && a.qabort)
+        stop = uniform(__atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) != 0;
+    };
+
+    // test a candidate entry e of position lv's u_min segment (joins with the
+    // other backward neighbours of lv)
+    auto test = [&](uint32_t lv, uint32_t e, uint32_t end, uint32_t& vout) -> bool {
+      bool ok = e < end;
+      uint32_t v = 0;
+      if (ok) {
+        v = adj[e] & vmask;
+        if (VERSIONED && S.udirty[lv]) ok = ts_visible(ats[e], pos);
+      }
+      if (ok && !a.homo) {
+        for (uint32_t i = 0; i < lv; ++i) ok = ok && (S.m[i] != v);
+      }
+      if (SYMBI && ok) ok = d2_bit(a.d2, P->d2row[lv], a.d2_words, v);
+      const uint32_t nb = uniform(P->nb[lv]);
+      const uint32_t jm = S.jmin[lv];
+      for (uint32_t j = 0; j < nb; ++j) {
+        if (j == jm) continue;
+        if (!ok) continue;
+        uint32_t w = P->bw[lv][j];
+        uint32_t x = ((w >> 8) << vbits) | v;
+        bool vd = VERSIONED && ((S.pdirty[lv] >> j) & 1u);
+        ok = contains(adj, ats, S.plo[lv][j], S.phi[lv][j], x, vd, pos);
+      }
+      vout = v;
+      return ok;
+    };
+
+    // is w a valid last-position vertex w.r.t. the fixed neighbours (all but
+    // `skip`), injectivity against m[0..k-3] and the d2 bit
+    auto last_fixed_ok = [&](uint32_t w, uint32_t skip) -> bool {
+      bool ok = true;
+      if (!a.homo)
+        for (uint32_t i = 0; i < L2; ++i) ok = ok && (S.m[i] != w);
+      if (SYMBI && ok) ok = d2_bit(a.d2, P->d2row[L1], a.d2_words, w);
+      const uint32_t nb = uniform(P->nb[L1]);
+      for (uint32_t j = 0; j < nb; ++j) {
+        if (j == skip || j == S.ljp || !ok) continue;
+        uint32_t bw = P->bw[L1][j];
+        uint32_t x = ((bw >> 8) << vbits) | w;
+        bool vd = VERSIONED && ((S.pdirty[L1] >> j) & 1u);
+        ok = contains(adj, ats, S.plo[L1][j], S.phi[L1][j], x, vd, pos);
+      }
+      return ok;
+    };
+
+    // fixed neighbours of the last position, once per entry of position k-2;
+    // false if some fixed segment is empty (then no embedding can complete)
+    auto prepare_last = [&]() -> bool {
+      const uint32_t nb = uniform(P->nb[L1]);
+      uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu, is_par = 0;
+      if (lane < nb) {
+        uint32_t w = P->bw[L1][lane];
+        uint32_t bp = w & 0xFFu;
+        if (bp == L2) {
+          is_par = 1;
+        } else {
+          uint2 r = a.g.vrec[S.m[bp]];
+          uint2 sg = segment(adj, r.x, r.y & LEN_MASK, w >> 8, vbits);
+          my_lo = sg.x;
+          my_hi = sg.y;
+          my_size = sg.y - sg.x;
+          my_dirty = VERSIONED ? (r.y >> 31) : 0u;
+          S.plo[L1][lane] = my_lo;
+          S.phi[L1][lane] = my_hi;
+        }
+      }
+      uint64_t pm = ballot(is_par != 0);
+      uint32_t jp = pm ? (uint32_t)__builtin_ctzll(pm) : NONE;
+      uint64_t mn = wave_min_u64(((uint64_t)my_size << 32) | lane);
+      uint32_t jf = (uint32_t)(mn >> 32) == 0xFFFFFFFFu ? NONE : uniform((uint32_t)mn);
+      uint32_t fsize = uniform((uint32_t)(mn >> 32));
+      S.ljp = jp;
+      S.ljf = jf;
+      S.lfsize = fsize;
+      S.pdirty[L1] = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
+      n_bytes += 4ull * wave_sum((unsigned long long)(my_size == 0xFFFFFFFFu ? 0u : my_size));
+      wave_sync();
+      if (jf != NONE && fsize == 0) return false;
+      if (jp == NONE && !HASH) {
+        // C = valid leaves of the shared segment
+        const uint32_t lo = uniform(__shfl(my_lo, jf)), hi = uniform(__shfl(my_hi, jf));
+        const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
+        uint32_t C = 0;
+        for (uint32_t b = lo; b < hi; b += WAVE) {
+          uint32_t e = b + lane;
+          bool ok = e < hi;
+          uint32_t w = 0;
+          if (ok) {
+            w = adj[e] & vmask;
+            if (fd) ok = ts_visible(ats[e], pos);
+          }
+          ok = ok && last_fixed_ok(w, jf);
+          C += popc64(ballot(ok));
+          n_cands += (hi - b) < WAVE ? (hi - b) : WAVE;
+        }
+        S.lC = C;
+      }
+      return true;
+    };
+
+    // enter position lv: partner segments, u_min, candidate range
     auto enter = [&](uint32_t lv, uint32_t rlo, uint32_t rhi, int forced_j) -> bool {
       const uint32_t nb = uniform(P->nb[lv]);
       uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu;
@@ -739,33 +876,108 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
       S.jmin[lv] = jm;
       S.pdirty[lv] = dmask;
       S.udirty[lv] = (dmask >> jm) & 1u;
-      return lo < hi;
+      if (lo >= hi) return false;
+      if (lv == L2) return prepare_last();
+      return true;
     };
 
-    // test entry e of position lv's u_min segment
-    auto test = [&](uint32_t lv, uint32_t e, uint32_t end, uint32_t& vout) -> bool {
-      bool ok = e < end;
-      uint32_t v = 0;
-      if (ok) {
-        v = adj[e] & vmask;
-        if (VERSIONED && S.udirty[lv]) ok = ts_visible(ats[e], pos);
+    // all leaves under the bn parents now in cand[k-2]
+    auto finish_parents = [&](uint32_t bn) {
+      const uint32_t jp = S.ljp, jf = S.ljf;
+      uint32_t v = lane < bn ? S.cand[L2][lane] : 0u;
+      if (jp == NONE && !HASH) {
+        // C - [parent is itself a valid leaf]
+        bool self = false;
+        if (lane < bn && !a.homo) {
+          const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
+          uint32_t bw = P->bw[L1][jf];
+          uint32_t x = ((bw >> 8) << vbits) | v;
+          self = contains(adj, ats, S.plo[L1][jf], S.phi[L1][jf], x, fd, pos) &&
+                 last_fixed_ok(v, jf);
+        }
+        cnt += (unsigned long long)bn * S.lC - popc64(ballot(self));
+        n_bytes += 4ull * bn * S.lfsize;
+        return;
       }
-      if (ok && !a.homo) {
-        for (uint32_t i = 0; i < lv; ++i) ok = ok && (S.m[i] != v);
+      // per-parent source: own segment S(v, key_jp) or the fixed minimum
+      uint32_t olo = 0, ohi = 0, odirty = 0, size = 0, src = 0;
+      if (lane < bn) {
+        if (jp != NONE) {
+          uint2 r = a.g.vrec[v];
+          uint2 sg = segment(adj, r.x, r.y & LEN_MASK, P->bw[L1][jp] >> 8, vbits);
+          olo = sg.x;
+          ohi = sg.y;
+          odirty = VERSIONED ? (r.y >> 31) : 0u;
+          uint32_t osz = sg.y - sg.x;
+          if (osz == 0) size = 0;
+          else if (jf != NONE && S.lfsize < osz) { size = S.lfsize; src = 0; }
+          else { size = osz; src = 1; }
+          n_bytes += 4ull * osz;
+        } else {
+          size = S.lfsize;
+          src = 0;
+        }
+        S.olo[lane] = olo;
+        S.ohi[lane] = ohi;
+        S.src[lane] = src | (odirty << 1);
       }
-      if (SYMBI && ok) ok = d2_bit(a.d2, P->d2row[lv], a.d2_words, v);
-      const uint32_t nb = uniform(P->nb[lv]);
-      const uint32_t jm = S.jmin[lv];
-      for (uint32_t j = 0; j < nb; ++j) {
-        if (j == jm) continue;
-        if (!ok) continue;
-        uint32_t w = P->bw[lv][j];
-        uint32_t x = ((w >> 8) << vbits) | v;
-        bool vd = VERSIONED && ((S.pdirty[lv] >> j) & 1u);
-        ok = contains(adj, ats, S.plo[lv][j], S.phi[lv][j], x, vd, pos);
+      // exclusive prefix of sizes over the parents
+      uint32_t incl = size;
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(incl, o);
+        if ((int)lane >= o) incl += y;
       }
-      vout = v;
-      return ok;
+      const uint32_t total = uniform(__shfl(incl, WAVE - 1));
+      S.pref[lane] = incl - size;
+      if (lane == 0) S.pref[WAVE] = total;
+      wave_sync();
+      const uint32_t nfb = uniform(P->nb[L1]);
+      const uint32_t fsum_lo = jf != NONE ? S.plo[L1][jf] : 0u;
+      for (uint32_t b = 0; b < total; b += WAVE) {
+        const uint32_t g = b + lane;
+        bool ok = g < total;
+        uint32_t w = 0, p = 0;
+        if (ok) {
+          // parent p: last index with pref[p] <= g (pref is non-decreasing)
+          uint32_t lo = 0, hi = bn;
+          while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (S.pref[mid] <= g) lo = mid; else hi = mid;
+          }
+          p = lo;
+          const uint32_t sp = S.src[p];
+          const uint32_t off = g - S.pref[p];
+          uint32_t e;
+          bool dirty;
+          if (sp & 1u) { e = S.olo[p] + off; dirty = (sp >> 1) & 1u; }
+          else { e = fsum_lo + off; dirty = VERSIONED && ((S.pdirty[L1] >> jf) & 1u); }
+          w = adj[e] & vmask;
+          if (VERSIONED && dirty) ok = ts_visible(ats[e], pos);
+          const uint32_t vp = S.cand[L2][p];
+          if (ok && !a.homo) ok = w != vp;
+          ok = ok && last_fixed_ok(w, (sp & 1u) ? NONE : jf);
+          if (ok && jp != NONE && !(sp & 1u)) {
+            uint32_t x = ((P->bw[L1][jp] >> 8) << vbits) | w;
+            ok = contains(adj, ats, S.olo[p], S.ohi[p], x, VERSIONED && ((sp >> 1) & 1u), pos);
+          }
+        }
+        (void)nfb;
+        cnt += popc64(ballot(ok));
+        n_cands += (total - b) < WAVE ? (total - b) : WAVE;
+        if (HASH) {
+          uint64_t h = 0;
+          if (ok) {
+            const uint32_t vp = S.cand[L2][p];
+            for (uint32_t i = 0; i < k; ++i) {
+              uint32_t mv = (i == P->qv[L1]) ? w : (i == P->qv[L2]) ? vp : S.mq[i];
+              h = mix64(h + (uint64_t)mv + 1);
+            }
+          }
+          hsum += wave_sum(h);
+        }
+      }
+      if (jp == NONE) n_bytes += 4ull * bn * S.lfsize;
     };
 
     const uint32_t off = c - sd.base;
@@ -774,28 +986,45 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
     rhi = rhi < sd.hi ? rhi : sd.hi;
     int lv = 2;
     bool live = enter(2, rlo, rhi, (int)sd.jmin);
-    while (live) {
+    while (live && !stop) {
       const uint32_t L = (uint32_t)lv;
-      if (L == k - 1) {
+      if (L == L1) {
+        // k == 3: the task's range is the last position itself
         uint32_t cur = S.cur[L], end = S.end[L];
         for (; cur < end; cur += WAVE) {
           uint32_t v;
           bool ok = test(L, cur + lane, end, v);
-          uint64_t m = ballot(ok);
-          cnt += popc64(m);
+          cnt += popc64(ballot(ok));
           n_cands += (end - cur) < WAVE ? (end - cur) : WAVE;
           if (HASH) {
             uint64_t h = 0;
-            if (ok) {
+            if (ok)
               for (uint32_t i = 0; i < k; ++i) {
                 uint32_t mv = (i == P->qv[L]) ? v : S.mq[i];
                 h = mix64(h + (uint64_t)mv + 1);
               }
-            }
             hsum += wave_sum(h);
           }
         }
-        n_bytes += 0;
+        break;
+      }
+      if (++ticks == 256) {
+        ticks = 0;
+        if (cnt - flushed > (1ull << 20) || a.qabort) flush(false);
+      }
+      uint32_t cur = S.cur[L], end = S.end[L];
+      if (L == L2) {
+        if (cur < end) {
+          uint32_t v;
+          bool ok = test(L, cur + lane, end, v);
+          uint64_t m = ballot(ok);
+          n_cands += (end - cur) < WAVE ? (end - cur) : WAVE;
+          if (ok) S.cand[L][popc64(m & lt_mask)] = v;
+          S.cur[L] = cur + WAVE;
+          wave_sync();
+          if (m) finish_parents(popc64(m));
+          continue;
+        }
         --lv;
         if (lv < 2) break;
         continue;
@@ -810,14 +1039,12 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
         if (!enter(L + 1, 0u, 0xFFFFFFFFu, -1)) --lv;
         continue;
       }
-      uint32_t cur = S.cur[L], end = S.end[L];
       if (cur < end) {
         uint32_t v;
         bool ok = test(L, cur + lane, end, v);
         uint64_t m = ballot(ok);
         n_cands += (end - cur) < WAVE ? (end - cur) : WAVE;
-        uint32_t slot = popc64(m & ((1ull << lane) - 1ull));
-        if (ok) S.cand[L][slot] = v;
+        if (ok) S.cand[L][popc64(m & lt_mask)] = v;
         S.bn[L] = popc64(m);
         S.bi[L] = 0;
         S.cur[L] = cur + WAVE;
@@ -827,10 +1054,8 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
       --lv;
       if (lv < 2) break;
     }
-    if (lane == 0) {
-      if (cnt) atomicAdd(&a.counts[(size_t)t * a.nq + q], cnt);
-      if (HASH && hsum) atomicAdd(&a.hashes[(size_t)t * a.nq + q], hsum);
-    }
+    flush(true);
+    if (lane == 0 && HASH && hsum) atomicAdd(&a.hashes[(size_t)t * a.nq + q], hsum);
   }
   if (lane == 0) {
     atomicAdd(&a.ctrl->stat_tasks, n_tasks);
@@ -838,7 +1063,6 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
     atomicAdd(&a.ctrl->stat_bytes, n_bytes + 4ull * n_cands);
   }
 }
-
 
 // ------------------------------------------------------------ batch control
 __global__ void k_batch_begin(Ctrl* c) {

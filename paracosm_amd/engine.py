@@ -51,7 +51,8 @@ class _Opts(ctypes.Structure):
                 ("threads", ctypes.c_uint32), ("auto_tuning", ctypes.c_uint32),
                 ("max_batch", ctypes.c_uint32), ("want_hash", ctypes.c_uint32),
                 ("time_limit_s", ctypes.c_double), ("device_budget", ctypes.c_uint64),
-                ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32)]
+                ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32),
+                ("match_budget", ctypes.c_uint64)]
 
 
 UPDATE_DTYPE = np.dtype([("type", np.uint8), ("is_add", np.uint8), ("pad", np.uint16),
@@ -83,10 +84,12 @@ def lib():
             "csm_apply_batch": (ctypes.c_int, [vp, vp, sz, vp, vp, vp, vp]),
             "csm_apply_batch_device": (ctypes.c_int, [vp, vp, sz, vp, vp, vp]),
             "csm_sync": (ctypes.c_int, [vp]),
+            "csm_query_status": (ctypes.c_int, [vp, vp, sz]),
             "csm_totals": (ctypes.c_int, [vp, P(u64), P(u64), P(u64)]),
             "csm_memory": (ctypes.c_int, [vp, P(sz), P(sz)]),
             "csm_device_bytes": (ctypes.c_int, [vp, P(u64)]),
             "csm_stats": (ctypes.c_int, [vp, P(u64), sz]),
+            "csm_index_dump": (ctypes.c_int, [vp, u32, u32, vp, sz]),
             "csm_stream": (vp, [vp]),
             "csm_last_error": (ctypes.c_char_p, [vp]),
             "csm_destroy": (None, [vp]),
@@ -103,8 +106,9 @@ def exported_symbols() -> Sequence[str]:
     return ("csm_create", "csm_register_query", "csm_num_queries", "csm_preprocess",
             "csm_initial", "csm_classify", "csm_add_edge", "csm_remove_edge",
             "csm_add_vertex", "csm_remove_vertex", "csm_apply_batch",
-            "csm_apply_batch_device", "csm_sync", "csm_totals", "csm_memory",
-            "csm_device_bytes", "csm_stats", "csm_stream", "csm_last_error", "csm_destroy")
+            "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_totals", "csm_memory",
+            "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_stream", "csm_last_error",
+            "csm_destroy")
 
 
 def _u32(a) -> np.ndarray:
@@ -141,13 +145,13 @@ class Matching:
                  max_results: int = 0, print_prep: bool = False, print_enum: bool = False,
                  homo: bool = False, threads: int = 0, auto_tuning: int = 1,
                  max_batch: int = 0, want_hash: bool = False, device_budget: int = 0,
-                 split_chunk: int = 0, device: int = 0):
+                 split_chunk: int = 0, device: int = 0, match_budget: int = 0):
         L = lib()
         self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         d = _GraphArgs(vlabels, src, dst, elabels)
         o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
                   auto_tuning, max_batch, int(want_hash), 0.0, device_budget, split_chunk,
-                  device)
+                  device, match_budget)
         err = ctypes.c_int(0)
         self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
                               ctypes.byref(o), ctypes.byref(err))
@@ -235,6 +239,12 @@ class Matching:
     def num_queries(self) -> int:
         return int(lib().csm_num_queries(self.h))
 
+    def query_status(self) -> np.ndarray:
+        """1 per query whose search the last batch stopped at the match budget."""
+        out = np.zeros(max(1, self.num_queries), np.uint8)
+        self._check(lib().csm_query_status(self.h, out.ctypes.data, out.size))
+        return out[: self.num_queries]
+
     def totals(self):
         a, b, c = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
         self._check(lib().csm_totals(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
@@ -250,6 +260,12 @@ class Matching:
         b = ctypes.c_uint64(0)
         self._check(lib().csm_device_bytes(self.h, ctypes.byref(b)))
         return int(b.value)
+
+    def index_dump(self, qid: int, what: int, nv: int, rows: int) -> np.ndarray:
+        """SymBi index rows of query `qid` (see csm_index_dump)."""
+        out = np.zeros(max(1, rows * nv), np.uint32)
+        self._check(lib().csm_index_dump(self.h, qid, what, out.ctypes.data, out.size))
+        return out if what == 0 else out.reshape(rows, nv)
 
     def stream(self) -> int:
         return int(lib().csm_stream(self.h) or 0)

@@ -102,15 +102,18 @@ def admit(d: Data, *, sample: int = 256, budget_per_update: float = 1e4,
             got: List[gen.Query] = []
             for rnd in range(max_rounds):
                 cands = candidates(d, k, dens, 4 * d.cfg.per_size, rnd)
-                m = Matching(algo, cands[0], w.vlabels, w.init_src, w.init_dst, w.init_elab)
+                budget = int(budget_per_update * n)
+                m = Matching(algo, cands[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                             match_budget=budget)
                 for q in cands[1:]:
                     m.register_query(q)
                 m.Preprocessing()
                 pos, neg, _, _ = m.apply_batch(ups)
+                over = m.query_status()
                 m.close()
                 tot = (pos + neg).sum(axis=0)
-                for q, t in zip(cands, tot):
-                    if t <= budget_per_update * n and len(got) < d.cfg.per_size:
+                for q, t, o in zip(cands, tot, over):
+                    if not o and t <= budget and len(got) < d.cfg.per_size:
                         got.append(q)
                 log(f"admission k={k} {dens}: round {rnd}: {len(got)}/{d.cfg.per_size} "
                     f"(sample totals {sorted(int(x) for x in tot)[:6]}...)")

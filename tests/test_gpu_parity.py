@@ -209,3 +209,68 @@ def test_rmat_config1_slice(oracle_lib):
                 pos, neg, _, _ = m.apply_batch(make_updates(*ups))
             print("rmat12 query", qi, algo, int(ref.pos.sum()), flush=True)
             np.testing.assert_array_equal(pos[:, 0], ref.pos)
+
+
+def _edges_after(init, ups, upto):
+    edges = {(min(int(a), int(b)), max(int(a), int(b))): int(l) for a, b, l in zip(*init)}
+    add, v1, v2, lab = ups
+    for i in range(upto):
+        key = (min(int(v1[i]), int(v2[i])), max(int(v1[i]), int(v2[i])))
+        if add[i]:
+            edges.setdefault(key, int(lab[i]))
+        else:
+            edges.pop(key, None)
+    ks = sorted(edges)
+    return (np.asarray([k[0] for k in ks], np.uint32), np.asarray([k[1] for k in ks], np.uint32),
+            np.asarray([edges[k] for k in ks], np.uint32))
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_symbi_index_is_the_fixpoint(oracle_lib, seed):
+    """After every batch the device SymBi index (d1, d2, np1, nc2, n1, n2)
+    equals a from-scratch BuildDCS of the graph at that point."""
+    O = oracle_lib
+    vl, init, ups, q = _case(40 + seed, nv=45, p=0.25, n_vl=2, n_el=2, n_upd=150,
+                             k=4 + seed % 3, extra=seed % 3)
+    from paracosm_amd.engine import make_updates
+    nv = len(vl)
+    k = q.k
+    npar = None
+    ref0 = None
+    u = make_updates(*ups)
+    with _engine("parallel_symbi", q, vl, init) as m:
+        bounds = [0, 40, 41, 100, len(u)]
+        for bi in range(len(bounds)):
+            if bi > 0:
+                m.apply_batch(u[bounds[bi - 1]:bounds[bi]])
+            dg = O.OGraph(vl, *_edges_after(init, ups, bounds[bi]))
+            qg = O.OGraph.from_query(q)
+            assert m.index_dump(0, 0, nv, 1)[0] == O.symbi_dump(dg, qg, 0, nv, 1)[0]
+            # rows per dump kind: k, k, k, k, #DAG edges (= |E_q|), 2|E_q|
+            for what, rows in ((1, k), (2, k), (3, k), (4, k), (5, q.num_edges),
+                               (6, 2 * q.num_edges)):
+                dev = m.index_dump(0, what, nv, rows)
+                ref = O.symbi_dump(dg, qg, what, nv, rows)
+                np.testing.assert_array_equal(dev, ref, err_msg=f"batch {bi} what {what}")
+
+
+def test_match_budget_stops_only_queries_over_it(oracle_lib):
+    """A query whose matches in a batch exceed the budget is stopped and
+    flagged; a query under it is exact and unflagged."""
+    from paracosm_amd.engine import make_updates
+    nv = 30
+    vl = np.zeros(nv, np.uint32)
+    rng = np.random.default_rng(8)
+    _, s, d, el = random_graph(nv, 0.6, 1, 1, rng)
+    init, ups = mixed_stream_small(s, d, el, nv, 80, 1.0, 1, rng)
+    heavy, light = query_star(4), query_path2()
+    rh = _oracle(oracle_lib, heavy, vl, init, ups)
+    rl = _oracle(oracle_lib, light, vl, init, ups)
+    budget = int(rl.pos.sum()) + 10
+    assert int(rh.pos.sum()) > budget
+    with _engine("parallel_graphflow", light, vl, init, match_budget=budget) as m:
+        m.register_query(heavy)
+        pos, _, _, _ = m.apply_batch(make_updates(*ups))
+        st = m.query_status()
+    assert st.tolist() == [0, 1]
+    np.testing.assert_array_equal(pos[:, 0], rl.pos)
