@@ -95,7 +95,9 @@ def main() -> int:
         return n
 
     for i in range(args.warmup):
+        tw = time.perf_counter()
         step(i)
+        log(f"warmup step {i}: {time.perf_counter() - tw:.3f}s {m.stats()}")
     torch.cuda.synchronize()
     m.sync()
     st0 = m.stats()
@@ -107,10 +109,12 @@ def main() -> int:
         tb = time.perf_counter()
         n_done += step(i)
         lat.append(time.perf_counter() - tb)
+        log(f"step {i}: {lat[-1]:.3f}s")
     m.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     st1 = m.stats()
+    log(f"engine stats after the timed steps: {st1}")
     counts = d_counts.cpu().numpy().astype(np.uint64).reshape(n_total, nq)
     timed = counts[args.warmup * args.batch: args.warmup * args.batch + n_done]
     matches = int(timed.sum())
@@ -129,6 +133,8 @@ def main() -> int:
     pcounts = np.zeros(P, np.uint64)
     tp0 = time.perf_counter()
     for i in range(P):
+        if i % 500 == 0:
+            log(f"plain per-update: {i}/{P}")
         ta = time.perf_counter()
         if w.upd_add[i]:
             pcounts[i] = mp.AddEdge(int(w.upd_v1[i]), int(w.upd_v2[i]), int(w.upd_label[i]))

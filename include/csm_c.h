@@ -72,6 +72,10 @@ typedef struct csm_opts {
   uint64_t match_budget;    /* per query and batch: stop searching a query
                                once its matches in the batch exceed this
                                (0 = unbounded); see csm_query_status       */
+  uint32_t spill_budget;    /* DFS loop iterations before a search task
+                               hands the rest of its subtree to the next
+                               round (0 = default)                         */
+  uint32_t pad0;
 } csm_opts;
 
 /* Creates an engine and uploads the data graph.  `algo` is one of the
@@ -154,7 +158,8 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
  * entries tested, [2] algorithmic bytes of the search (4 B per candidate
  * entry scanned plus 4 B per entry of every partner segment, merge model),
  * [3] batches, [4] pool repacks, [5] search-kernel time in ns (HIP events on
- * the engine stream), [6] search launches timed. */
+ * the engine stream), [6] search launches timed, [7] most search rounds one
+ * batch needed. */
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
 
 /* Diagnostic read-back of the SymBi index of one query (parallel_symbi

@@ -95,10 +95,15 @@ struct csm_engine {
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   uint32_t* d_elmap = nullptr;
-  Seed* d_seeds = nullptr;
+  Task* d_seeds = nullptr;
   uint32_t seed_cap = 0;
   uint32_t* d_chunk_owner = nullptr;
   uint32_t chunk_cap = 0;
+  Task* d_spill[2] = {nullptr, nullptr};          // ping-pong task lists of later rounds
+  uint32_t* d_spill_owner[2] = {nullptr, nullptr};
+  uint32_t spill_cap = 0, spill_chunk_cap = 0;
+  uint32_t budget_iters = 1024;
+  uint32_t max_rounds = 0;                        // stats: deepest round count seen
   unsigned long long* d_counts = nullptr;
   unsigned long long* d_hashes = nullptr;
   unsigned long long* d_qcount = nullptr;  // per-query running matches (budget)
@@ -476,10 +481,16 @@ int alloc_batch(csm_engine* e) {
       (rc = dalloc(e, &e->d_dirty[0], 2 * (size_t)B + 2)) ||
       (rc = dalloc(e, &e->d_dirty[1], 2 * (size_t)B + 2)) || (rc = dalloc(e, &e->d_safe, B)))
     return rc;
-  e->seed_cap = std::max<uint32_t>(1u << 16, std::min<uint64_t>(16ull * B * 8, 1u << 24));
+  e->seed_cap = std::max<uint32_t>(1u << 16, std::min<uint64_t>(16ull * B, 1u << 22));
   e->chunk_cap = 1u << 26;
+  e->spill_cap = 1u << 21;
+  e->spill_chunk_cap = 1u << 24;
   if ((rc = dalloc(e, &e->d_seeds, e->seed_cap)) ||
-      (rc = dalloc(e, &e->d_chunk_owner, e->chunk_cap)))
+      (rc = dalloc(e, &e->d_chunk_owner, e->chunk_cap)) ||
+      (rc = dalloc(e, &e->d_spill[0], e->spill_cap)) ||
+      (rc = dalloc(e, &e->d_spill[1], e->spill_cap)) ||
+      (rc = dalloc(e, &e->d_spill_owner[0], e->spill_chunk_cap)) ||
+      (rc = dalloc(e, &e->d_spill_owner[1], e->spill_chunk_cap)))
     return rc;
   HIP_TRY(e, hipHostMalloc((void**)&e->h_upd, (size_t)B * sizeof(UpdateRec)));
   HIP_TRY(e, hipHostMalloc((void**)&e->h_safe, B));
@@ -543,25 +554,7 @@ void launch_search(csm_engine* e, const SearchArgs& sa) {
   k_search<V, H, S><<<e->search_blocks, SEARCH_WAVES * WAVE, 0, e->stream>>>(sa);
 }
 
-void search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned long long* hashes) {
-  SearchArgs sa;
-  sa.g = view(e);
-  sa.plans = e->d_plans;
-  sa.seeds = e->d_seeds;
-  sa.chunk_owner = e->d_chunk_owner;
-  sa.ctrl = e->d_ctrl;
-  sa.counts = counts;
-  sa.hashes = hashes;
-  sa.d2 = e->d_d2;
-  sa.d2_words = e->d2_words;
-  sa.nq = (uint32_t)e->queries.size();
-  sa.chunk = e->opts.split_chunk;
-  sa.homo = e->opts.homo;
-  sa.pos_offset = 0;
-  sa.qcount = e->opts.match_budget ? e->d_qcount : nullptr;
-  sa.qabort = e->opts.match_budget ? e->d_qabort : nullptr;
-  sa.budget = e->opts.match_budget;
-  const bool h = hashes != nullptr;
+void launch_round(csm_engine* e, SearchArgs& sa, bool versioned, bool h) {
   const bool s = e->algo == ALGO_SYMBI && e->d_d2 != nullptr;
   if (e->ev_search[0]) (void)hipEventRecord(e->ev_search[0], e->stream);
   if (versioned) {
@@ -574,6 +567,55 @@ void search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned 
   if (e->ev_search[1]) {
     (void)hipEventRecord(e->ev_search[1], e->stream);
     e->ev_pending = true;
+  }
+}
+
+void collect_search_time(csm_engine* e);
+int read_ctrl(csm_engine* e);
+
+// Search rounds: round 0 runs the seeds; a task that exceeds its iteration
+// budget spills the rest of its DFS into the next round's task list, and
+// rounds repeat until nothing spills.
+int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned long long* hashes,
+           uint32_t pos_offset = 0) {
+  SearchArgs sa;
+  sa.g = view(e);
+  sa.plans = e->d_plans;
+  sa.tasks = e->d_seeds;
+  sa.chunk_owner = e->d_chunk_owner;
+  sa.ctrl = e->d_ctrl;
+  sa.counts = counts;
+  sa.hashes = hashes;
+  sa.d2 = e->d_d2;
+  sa.d2_words = e->d2_words;
+  sa.nq = (uint32_t)e->queries.size();
+  sa.chunk = e->opts.split_chunk;
+  sa.homo = e->opts.homo;
+  sa.pos_offset = pos_offset;
+  sa.qcount = e->opts.match_budget ? e->d_qcount : nullptr;
+  sa.qabort = e->opts.match_budget ? e->d_qabort : nullptr;
+  sa.budget = e->opts.match_budget;
+  sa.spill_cap = e->spill_cap;
+  sa.spill_chunk_cap = e->spill_chunk_cap;
+  sa.budget_iters = e->budget_iters;
+  const bool h = hashes != nullptr;
+  int out = 0;
+  for (uint32_t round = 0;; ++round) {
+    sa.spill = e->d_spill[out];
+    sa.spill_owner = e->d_spill_owner[out];
+    launch_round(e, sa, versioned, h);
+    int rc = read_ctrl(e);
+    if (rc) return rc;
+    collect_search_time(e);
+    if (e->h_ctrl->nspill == 0) {
+      e->max_rounds = std::max(e->max_rounds, round + 1);
+      return CSM_OK;
+    }
+    k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+    sa.tasks = e->d_spill[out];
+    sa.chunk_owner = e->d_spill_owner[out];
+    out = 1 - out;
+    if (round > 100000) return fail(e, CSM_E_DEVICE, "search did not converge");
   }
 }
 
@@ -686,7 +728,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
     if ((rc = symbi_batch_insert(e, n))) return rc;
   // 5. seeds + search
   seeds(e, n, counts, hashes, safe, true);
-  search(e, true, counts, hashes);
+  if ((rc = search(e, true, counts, hashes))) return rc;
   // 6. ... and deletions (after it)
   if (e->algo == ALGO_SYMBI && e->d_d2)
     if ((rc = symbi_batch_delete(e, n))) return rc;
@@ -699,6 +741,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
 
 void accumulate_stats(csm_engine* e) {
   collect_search_time(e);
+  e->stats[7] = e->max_rounds;
   e->stats[0] = e->h_ctrl->stat_tasks;
   e->stats[1] = e->h_ctrl->stat_cands;
   e->stats[2] = e->h_ctrl->stat_bytes;
@@ -995,26 +1038,7 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
       continue;
     }
     // search with the end-of-batch view (position 0xFFFF)
-    SearchArgs sa;
-    sa.g = g;
-    sa.plans = e->d_plans;
-    sa.seeds = e->d_seeds;
-    sa.chunk_owner = e->d_chunk_owner;
-    sa.ctrl = e->d_ctrl;
-    sa.counts = e->d_counts;
-    sa.hashes = nullptr;
-    sa.d2 = e->d_d2;
-    sa.d2_words = e->d2_words;
-    sa.nq = (uint32_t)nq;
-    sa.chunk = e->opts.split_chunk;
-    sa.homo = e->opts.homo;
-    sa.pos_offset = 0xFFFEu;
-    sa.qcount = nullptr;
-    sa.qabort = nullptr;
-    sa.budget = 0;
-    if (e->algo == ALGO_SYMBI && e->d_d2) launch_search<true, false, true>(e, sa);
-    else launch_search<true, false, false>(e, sa);
-    if ((rc = read_ctrl(e))) return rc;
+    if ((rc = search(e, true, e->d_counts, nullptr, 0xFFFEu))) return rc;
     v0 = v1;
   }
   unsigned long long c = 0;
@@ -1041,6 +1065,7 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   if (e->opts.max_batch == 0 || e->opts.max_batch > DEFAULT_MAX_BATCH) e->opts.max_batch = DEFAULT_MAX_BATCH;
   if (e->opts.split_chunk == 0) e->opts.split_chunk = 64;
   e->max_batch = e->opts.max_batch;
+  if (e->opts.spill_budget) e->budget_iters = e->opts.spill_budget;
   e->dev = (int)e->opts.device;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= e->dev) { set(CSM_E_DEVICE); return nullptr; }

@@ -53,7 +53,9 @@ struct Ctrl {
   uint32_t ndirty_next;   // dirty vertices recorded for the next batch
   uint32_t ndirty_cur;    // dirty vertices to compact at the start of this batch
   uint32_t abort;         // host-set: stop searching
-  uint32_t pad[6];
+  uint32_t nspill;        // tasks spilled by the current search round
+  uint32_t nspill_chunks; // their chunks
+  uint32_t pad[4];
   unsigned long long live;        // live entries (sum of list lengths)
   unsigned long long stat_tasks;
   unsigned long long stat_cands;
@@ -72,16 +74,22 @@ struct PlanRec {
   uint32_t d2row[MAXK];          // row of position p's query vertex in the d2 bitmap
 };
 
-// A seed: one (update, query edge orientation) with m[0], m[1] fixed, and the
-// depth-2 candidate segment that its chunks split.
-struct Seed {
-  uint32_t t;        // update index in the batch
-  uint32_t qo;       // plan index
-  uint32_t m0, m1;
-  uint32_t lo, hi;   // depth-2 candidate segment (absolute pool offsets)
+// A search task: update t, plan qo, the first `depth` positions fixed in m[],
+// and the candidate range of position `depth` (absolute pool offsets into the
+// segment of backward neighbour `jmin`; hi == NONE: the whole segment).  The
+// seeds of an update are depth-2 tasks; tasks of deeper prefixes come from
+// spills.  Chunk c of a task covers [lo + c*chunk, lo + (c+1)*chunk).
+struct Task {
+  uint32_t t;
+  uint32_t qo;
+  uint32_t depth;
+  uint32_t jmin;
+  uint32_t lo, hi;
   uint32_t base;     // first chunk id
-  uint32_t jmin;     // which backward neighbour supplied the segment
+  uint32_t nch;      // chunks
+  uint32_t m[MAXK];
 };
+constexpr uint32_t TASK_WORDS = sizeof(Task) / 4;
 
 struct GraphView {
   const uint2* __restrict__ vrec;
@@ -137,6 +145,17 @@ __device__ __forceinline__ bool contains(const uint32_t* __restrict__ adj,
   for (; i < hi && adj[i] == x; ++i)
     if (ts_visible(ats[i], pos)) return true;
   return false;
+}
+
+// reserve n slots below cap without leaving holes; NONE if they do not fit
+__device__ __forceinline__ uint32_t reserve(uint32_t* counter, uint32_t n, uint32_t cap) {
+  uint32_t cur = __atomic_load_n(counter, __ATOMIC_RELAXED);
+  while (true) {
+    if ((uint64_t)cur + n > cap) return NONE;
+    uint32_t prev = atomicCAS(counter, cur, cur + n);
+    if (prev == cur) return cur;
+    cur = prev;
+  }
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -556,7 +575,7 @@ struct SeedArgs {
   uint32_t versioned;
   uint32_t homo;
   uint32_t want_hash;
-  Seed* seeds;
+  Task* seeds;
   uint32_t seed_cap;
   uint32_t* chunk_owner;
   uint32_t chunk_cap;
@@ -618,17 +637,13 @@ __global__ void k_seeds(SeedArgs a) {
       }
       if (best == 0 || best == NONE) continue;
       uint32_t nch = (best + a.chunk - 1) / a.chunk;
-      uint32_t sidx = atomicAdd(&a.ctrl->nseeds, 1u);
-      if (sidx >= a.seed_cap) { atomicOr(&a.ctrl->err, ERR_SEED_OVERFLOW); continue; }
-      uint32_t base = atomicAdd(&a.ctrl->nchunks, nch);
-      if ((uint64_t)base + nch > a.chunk_cap) {
-        atomicOr(&a.ctrl->err, ERR_CHUNK_OVERFLOW);
-        continue;
-      }
-      Seed sd;
-      sd.t = i; sd.qo = qo; sd.m0 = m0; sd.m1 = m1; sd.lo = blo; sd.hi = bhi;
-      sd.base = base; sd.jmin = bj;
-      a.seeds[sidx] = sd;
+      uint32_t sidx = reserve(&a.ctrl->nseeds, 1u, a.seed_cap);
+      if (sidx == NONE) { atomicOr(&a.ctrl->err, ERR_SEED_OVERFLOW); continue; }
+      uint32_t base = reserve(&a.ctrl->nchunks, nch, a.chunk_cap);
+      if (base == NONE) { atomicOr(&a.ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
+      Task* tk = a.seeds + sidx;
+      tk->t = i; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
+      tk->base = base; tk->nch = nch; tk->m[0] = m0; tk->m[1] = m1;
       for (uint32_t c = 0; c < nch; ++c) a.chunk_owner[base + c] = sidx;
     }
     (void)pos;
@@ -639,8 +654,12 @@ __global__ void k_seeds(SeedArgs a) {
 struct SearchArgs {
   GraphView g;
   const PlanRec* __restrict__ plans;
-  const Seed* __restrict__ seeds;
+  const Task* __restrict__ tasks;
   const uint32_t* __restrict__ chunk_owner;
+  Task* spill;                // next round's tasks (nullable: never spill)
+  uint32_t* spill_owner;      // next round's chunk owners
+  uint32_t spill_cap, spill_chunk_cap;
+  uint32_t budget_iters;      // loop iterations before a task spills
   Ctrl* ctrl;
   unsigned long long* counts;
   unsigned long long* hashes;
@@ -706,19 +725,19 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
     if (c >= nch) break;
     if (__atomic_load_n(&a.ctrl->abort, __ATOMIC_RELAXED)) break;
     const uint32_t sidx = uniform(a.chunk_owner[c]);
-    const Seed sd = a.seeds[sidx];
-    const uint32_t t = uniform(sd.t), qo = uniform(sd.qo);
+    const Task* __restrict__ TK = a.tasks + sidx;
+    const uint32_t t = uniform(TK->t), qo = uniform(TK->qo);
+    const uint32_t d0 = uniform(TK->depth);
     const PlanRec* __restrict__ P = a.plans + qo;
     const uint32_t k = uniform(P->k);
     const uint32_t pos = t + 1 + a.pos_offset;
     const uint32_t q = uniform(P->q);
     if (a.qabort && __atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) continue;
     ++n_tasks;
-    S.m[0] = sd.m0;
-    S.m[1] = sd.m1;
-    if (HASH) {
-      S.mq[P->qv[0]] = sd.m0;
-      S.mq[P->qv[1]] = sd.m1;
+    for (uint32_t i = 0; i < d0; ++i) {
+      uint32_t v = TK->m[i];
+      S.m[i] = v;
+      if (HASH) S.mq[P->qv[i]] = v;
     }
     unsigned long long cnt = 0, hsum = 0, flushed = 0;
     uint32_t ticks = 0;
@@ -980,13 +999,77 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
       if (jp == NONE) n_bytes += 4ull * bn * S.lfsize;
     };
 
-    const uint32_t off = c - sd.base;
-    uint32_t rlo = sd.lo + off * a.chunk;
-    uint32_t rhi = rlo + a.chunk;
-    rhi = rhi < sd.hi ? rhi : sd.hi;
-    int lv = 2;
-    bool live = enter(2, rlo, rhi, (int)sd.jmin);
+    // spill the rest of this task's DFS as new tasks (one per remaining range
+    // and one per buffered candidate); false if the spill buffers are full
+    auto spill = [&](uint32_t lv_now) -> bool {
+      uint32_t n_t = 0, n_c = 0;
+      for (uint32_t L = d0; L <= lv_now; ++L) {
+        uint32_t cu = S.cur[L], en = S.end[L];
+        if (cu < en) { ++n_t; n_c += (en - cu + a.chunk - 1) / a.chunk; }
+        if (L < L2) { uint32_t r = S.bn[L] - S.bi[L]; n_t += r; n_c += r; }
+      }
+      n_t = uniform(n_t);
+      n_c = uniform(n_c);
+      if (n_t == 0) return true;
+      uint32_t tb = 0, cb = 0;
+      if (lane == 0) {
+        tb = reserve(&a.ctrl->nspill, n_t, a.spill_cap);
+        cb = tb == NONE ? NONE : reserve(&a.ctrl->nspill_chunks, n_c, a.spill_chunk_cap);
+      }
+      tb = uniform(__shfl(tb, 0));
+      cb = uniform(__shfl(cb, 0));
+      if (tb == NONE || cb == NONE) return false;  // (a task slot may leak; never read)
+      uint32_t ti = tb, ci = cb;
+      auto emit = [&](uint32_t depth, uint32_t jm, uint32_t lo, uint32_t hi, uint32_t nch,
+                      uint32_t extra) {
+        Task* o = a.spill + ti;
+        uint32_t word = 0;
+        if (lane < TASK_WORDS) {
+          if (lane == 0) word = t;
+          else if (lane == 1) word = qo;
+          else if (lane == 2) word = depth;
+          else if (lane == 3) word = jm;
+          else if (lane == 4) word = lo;
+          else if (lane == 5) word = hi;
+          else if (lane == 6) word = ci;
+          else if (lane == 7) word = nch;
+          else {
+            uint32_t p = lane - 8;
+            word = p + 1 < depth ? S.m[p] : (p + 1 == depth ? extra : 0u);
+          }
+          reinterpret_cast<uint32_t*>(o)[lane] = word;
+        }
+        for (uint32_t j = lane; j < nch; j += WAVE) a.spill_owner[ci + j] = ti;
+        ++ti;
+        ci += nch;
+      };
+      for (uint32_t L = d0; L <= lv_now; ++L) {
+        uint32_t cu = S.cur[L], en = S.end[L];
+        if (cu < en) emit(L, S.jmin[L], cu, en, (en - cu + a.chunk - 1) / a.chunk, L > 0 ? S.m[L - 1] : 0u);
+        if (L < L2)
+          for (uint32_t i = S.bi[L]; i < S.bn[L]; ++i) emit(L + 1, NONE, 0u, NONE, 1u, S.cand[L][i]);
+      }
+      return true;
+    };
+
+    const uint32_t off = c - uniform(TK->base);
+    const uint32_t tlo = uniform(TK->lo), thi = uniform(TK->hi);
+    uint32_t rlo = 0, rhi = NONE;
+    if (thi != NONE) {
+      rlo = tlo + off * a.chunk;
+      rhi = rlo + a.chunk;
+      rhi = rhi < thi ? rhi : thi;
+    }
+    const uint32_t tj = uniform(TK->jmin);
+    int lv = (int)d0;
+    bool live = enter(d0, rlo, rhi, tj == NONE ? -1 : (int)tj);
+    uint32_t work = 0;
+    bool may_spill = a.spill != nullptr;
     while (live && !stop) {
+      if (may_spill && ++work > a.budget_iters && (uint32_t)lv < L1) {
+        if (spill((uint32_t)lv)) break;
+        may_spill = false;
+      }
       const uint32_t L = (uint32_t)lv;
       if (L == L1) {
         // k == 3: the task's range is the last position itself
@@ -1026,7 +1109,7 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
           continue;
         }
         --lv;
-        if (lv < 2) break;
+        if (lv < (int)d0) break;
         continue;
       }
       uint32_t bi = S.bi[L], bn = S.bn[L];
@@ -1052,7 +1135,7 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
         continue;
       }
       --lv;
-      if (lv < 2) break;
+      if (lv < (int)d0) break;
     }
     flush(true);
     if (lane == 0 && HASH && hsum) atomicAdd(&a.hashes[(size_t)t * a.nq + q], hsum);
@@ -1070,10 +1153,22 @@ __global__ void k_batch_begin(Ctrl* c) {
     c->nseeds = 0;
     c->nchunks = 0;
     c->chunk_head = 0;
+    c->nspill = 0;
+    c->nspill_chunks = 0;
     c->nruns = 0;
     c->ndirty_next = 0;
     c->err = 0;
     c->err_at = 0xFFFFFFFFu;
+  }
+}
+
+// the spilled tasks of the last round become the input of the next
+__global__ void k_next_round(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    c->nchunks = c->nspill_chunks;
+    c->chunk_head = 0;
+    c->nspill = 0;
+    c->nspill_chunks = 0;
   }
 }
 
@@ -1089,7 +1184,7 @@ __global__ void k_batch_end(Ctrl* c) {
 // one directed data edge).  Vertices [v0, v1).
 __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, uint32_t qo,
                                 uint32_t la, uint32_t key01, uint32_t v0, uint32_t v1,
-                                uint32_t chunk, Seed* seeds, uint32_t seed_cap,
+                                uint32_t chunk, Task* seeds, uint32_t seed_cap,
                                 uint32_t* chunk_owner, uint32_t chunk_cap, Ctrl* ctrl,
                                 const uint32_t* __restrict__ d2, uint32_t d2_words, uint32_t symbi,
                                 unsigned long long* counts) {
@@ -1117,13 +1212,13 @@ __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, 
       }
       if (best == 0 || best == NONE) continue;
       uint32_t nch = (best + chunk - 1) / chunk;
-      uint32_t sidx = atomicAdd(&ctrl->nseeds, 1u);
-      if (sidx >= seed_cap) { atomicOr(&ctrl->err, ERR_SEED_OVERFLOW); continue; }
-      uint32_t base = atomicAdd(&ctrl->nchunks, nch);
-      if ((uint64_t)base + nch > chunk_cap) { atomicOr(&ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
-      Seed sd;
-      sd.t = 0; sd.qo = qo; sd.m0 = v; sd.m1 = w; sd.lo = blo; sd.hi = bhi; sd.base = base; sd.jmin = bj;
-      seeds[sidx] = sd;
+      uint32_t sidx = reserve(&ctrl->nseeds, 1u, seed_cap);
+      if (sidx == NONE) { atomicOr(&ctrl->err, ERR_SEED_OVERFLOW); continue; }
+      uint32_t base = reserve(&ctrl->nchunks, nch, chunk_cap);
+      if (base == NONE) { atomicOr(&ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
+      Task* tk = seeds + sidx;
+      tk->t = 0; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
+      tk->base = base; tk->nch = nch; tk->m[0] = v; tk->m[1] = w;
       for (uint32_t c = 0; c < nch; ++c) chunk_owner[base + c] = sidx;
     }
   }

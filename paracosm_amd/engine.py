@@ -52,7 +52,8 @@ class _Opts(ctypes.Structure):
                 ("max_batch", ctypes.c_uint32), ("want_hash", ctypes.c_uint32),
                 ("time_limit_s", ctypes.c_double), ("device_budget", ctypes.c_uint64),
                 ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32),
-                ("match_budget", ctypes.c_uint64)]
+                ("match_budget", ctypes.c_uint64), ("spill_budget", ctypes.c_uint32),
+                ("pad0", ctypes.c_uint32)]
 
 
 UPDATE_DTYPE = np.dtype([("type", np.uint8), ("is_add", np.uint8), ("pad", np.uint16),
@@ -145,13 +146,14 @@ class Matching:
                  max_results: int = 0, print_prep: bool = False, print_enum: bool = False,
                  homo: bool = False, threads: int = 0, auto_tuning: int = 1,
                  max_batch: int = 0, want_hash: bool = False, device_budget: int = 0,
-                 split_chunk: int = 0, device: int = 0, match_budget: int = 0):
+                 split_chunk: int = 0, device: int = 0, match_budget: int = 0,
+                 spill_budget: int = 0):
         L = lib()
         self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         d = _GraphArgs(vlabels, src, dst, elabels)
         o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
                   auto_tuning, max_batch, int(want_hash), 0.0, device_budget, split_chunk,
-                  device, match_budget)
+                  device, match_budget, spill_budget, 0)
         err = ctypes.c_int(0)
         self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
                               ctypes.byref(o), ctypes.byref(err))
@@ -251,10 +253,11 @@ class Matching:
         return int(a.value), int(b.value), int(c.value)
 
     def stats(self):
-        out = (ctypes.c_uint64 * 7)()
-        self._check(lib().csm_stats(self.h, out, 7))
+        out = (ctypes.c_uint64 * 8)()
+        self._check(lib().csm_stats(self.h, out, 8))
         return {"tasks": out[0], "candidates": out[1], "alg_bytes": out[2], "batches": out[3],
-                "repacks": out[4], "search_ns": out[5], "search_launches": out[6]}
+                "repacks": out[4], "search_ns": out[5], "search_launches": out[6],
+                "max_rounds": out[7]}
 
     def device_bytes(self) -> int:
         b = ctypes.c_uint64(0)

@@ -274,3 +274,25 @@ def test_match_budget_stops_only_queries_over_it(oracle_lib):
         st = m.query_status()
     assert st.tolist() == [0, 1]
     np.testing.assert_array_equal(pos[:, 0], rl.pos)
+
+
+@pytest.mark.parametrize("budget", [1, 2, 3, 17])
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_spilled_subtrees_are_exact(oracle_lib, budget, algo):
+    """Tasks that stop after `budget` DFS iterations hand their remaining
+    ranges and buffered candidates to the next round; counts and hashes stay
+    bit-exact however finely the search is cut."""
+    from paracosm_amd.engine import make_updates
+    for seed in (0, 3, 6):
+        vl, init, ups, q = _case(300 + seed, nv=40, p=0.35, n_vl=2, n_upd=60, k=5 + seed % 3,
+                                 extra=1)
+        ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+        with _engine(algo, q, vl, init, want_hash=True, spill_budget=budget, split_chunk=4) as m:
+            pos, neg, _, hsh = m.apply_batch(make_updates(*ups))
+            rounds = m.stats()["max_rounds"]
+        print("spill", algo, budget, seed, "rounds", rounds, "total",
+              int(ref.pos.sum() + ref.neg.sum()), flush=True)
+        np.testing.assert_array_equal(pos[:, 0], ref.pos)
+        np.testing.assert_array_equal(neg[:, 0], ref.neg)
+        np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+        assert budget > 5 or rounds > 1
