@@ -42,7 +42,7 @@ def pct(xs, p):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=15)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=62500)
     ap.add_argument("--algo", default="parallel_graphflow")
@@ -53,6 +53,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--split-chunk", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -63,6 +65,13 @@ def main() -> int:
     from paracosm_amd.engine import Matching, make_updates, lib
 
     lib()  # fail loudly if the engine is not built
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 268435456 bytes read and written")
+        del src, dst
     t0 = time.time()
     d = workloads.build(args.config, seed=args.seed, scale=args.scale)
     w = d.w

@@ -89,8 +89,8 @@ def candidates(d: Data, k: int, density: Optional[str], count: int, round_: int 
                               density)
 
 
-def admit(d: Data, *, sample: int = 256, budget_per_update: float = 1e4,
-          algo: str = "parallel_graphflow", max_rounds: int = 6, log=print) -> List[gen.Query]:
+def admit(d: Data, *, sample: int = 4096, budget_per_update: float = 3e3,
+          algo: str = "parallel_graphflow", max_rounds: int = 16, log=print) -> List[gen.Query]:
     """Admitted queries for every (size, density) slot of the config."""
     from .engine import Matching, make_updates
     w = d.w
