@@ -27,7 +27,7 @@ using namespace csm;
 
 namespace {
 
-enum Algo { ALGO_GRAPHFLOW = 0, ALGO_SYMBI = 1, ALGO_TURBOFLUX = 2 };
+enum Algo { ALGO_GRAPHFLOW = 0, ALGO_SYMBI = 1, ALGO_TURBOFLUX = 2, ALGO_NONE = 3 };
 
 constexpr uint32_t DEFAULT_MAX_BATCH = 65534;
 constexpr int BUILD_BS = 256;
@@ -234,6 +234,7 @@ int parse_algo(const char* a, int* out) {
   if (s == "parallel_graphflow" || s == "graphflow") *out = ALGO_GRAPHFLOW;
   else if (s == "parallel_symbi" || s == "symbi") *out = ALGO_SYMBI;
   else if (s == "parallel_turboflux" || s == "turboflux") *out = ALGO_TURBOFLUX;
+  else if (s == "none") *out = ALGO_NONE;  // the base plugin: updates only, no search
   else return CSM_E_UNKNOWN_ALGO;
   return CSM_OK;
 }
@@ -301,6 +302,7 @@ void build_seed_table(csm_engine* e) {
   const uint32_t nb = e->nlab * e->nlab * e->nel;
   std::vector<std::vector<uint32_t>> buckets(nb);
   for (auto& qp : e->queries) {
+    if (e->algo == ALGO_NONE) break;
     QueryInfo& qi = *qp;
     for (size_t oi = 0; oi < qi.orders.size(); ++oi) {
       uint32_t qo = qi.plan_base + (uint32_t)oi;
@@ -655,7 +657,7 @@ void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long 
   a.counts = counts;
   a.hashes = hashes;
   a.safe = safe;
-  k_seeds<<<grid_for(n, 256), 256, 0, e->stream>>>(a);
+  k_seeds<<<grid_for((uint64_t)n * WAVE, 256, 8192), 256, 0, e->stream>>>(a);
 }
 
 int check_errors(csm_engine* e) {
@@ -1220,6 +1222,10 @@ int csm_initial(csm_engine* e, uint64_t* num_results) {
   HIP_TRY(e, hipSetDevice(e->dev));
   QueryInfo& q = *e->queries[0];
   uint64_t total = 0;
+  if (e->algo == ALGO_NONE) {  // matching::InitialMatching does nothing
+    *num_results = 0;
+    return CSM_OK;
+  }
   if (q.g.ne == 0) {
     for (uint32_t v = 0; v < e->nv; ++v) total += (e->h_vlab[v] != NO_LABEL && e->h_vlab[v] == q.dlab[0]);
     *num_results = total;
@@ -1237,6 +1243,7 @@ int csm_classify(csm_engine* e, uint32_t v1, uint32_t v2, uint32_t label, int* s
   if (!e || !safe) return CSM_E_ARG;
   if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
   if (v1 >= e->nv || v2 >= e->nv) return fail(e, CSM_E_ARG, "vertex out of range");
+  if (e->algo == ALGO_NONE) { *safe = 0; return CSM_OK; }  // matching::Classify (matching.cpp:66-68)
   uint32_t l1 = e->h_vlab[v1], l2 = e->h_vlab[v2], el = dense_of(e->el_map, label);
   if (l1 == NO_LABEL || l2 == NO_LABEL || el == NO_LABEL) { *safe = 1; return CSM_OK; }
   uint32_t bucket = (l1 * e->nlab + l2) * e->nel + el;

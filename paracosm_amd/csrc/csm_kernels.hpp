@@ -86,10 +86,19 @@ struct Task {
   uint32_t jmin;
   uint32_t lo, hi;
   uint32_t base;     // first chunk id
-  uint32_t nch;      // chunks
+  uint32_t csize;    // candidates per chunk
   uint32_t m[MAXK];
 };
 constexpr uint32_t TASK_WORDS = sizeof(Task) / 4;
+constexpr uint32_t MAX_TASK_CHUNKS = 64;   // a task is cut into at most this many chunks
+
+// chunks of a range of n candidates: at least `chunk` each, at most MAX_TASK_CHUNKS
+__device__ __forceinline__ uint2 chunking(uint32_t n, uint32_t chunk) {
+  uint32_t nch = (n + chunk - 1) / chunk;
+  if (nch > MAX_TASK_CHUNKS) nch = MAX_TASK_CHUNKS;
+  if (nch == 0) nch = 1;
+  return make_uint2(nch, (n + nch - 1) / nch);
+}
 
 struct GraphView {
   const uint2* __restrict__ vrec;
@@ -585,13 +594,16 @@ struct SeedArgs {
   uint8_t* safe;
 };
 
-// One thread per (update, seed template): fix m[0], m[1], find the depth-2
-// candidate segment and emit ceil(|segment| / chunk) search tasks.  Updates
-// with no template are "safe" (the label-triple filter of
+// One wavefront per update, one lane per seed template: fix m[0], m[1], find
+// the depth-2 candidate segment (the smaller label-filtered segment of the
+// two endpoints) and emit the seed as a task cut into chunks.  Updates with
+// no template are "safe" (the label-triple filter of
 // parallel_graphflow.cpp:1990-2016).
 __global__ void k_seeds(SeedArgs a) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.b.n;
-       i += gridDim.x * blockDim.x) {
+  const uint32_t lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
+  for (uint32_t i = wid; i < a.b.n; i += nw) {
     uint32_t l = a.seed_label[i];
     uint32_t x = a.b.v1[i], y = a.b.v2[i];
     bool edge = a.b.type[i] == 'e' && l != NO_LABEL && x < a.g.nv && y < a.g.nv;
@@ -604,9 +616,8 @@ __global__ void k_seeds(SeedArgs a) {
         e = a.tab_off[bucket + 1];
       }
     }
-    if (a.safe) a.safe[i] = (s == e);
-    uint32_t pos = i + 1;
-    for (uint32_t j = s; j < e; ++j) {
+    if (lane == 0 && a.safe) a.safe[i] = (s == e);
+    for (uint32_t j = s + lane; j < e; j += WAVE) {
       uint32_t ent = a.tab_ent[j];
       uint32_t qo = ent >> 1;
       const PlanRec* P = a.plans + qo;
@@ -636,17 +647,16 @@ __global__ void k_seeds(SeedArgs a) {
         if (best == NONE || sz < best) { best = sz; blo = sg.x; bhi = sg.y; bj = jj; }
       }
       if (best == 0 || best == NONE) continue;
-      uint32_t nch = (best + a.chunk - 1) / a.chunk;
+      uint2 ch = chunking(best, a.chunk);
       uint32_t sidx = reserve(&a.ctrl->nseeds, 1u, a.seed_cap);
       if (sidx == NONE) { atomicOr(&a.ctrl->err, ERR_SEED_OVERFLOW); continue; }
-      uint32_t base = reserve(&a.ctrl->nchunks, nch, a.chunk_cap);
+      uint32_t base = reserve(&a.ctrl->nchunks, ch.x, a.chunk_cap);
       if (base == NONE) { atomicOr(&a.ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
       Task* tk = a.seeds + sidx;
       tk->t = i; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
-      tk->base = base; tk->nch = nch; tk->m[0] = m0; tk->m[1] = m1;
-      for (uint32_t c = 0; c < nch; ++c) a.chunk_owner[base + c] = sidx;
+      tk->base = base; tk->csize = ch.y; tk->m[0] = m0; tk->m[1] = m1;
+      for (uint32_t c = 0; c < ch.x; ++c) a.chunk_owner[base + c] = sidx;
     }
-    (void)pos;
   }
 }
 
@@ -1005,7 +1015,7 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
       uint32_t n_t = 0, n_c = 0;
       for (uint32_t L = d0; L <= lv_now; ++L) {
         uint32_t cu = S.cur[L], en = S.end[L];
-        if (cu < en) { ++n_t; n_c += (en - cu + a.chunk - 1) / a.chunk; }
+        if (cu < en) { ++n_t; n_c += chunking(en - cu, a.chunk).x; }
         if (L < L2) { uint32_t r = S.bn[L] - S.bi[L]; n_t += r; n_c += r; }
       }
       n_t = uniform(n_t);
@@ -1020,8 +1030,9 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
       cb = uniform(__shfl(cb, 0));
       if (tb == NONE || cb == NONE) return false;  // (a task slot may leak; never read)
       uint32_t ti = tb, ci = cb;
-      auto emit = [&](uint32_t depth, uint32_t jm, uint32_t lo, uint32_t hi, uint32_t nch,
+      auto emit = [&](uint32_t depth, uint32_t jm, uint32_t lo, uint32_t hi, uint2 ch,
                       uint32_t extra) {
+        const uint32_t nch = ch.x;
         Task* o = a.spill + ti;
         uint32_t word = 0;
         if (lane < TASK_WORDS) {
@@ -1032,7 +1043,7 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
           else if (lane == 4) word = lo;
           else if (lane == 5) word = hi;
           else if (lane == 6) word = ci;
-          else if (lane == 7) word = nch;
+          else if (lane == 7) word = ch.y;
           else {
             uint32_t p = lane - 8;
             word = p + 1 < depth ? S.m[p] : (p + 1 == depth ? extra : 0u);
@@ -1045,9 +1056,10 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
       };
       for (uint32_t L = d0; L <= lv_now; ++L) {
         uint32_t cu = S.cur[L], en = S.end[L];
-        if (cu < en) emit(L, S.jmin[L], cu, en, (en - cu + a.chunk - 1) / a.chunk, L > 0 ? S.m[L - 1] : 0u);
+        if (cu < en) emit(L, S.jmin[L], cu, en, chunking(en - cu, a.chunk), L > 0 ? S.m[L - 1] : 0u);
         if (L < L2)
-          for (uint32_t i = S.bi[L]; i < S.bn[L]; ++i) emit(L + 1, NONE, 0u, NONE, 1u, S.cand[L][i]);
+          for (uint32_t i = S.bi[L]; i < S.bn[L]; ++i)
+            emit(L + 1, NONE, 0u, NONE, make_uint2(1u, NONE), S.cand[L][i]);
       }
       return true;
     };
@@ -1056,8 +1068,9 @@ __global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
     const uint32_t tlo = uniform(TK->lo), thi = uniform(TK->hi);
     uint32_t rlo = 0, rhi = NONE;
     if (thi != NONE) {
-      rlo = tlo + off * a.chunk;
-      rhi = rlo + a.chunk;
+      const uint32_t cs = uniform(TK->csize);
+      rlo = tlo + off * cs;
+      rhi = rlo + cs;
       rhi = rhi < thi ? rhi : thi;
     }
     const uint32_t tj = uniform(TK->jmin);
@@ -1211,15 +1224,15 @@ __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, 
         if (best == NONE || sz < best) { best = sz; blo = s2.x; bhi = s2.y; bj = jj; }
       }
       if (best == 0 || best == NONE) continue;
-      uint32_t nch = (best + chunk - 1) / chunk;
+      uint2 ch = chunking(best, chunk);
       uint32_t sidx = reserve(&ctrl->nseeds, 1u, seed_cap);
       if (sidx == NONE) { atomicOr(&ctrl->err, ERR_SEED_OVERFLOW); continue; }
-      uint32_t base = reserve(&ctrl->nchunks, nch, chunk_cap);
+      uint32_t base = reserve(&ctrl->nchunks, ch.x, chunk_cap);
       if (base == NONE) { atomicOr(&ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
       Task* tk = seeds + sidx;
       tk->t = 0; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
-      tk->base = base; tk->nch = nch; tk->m[0] = v; tk->m[1] = w;
-      for (uint32_t c = 0; c < nch; ++c) chunk_owner[base + c] = sidx;
+      tk->base = base; tk->csize = ch.y; tk->m[0] = v; tk->m[1] = w;
+      for (uint32_t c = 0; c < ch.x; ++c) chunk_owner[base + c] = sidx;
     }
   }
 }
