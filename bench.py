@@ -52,6 +52,8 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--split-chunk", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (HBM counter calibration)")
@@ -81,6 +83,29 @@ def main() -> int:
     queries = workloads.admit(d, log=log)
     log(f"admitted {len(queries)} queries in {time.time() - t0:.1f}s: "
         f"k={[q.k for q in queries]} edges={[q.num_edges for q in queries]}")
+
+    # plain per-update GPU path on the stream's first updates (fresh engine)
+    P = min(args.plain_sample, (args.warmup + args.steps) * args.batch, w.num_updates)
+    mp = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    for q in queries[1:]:
+        mp.register_query(q)
+    mp.Preprocessing()
+    plat = []
+    pcounts = np.zeros(P, np.uint64)
+    tp0 = time.perf_counter()
+    for i in range(P):
+        if i % 500 == 0:
+            log(f"plain per-update: {i}/{P}")
+        ta = time.perf_counter()
+        if w.upd_add[i]:
+            pcounts[i] = mp.AddEdge(int(w.upd_v1[i]), int(w.upd_v2[i]), int(w.upd_label[i]))
+        else:
+            pcounts[i] = mp.RemoveEdge(int(w.upd_v1[i]), int(w.upd_v2[i]))
+        plat.append(time.perf_counter() - ta)
+    plain_elapsed = time.perf_counter() - tp0
+    mp.close()
+    plain_ups = P / plain_elapsed
+    log(f"plain per-update: {P} updates in {plain_elapsed:.3f}s = {plain_ups:.0f} updates/s")
 
     n_total = min(w.num_updates, (args.warmup + args.steps) * args.batch)
     ups = make_updates(w.upd_add[:n_total], w.upd_v1[:n_total], w.upd_v2[:n_total],
@@ -130,32 +155,34 @@ def main() -> int:
     value = n_done / elapsed
     log(f"batched: {n_done} updates in {elapsed:.3f}s = {value:.0f} updates/s, "
         f"{matches} matches ({matches / elapsed:.3e}/s)")
-    m.close()
-
-    # plain per-update GPU path on the stream's first updates (fresh engine)
-    P = min(args.plain_sample, n_total)
-    mp = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab)
-    for q in queries[1:]:
-        mp.register_query(q)
-    mp.Preprocessing()
-    plat = []
-    pcounts = np.zeros(P, np.uint64)
-    tp0 = time.perf_counter()
-    for i in range(P):
-        if i % 500 == 0:
-            log(f"plain per-update: {i}/{P}")
-        ta = time.perf_counter()
-        if w.upd_add[i]:
-            pcounts[i] = mp.AddEdge(int(w.upd_v1[i]), int(w.upd_v2[i]), int(w.upd_label[i]))
-        else:
-            pcounts[i] = mp.RemoveEdge(int(w.upd_v1[i]), int(w.upd_v2[i]))
-        plat.append(time.perf_counter() - ta)
-    plain_elapsed = time.perf_counter() - tp0
-    mp.close()
-    plain_ups = P / plain_elapsed
     exact = bool(np.array_equal(pcounts, counts[:P].sum(axis=1)))
-    log(f"plain per-update: {P} updates in {plain_elapsed:.3f}s = {plain_ups:.0f} updates/s; "
-        f"counts equal to batched: {exact}")
+    log(f"plain per-update counts equal to batched: {exact}")
+
+    # served path: the same engine continues the stream, requests of 1..4096
+    # updates arriving as a Poisson process, continuous batching
+    served = []
+    if not args.no_serve:
+        from paracosm_amd.serve import Server, poisson_requests
+        srv = Server(m, max_batch=args.batch)
+        cursor = n_total
+        for load in (10.0 * plain_ups, 0.8 * value):
+            n_srv = int(min(args.serve_seconds * load, w.num_updates - cursor))
+            if n_srv <= 0:
+                break
+            sl = slice(cursor, cursor + n_srv)
+            reqs = poisson_requests(make_updates(w.upd_add[sl], w.upd_v1[sl], w.upd_v2[sl],
+                                                 w.upd_label[sl]), load, seed=args.seed)
+            st = srv.serve(reqs)
+            cursor += n_srv
+            served.append({"offered_updates_per_s": round(load, 1),
+                           "updates_per_s": round(st.updates_per_s, 1),
+                           "p50_update_latency_ms": round(st.p50_ms, 3),
+                           "p99_update_latency_ms": round(st.p99_ms, 3),
+                           "requests": st.requests, "updates": st.updates,
+                           "batches": st.batches, "mean_batch": round(st.mean_batch, 1),
+                           "matches_per_s": round(st.matches / st.elapsed, 1)})
+            log(f"served at {load:.0f}/s offered: {served[-1]}")
+    m.close()
 
     # roofline of the search kernel: algorithmic bytes per launch over its
     # average launch duration (HIP events on the engine stream)
@@ -195,6 +222,8 @@ def main() -> int:
         "p50_update_latency_ms": round(1e3 * pct(lat, 50), 3),
         "p99_update_latency_ms": round(1e3 * pct(lat, 99), 3),
         "latency_note": "closed loop: each update's latency is its batch's latency",
+        "served": served,
+        "served_vs_plain": (round(served[0]["updates_per_s"] / plain_ups, 2) if served else None),
         "plain_updates_per_s": round(plain_ups, 1),
         "plain_p50_update_latency_ms": round(1e3 * pct(plat, 50), 4),
         "plain_p99_update_latency_ms": round(1e3 * pct(plat, 99), 4),
