@@ -62,18 +62,12 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         return 2
 
-    import torch
+    # One HIP runtime per process: device memory comes from the engine's
+    # runtime (the library links the system HIP; no second runtime is loaded).
     from paracosm_amd import workloads
     from paracosm_amd.engine import Matching, make_updates, lib
 
     lib()  # fail loudly if the engine is not built
-    if args.copy_calibration:
-        src = torch.ones(64 << 20, dtype=torch.int32, device="cuda:0")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy calibration: 268435456 bytes read and written")
-        del src, dst
     t0 = time.time()
     d = workloads.build(args.config, seed=args.seed, scale=args.scale)
     w = d.w
@@ -110,34 +104,38 @@ def main() -> int:
     n_total = min(w.num_updates, (args.warmup + args.steps) * args.batch)
     ups = make_updates(w.upd_add[:n_total], w.upd_v1[:n_total], w.upd_v2[:n_total],
                        w.upd_label[:n_total])
-    dev = torch.device("cuda:0")
-    d_ups = torch.from_numpy(ups.view(np.uint8).copy()).to(dev)
     nq = len(queries)
-    d_counts = torch.zeros(n_total * nq, dtype=torch.int64, device=dev)
-
     m = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
                  max_batch=args.batch, split_chunk=args.split_chunk)
     for q in queries[1:]:
         m.register_query(q)
     m.Preprocessing()
+    if args.copy_calibration:
+        cal = m.device_buffer(512 << 20)
+        cal.copy_within(256 << 20, 0, 256 << 20)
+        log("copy calibration: one device copy read 268435456 bytes and wrote 268435456")
+        cal.close()
     rec = 16
+    # the whole timed input resident in HBM before timing starts
+    d_ups = m.device_buffer(n_total * rec)
+    d_ups.upload(ups.view(np.uint8))
+    d_counts = m.device_buffer(n_total * nq * 8)
 
     def step(i):
         s = i * args.batch
         n = min(args.batch, n_total - s)
-        m.apply_batch_device(d_ups.data_ptr() + s * rec, n, d_counts.data_ptr() + s * nq * 8)
+        m.apply_batch_device(d_ups.ptr + s * rec, n, d_counts.ptr + s * nq * 8)
         return n
 
     for i in range(args.warmup):
         tw = time.perf_counter()
         step(i)
         log(f"warmup step {i}: {time.perf_counter() - tw:.3f}s {m.stats()}")
-    torch.cuda.synchronize()
     m.sync()
     st0 = m.stats()
     lat = []
     n_done = 0
-    torch.cuda.synchronize()
+    m.sync()  # barrier: nothing of the warm-up is still running
     t_start = time.perf_counter()
     for i in range(args.warmup, args.warmup + args.steps):
         tb = time.perf_counter()
@@ -145,11 +143,10 @@ def main() -> int:
         lat.append(time.perf_counter() - tb)
         log(f"step {i}: {lat[-1]:.3f}s")
     m.sync()
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     st1 = m.stats()
     log(f"engine stats after the timed steps: {st1}")
-    counts = d_counts.cpu().numpy().astype(np.uint64).reshape(n_total, nq)
+    counts = d_counts.download(np.zeros(n_total * nq, np.uint64)).reshape(n_total, nq)
     timed = counts[args.warmup * args.batch: args.warmup * args.batch + n_done]
     matches = int(timed.sum())
     value = n_done / elapsed
@@ -182,6 +179,7 @@ def main() -> int:
                            "batches": st.batches, "mean_batch": round(st.mean_batch, 1),
                            "matches_per_s": round(st.matches / st.elapsed, 1)})
             log(f"served at {load:.0f}/s offered: {served[-1]}")
+        srv.close()
     m.close()
 
     # roofline of the search kernel: algorithmic bytes per launch over its

@@ -170,6 +170,19 @@ int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
  * 6 n2 (per query vertex, per neighbour in id order, [nv] each). */
 int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, size_t n);
 
+/* Device and pinned-host buffers on the engine's device and runtime, for
+ * callers that keep inputs resident in HBM without a HIP runtime of their
+ * own (one HIP runtime per process).  Device bytes count towards
+ * csm_device_bytes.  Copies are synchronous on the engine stream.  No
+ * reference counterpart. */
+int csm_device_alloc(csm_engine* e, size_t bytes, void** ptr);
+int csm_device_free(csm_engine* e, void* ptr);
+int csm_host_alloc(csm_engine* e, size_t bytes, void** ptr);
+int csm_host_free(csm_engine* e, void* ptr);
+int csm_copy_to_device(csm_engine* e, void* dst, const void* src, size_t bytes);
+int csm_copy_to_host(csm_engine* e, void* dst, const void* src, size_t bytes);
+int csm_copy_device(csm_engine* e, void* dst, const void* src, size_t bytes);
+
 /* The HIP stream the engine launches on (hipStream_t). */
 void* csm_stream(csm_engine* e);
 

@@ -1478,6 +1478,62 @@ int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, si
   return CSM_OK;
 }
 
+int csm_device_alloc(csm_engine* e, size_t bytes, void** ptr) {
+  if (!e || !ptr) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  uint8_t* p = nullptr;
+  int rc = dalloc(e, &p, bytes);
+  if (rc) return rc;
+  *ptr = p;
+  return CSM_OK;
+}
+
+int csm_device_free(csm_engine* e, void* ptr) {
+  if (!e) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  uint8_t* p = static_cast<uint8_t*>(ptr);
+  dfree(e, p);
+  return CSM_OK;
+}
+
+int csm_host_alloc(csm_engine* e, size_t bytes, void** ptr) {
+  if (!e || !ptr) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  HIP_TRY(e, hipHostMalloc(ptr, std::max<size_t>(bytes, 64)));
+  return CSM_OK;
+}
+
+int csm_host_free(csm_engine* e, void* ptr) {
+  if (!e) return CSM_E_ARG;
+  if (ptr) HIP_TRY(e, hipHostFree(ptr));
+  return CSM_OK;
+}
+
+int csm_copy_to_device(csm_engine* e, void* dst, const void* src, size_t bytes) {
+  if (!e || (bytes && (!dst || !src))) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  HIP_TRY(e, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
+int csm_copy_to_host(csm_engine* e, void* dst, const void* src, size_t bytes) {
+  if (!e || (bytes && (!dst || !src))) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  HIP_TRY(e, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
+int csm_copy_device(csm_engine* e, void* dst, const void* src, size_t bytes) {
+  if (!e || (bytes && (!dst || !src))) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  HIP_TRY(e, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
 void* csm_stream(csm_engine* e) { return e ? (void*)e->stream : nullptr; }
 
 const char* csm_last_error(const csm_engine* e) { return e ? e->err.c_str() : "null engine"; }

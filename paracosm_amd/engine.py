@@ -91,6 +91,13 @@ def lib():
             "csm_device_bytes": (ctypes.c_int, [vp, P(u64)]),
             "csm_stats": (ctypes.c_int, [vp, P(u64), sz]),
             "csm_index_dump": (ctypes.c_int, [vp, u32, u32, vp, sz]),
+            "csm_device_alloc": (ctypes.c_int, [vp, sz, P(vp)]),
+            "csm_device_free": (ctypes.c_int, [vp, vp]),
+            "csm_host_alloc": (ctypes.c_int, [vp, sz, P(vp)]),
+            "csm_host_free": (ctypes.c_int, [vp, vp]),
+            "csm_copy_to_device": (ctypes.c_int, [vp, vp, vp, sz]),
+            "csm_copy_to_host": (ctypes.c_int, [vp, vp, vp, sz]),
+            "csm_copy_device": (ctypes.c_int, [vp, vp, vp, sz]),
             "csm_stream": (vp, [vp]),
             "csm_last_error": (ctypes.c_char_p, [vp]),
             "csm_destroy": (None, [vp]),
@@ -108,7 +115,9 @@ def exported_symbols() -> Sequence[str]:
             "csm_initial", "csm_classify", "csm_add_edge", "csm_remove_edge",
             "csm_add_vertex", "csm_remove_vertex", "csm_apply_batch",
             "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_totals", "csm_memory",
-            "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_stream", "csm_last_error",
+            "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
+            "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
+            "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_last_error",
             "csm_destroy")
 
 
@@ -270,6 +279,13 @@ class Matching:
         self._check(lib().csm_index_dump(self.h, qid, what, out.ctypes.data, out.size))
         return out if what == 0 else out.reshape(rows, nv)
 
+    # -- device buffers on the engine's own HIP runtime (no second runtime)
+    def device_buffer(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def host_buffer(self, nbytes: int) -> "HostBuffer":
+        return HostBuffer(self, nbytes)
+
     def stream(self) -> int:
         return int(lib().csm_stream(self.h) or 0)
 
@@ -289,7 +305,7 @@ class Matching:
 
     def apply_batch_device(self, d_updates_ptr: int, n: int, d_counts_ptr: int,
                            d_safe_ptr: int = 0, d_hash_ptr: int = 0) -> None:
-        """Device pointers (e.g. torch tensors' data_ptr()) in HBM; results
+        """Device pointers (e.g. DeviceBuffer.ptr) in HBM; results
         stay on the device.  Synchronous with respect to the engine's stream's
         control read-back."""
         self._check(lib().csm_apply_batch_device(self.h, d_updates_ptr, n, d_counts_ptr,
@@ -297,6 +313,57 @@ class Matching:
 
     def sync(self) -> None:
         self._check(lib().csm_sync(self.h))
+
+
+class DeviceBuffer:
+    """HBM buffer allocated through the engine (freed with the engine or close())."""
+
+    def __init__(self, m: Matching, nbytes: int):
+        self.m = m
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p(0)
+        m._check(lib().csm_device_alloc(m.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = int(p.value)
+
+    def upload(self, arr: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(arr)
+        assert offset + a.nbytes <= self.nbytes
+        self.m._check(lib().csm_copy_to_device(self.m.h, self.ptr + offset, a.ctypes.data, a.nbytes))
+
+    def download(self, out: np.ndarray, offset: int = 0) -> np.ndarray:
+        assert out.flags["C_CONTIGUOUS"] and offset + out.nbytes <= self.nbytes
+        self.m._check(lib().csm_copy_to_host(self.m.h, out.ctypes.data, self.ptr + offset,
+                                             out.nbytes))
+        return out
+
+    def copy_within(self, dst_offset: int, src_offset: int, nbytes: int) -> None:
+        """Device-to-device copy inside the buffer (one copy kernel)."""
+        assert max(dst_offset, src_offset) + nbytes <= self.nbytes
+        self.m._check(lib().csm_copy_device(self.m.h, self.ptr + dst_offset,
+                                            self.ptr + src_offset, nbytes))
+
+    def close(self) -> None:
+        if self.ptr and self.m.h:
+            lib().csm_device_free(self.m.h, self.ptr)
+        self.ptr = 0
+
+
+class HostBuffer:
+    """Pinned host buffer allocated through the engine's runtime, viewable
+    as a numpy array."""
+
+    def __init__(self, m: Matching, nbytes: int):
+        self.m = m
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p(0)
+        m._check(lib().csm_host_alloc(m.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = int(p.value)
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * self.nbytes).from_address(self.ptr))
+
+    def close(self) -> None:
+        if self.ptr and self.m.h:
+            lib().csm_host_free(self.m.h, self.ptr)
+        self.ptr = 0
 
 
 def run_stream(m: Matching, add, v1, v2, label, batch: Optional[int] = None):

@@ -74,22 +74,23 @@ def poisson_requests(updates: np.ndarray, offered_per_s: float, seed: int = 0,
 
 
 class Server:
-    def __init__(self, engine: Matching, *, max_batch: int = 65534, hbm_budget: int = 0,
-                 device: str = "cuda:0"):
-        import torch
-        self.torch = torch
+    def __init__(self, engine: Matching, *, max_batch: int = 65534, hbm_budget: int = 0):
         self.m = engine
         self.nq = engine.num_queries
         self.max_batch = max_batch
         self.hbm_budget = hbm_budget
-        self.dev = torch.device(device)
-        self.d_upd = torch.empty(max_batch * REC_BYTES, dtype=torch.uint8, device=self.dev)
-        self.d_cnt = torch.empty(max_batch * self.nq, dtype=torch.int64, device=self.dev)
-        self.d_safe = torch.empty(max_batch, dtype=torch.uint8, device=self.dev)
-        self.h_upd = torch.empty(max_batch * REC_BYTES, dtype=torch.uint8).pin_memory()
-        self.h_cnt = torch.empty(max_batch * self.nq, dtype=torch.int64).pin_memory()
-        self.h_safe = torch.empty(max_batch, dtype=torch.uint8).pin_memory()
+        # device and pinned buffers through the engine's own HIP runtime
+        self.d_upd = engine.device_buffer(max_batch * REC_BYTES)
+        self.d_cnt = engine.device_buffer(max_batch * self.nq * 8)
+        self.d_safe = engine.device_buffer(max_batch)
+        self.h_upd = engine.host_buffer(max_batch * REC_BYTES)
+        self.h_cnt = engine.host_buffer(max_batch * self.nq * 8)
+        self.h_safe = engine.host_buffer(max_batch)
         self.refused = 0
+
+    def close(self) -> None:
+        for b in (self.h_upd, self.h_cnt, self.h_safe):
+            b.close()
 
     def working_set(self, n: int) -> int:
         """Device bytes a batch of n updates needs beyond the engine's own."""
@@ -101,24 +102,21 @@ class Server:
         return self.m.device_bytes() + self.working_set(n) <= self.hbm_budget
 
     def run_batch(self, reqs: Sequence[Request], t0: float) -> None:
-        torch = self.torch
         ups = np.concatenate([r.updates for r in reqs]) if len(reqs) > 1 else reqs[0].updates
         n = len(ups)
-        hv = self.h_upd.numpy()
-        hv[: n * REC_BYTES] = np.ascontiguousarray(ups, UPDATE_DTYPE).view(np.uint8)
-        self.d_upd[: n * REC_BYTES].copy_(self.h_upd[: n * REC_BYTES], non_blocking=True)
-        torch.cuda.synchronize()
-        self.m.apply_batch_device(self.d_upd.data_ptr(), n, self.d_cnt.data_ptr(),
-                                  self.d_safe.data_ptr())
-        self.h_cnt[: n * self.nq].copy_(self.d_cnt[: n * self.nq])
-        self.h_safe[:n].copy_(self.d_safe[:n])
+        nb = n * REC_BYTES
+        self.h_upd.array[:nb] = np.ascontiguousarray(ups, UPDATE_DTYPE).view(np.uint8)
+        self.d_upd.upload(self.h_upd.array[:nb])
+        self.m.apply_batch_device(self.d_upd.ptr, n, self.d_cnt.ptr, self.d_safe.ptr)
+        self.d_cnt.download(self.h_cnt.array[: n * self.nq * 8])
+        self.d_safe.download(self.h_safe.array[:n])
         done = time.perf_counter() - t0
-        cnt = self.h_cnt[: n * self.nq].numpy().astype(np.uint64).reshape(n, self.nq)
-        safe = self.h_safe[:n].numpy().copy()
+        cnt = self.h_cnt.array[: n * self.nq * 8].view(np.uint64).reshape(n, self.nq).copy()
+        safe = self.h_safe.array[:n].copy()
         o = 0
         for r in reqs:
             k = len(r.updates)
-            r.counts = cnt[o:o + k].copy()
+            r.counts = cnt[o:o + k]
             r.safe = safe[o:o + k]
             r.done = done
             o += k
