@@ -27,12 +27,12 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_lib() -> str:
+def build_lib(out: str = LIB, extra=()) -> str:
     src = os.path.join(HERE, "csrc", "csm_engine.hip")
-    tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", src, "-o", tmp])
-    os.replace(tmp, LIB)
-    return LIB
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, *extra, "-shared", src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
 
 
 def build_cli() -> str:

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -63,7 +64,7 @@ struct csm_engine {
   int dev = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
-  int search_blocks = 0;
+  uint32_t max_k = 0;       // largest registered query (selects the search LDS layout)
   std::string err;
 
   // labels
@@ -249,6 +250,7 @@ int build_query(csm_engine* e, const csm_graph* q, std::unique_ptr<QueryInfo>& o
   qi->g.init(q->num_vertices, q->vlabels);
   for (size_t i = 0; i < q->num_edges; ++i) qi->g.add_edge(q->src[i], q->dst[i], q->elabels[i]);
   if (!qi->g.connected()) return fail(e, CSM_E_ARG, "query graph is not connected");
+  e->max_k = std::max(e->max_k, qi->g.k);
   out = std::move(qi);
   return CSM_OK;
 }
@@ -551,21 +553,31 @@ BatchView batch_view(const csm_engine* e, uint32_t n) {
   return b;
 }
 
-template <bool V, bool H, bool S>
+template <int MK, bool H, bool S>
+void launch_search_mk(csm_engine* e, const SearchArgs& sa) {
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search<MK, true, H, S>,
+                                                     SEARCH_WAVES * WAVE, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 2;
+  }
+  k_search<MK, true, H, S><<<e->num_cus * per_cu, SEARCH_WAVES * WAVE, 0, e->stream>>>(sa);
+}
+
+// the smallest LDS state that holds the largest registered query
+template <bool H, bool S>
 void launch_search(csm_engine* e, const SearchArgs& sa) {
-  k_search<V, H, S><<<e->search_blocks, SEARCH_WAVES * WAVE, 0, e->stream>>>(sa);
+  if (e->max_k <= 8) launch_search_mk<8, H, S>(e, sa);
+  else launch_search_mk<MAXK, H, S>(e, sa);
 }
 
 void launch_round(csm_engine* e, SearchArgs& sa, bool versioned, bool h) {
   const bool s = e->algo == ALGO_SYMBI && e->d_d2 != nullptr;
   if (e->ev_search[0]) (void)hipEventRecord(e->ev_search[0], e->stream);
-  if (versioned) {
-    if (h) { if (s) launch_search<true, true, true>(e, sa); else launch_search<true, true, false>(e, sa); }
-    else { if (s) launch_search<true, false, true>(e, sa); else launch_search<true, false, false>(e, sa); }
-  } else {
-    if (h) { if (s) launch_search<false, true, true>(e, sa); else launch_search<false, true, false>(e, sa); }
-    else { if (s) launch_search<false, false, true>(e, sa); else launch_search<false, false, false>(e, sa); }
-  }
+  (void)versioned;  // every list carries version words; clean lists skip them
+  if (h) { if (s) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa); }
+  else { if (s) launch_search<false, true>(e, sa); else launch_search<false, false>(e, sa); }
   if (e->ev_search[1]) {
     (void)hipEventRecord(e->ev_search[1], e->stream);
     e->ev_pending = true;
@@ -602,13 +614,20 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
   sa.budget_iters = e->budget_iters;
   const bool h = hashes != nullptr;
   int out = 0;
+  static const bool trace = getenv("CSM_TRACE_ROUNDS") != nullptr;
   for (uint32_t round = 0;; ++round) {
     sa.spill = e->d_spill[out];
     sa.spill_owner = e->d_spill_owner[out];
+    uint32_t chunks_in = trace ? e->h_ctrl->nspill_chunks : 0;
     launch_round(e, sa, versioned, h);
     int rc = read_ctrl(e);
     if (rc) return rc;
+    uint64_t ns0 = e->stats[5];
     collect_search_time(e);
+    if (trace)
+      fprintf(stderr, "[csm] round %u: %s %u chunks, %.3f ms, spilled %u tasks / %u chunks\n",
+              round, round ? "spilled" : "seed", round ? chunks_in : e->h_ctrl->nchunks,
+              (e->stats[5] - ns0) / 1e6, e->h_ctrl->nspill, e->h_ctrl->nspill_chunks);
     if (e->h_ctrl->nspill == 0) {
       e->max_rounds = std::max(e->max_rounds, round + 1);
       return CSM_OK;
@@ -1081,11 +1100,7 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
     set(CSM_E_DEVICE);
     return nullptr;
   }
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search<true, false, false>,
-                                                   SEARCH_WAVES * WAVE, 0) != hipSuccess || per_cu < 1)
-    per_cu = 2;
-  e->search_blocks = e->num_cus * per_cu;
+
 
   // label maps from data and query labels
   std::vector<uint32_t> vls(data->vlabels, data->vlabels + data->num_vertices);

@@ -684,15 +684,16 @@ struct SearchArgs {
   unsigned long long budget;  // per-query match budget in this launch (0: none)
 };
 
-// Per-wave DFS state in LDS.
+// Per-wave DFS state in LDS, sized for queries of up to MK vertices.
+template <int MK>
 struct WaveState {
-  uint32_t m[MAXK];           // data vertex at each position
-  uint32_t mq[MAXK];          // data vertex of each query vertex (hash)
-  uint32_t cur[MAXK], end[MAXK];
-  uint32_t bn[MAXK], bi[MAXK];
-  uint32_t jmin[MAXK], pdirty[MAXK], udirty[MAXK];
-  uint32_t plo[MAXK][MAXNB], phi[MAXK][MAXNB];
-  uint32_t cand[MAXK][WAVE];
+  uint32_t m[MK];             // data vertex at each position
+  uint32_t mq[MK];            // data vertex of each query vertex (hash)
+  uint32_t cur[MK], end[MK];
+  uint32_t bn[MK], bi[MK];
+  uint32_t jmin[MK], pdirty[MK], udirty[MK];
+  uint32_t plo[MK][MK - 1], phi[MK][MK - 1];
+  uint32_t cand[MK][WAVE];
   // fused last position: per-parent segments of the current parent buffer
   uint32_t olo[WAVE], ohi[WAVE], pref[WAVE + 1];
   uint32_t src[WAVE];
@@ -716,12 +717,16 @@ struct WaveState {
 //   * otherwise each lane finds its parent's own segment, the smaller of the
 //     own and the fixed segment is the candidate source, and the wave walks
 //     the concatenation of all parents' sources 64 entries at a time.
-template <bool VERSIONED, bool HASH, bool SYMBI>
-__global__ __launch_bounds__(SEARCH_WAVES * WAVE) void k_search(SearchArgs a) {
-  __shared__ WaveState S_all[SEARCH_WAVES];
+#ifndef CSM_SEARCH_WAVES_PER_EU
+#define CSM_SEARCH_WAVES_PER_EU 5
+#endif
+template <int MK, bool VERSIONED, bool HASH, bool SYMBI>
+__global__ __launch_bounds__(SEARCH_WAVES * WAVE)
+__attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) void k_search(SearchArgs a) {
+  __shared__ WaveState<MK> S_all[SEARCH_WAVES];
   const uint32_t lane = lane_id();
   const uint64_t lt_mask = (1ull << lane) - 1ull;
-  WaveState& S = S_all[threadIdx.x / WAVE];
+  WaveState<MK>& S = S_all[threadIdx.x / WAVE];
   const uint32_t* __restrict__ adj = a.g.adj;
   const uint32_t* __restrict__ ats = a.g.ats;
   const uint32_t vbits = a.g.vbits, vmask = a.g.vmask;

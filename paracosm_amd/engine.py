@@ -17,7 +17,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcsm_mi355x.so")
+# CSM_LIB selects another build of the engine library (A/B measurements)
+LIB_PATH = os.environ.get("CSM_LIB") or os.path.join(_HERE, "libcsm_mi355x.so")
 
 CSM_OK = 0
 CSM_E_ARG = -1

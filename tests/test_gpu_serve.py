@@ -67,8 +67,8 @@ def test_hbm_budget_queues_requests(oracle_lib):
     got = np.concatenate([r.counts for r in reqs])
     np.testing.assert_array_equal(got, want)
     assert st.refused_for_memory > 0 and st.mean_batch <= 100
+    from paracosm_amd.serve import Request
     srv.hbm_budget = m.device_bytes() + srv.working_set(1)
-    big = poisson_requests(make_updates(*ups)[:50], 1e9, seed=3, max_request=4096)
     with pytest.raises(MemoryError):
-        srv.serve(big[:1])
+        srv.serve([Request(arrival=0.0, updates=make_updates(*ups)[:50])])
     m.close()
