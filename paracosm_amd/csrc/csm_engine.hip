@@ -85,6 +85,12 @@ struct csm_engine {
   Ctrl* h_ctrl = nullptr;  // pinned
   uint32_t* d_dirty[2] = {nullptr, nullptr};
   int dcur = 0;            // d_dirty[dcur] = list compacted at the next batch start
+  // key directories (segment offsets per key) of lists with >= dir_threshold entries
+  uint32_t* d_dslot = nullptr;
+  uint32_t* d_dir = nullptr;
+  uint32_t dir_cap = 0, dstride = 0, dir_threshold = 0;
+  uint32_t last_batch = 0;  // updates in the previous batch (bounds its compaction list)
+  unsigned long long trace_life0 = 0, trace_waves0 = 0;  // phase-profile trace baselines
 
   // batch buffers
   uint32_t max_batch = DEFAULT_MAX_BATCH;
@@ -223,6 +229,9 @@ GraphView view(const csm_engine* e) {
   g.vlab = e->d_vlab;
   g.adj = e->d_adj[e->cur];
   g.ats = e->d_ats[e->cur];
+  g.dslot = e->d_dslot;
+  g.dir = e->d_dir;
+  g.dstride = e->dstride;
   g.vbits = e->vbits;
   g.vmask = e->vmask;
   g.nel = e->nel;
@@ -387,6 +396,8 @@ int sort_keys(csm_engine* e, uint64_t* in, uint64_t* out, uint32_t* vin, uint32_
 }
 
 // ------------------------------------------------------------ graph build
+int dir_init(csm_engine* e, uint64_t cap);
+
 int build_graph(csm_engine* e, const csm_graph* d) {
   const uint64_t m = d->num_edges;
   const uint64_t n2 = 2 * m;
@@ -436,6 +447,50 @@ int build_graph(csm_engine* e, const csm_graph* d) {
   HIP_TRY(e, hipMemcpy(e->d_ctrl, e->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice));
   dfree(e, d_src); dfree(e, d_dst); dfree(e, d_el);
   dfree(e, k1); dfree(e, k2); dfree(e, v1); dfree(e, v2); dfree(e, vend);
+  // key directories of the long lists
+  uint64_t nlong = 0;
+  for (auto& r : rec) nlong += (r.y & LEN_MASK) >= e->dir_threshold;
+  return dir_init(e, nlong + nlong / 4 + e->nv_cap / 64 + 1024);
+}
+
+// Allocate the key directory pool for `cap` slots and build the directories
+// of every list long enough to get one.
+int dir_init(csm_engine* e, uint64_t cap) {
+  int rc;
+  e->dir_cap = (uint32_t)std::min<uint64_t>(cap, e->nv_cap);
+  if ((rc = dalloc(e, &e->d_dslot, e->nv_cap)) ||
+      (rc = dalloc(e, &e->d_dir, (size_t)e->dir_cap * e->dstride)))
+    return rc;
+  HIP_TRY(e, hipMemsetAsync(e->d_dslot, 0xFF, (size_t)e->nv_cap * 4, e->stream));
+  k_dir_build<DIR_ALL><<<grid_for(e->nv_cap, 256 / WAVE, 8192), 256, 0, e->stream>>>(
+      e->d_vrec, e->d_adj[e->cur], e->d_dslot, e->d_dir, e->dstride, e->vbits, e->dir_threshold,
+      nullptr, nullptr, nullptr, e->nv_cap, e->d_ctrl, e->dir_cap);
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  HIP_TRY(e, hipMemcpy(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  return CSM_OK;
+}
+
+// Make room for the slots one batch of n updates can hand out: at most one per
+// rewritten list, i.e. per merged vertex (2n) and per vertex compacted from the
+// previous batch (2 per update).  Without room the extra lists keep using
+// binary search, so a budget that forbids the growth is not an error.
+int dir_reserve(csm_engine* e, uint32_t n) {
+  const uint64_t top = e->h_ctrl->dir_top;
+  const uint64_t need = top + 2ull * n + 2ull * e->last_batch + 1024;
+  e->last_batch = n;
+  if (need <= e->dir_cap || e->dir_cap >= e->nv_cap) return CSM_OK;
+  const uint64_t ncap = std::min<uint64_t>(std::max<uint64_t>(2ull * e->dir_cap, need), e->nv_cap);
+  const size_t bytes = (size_t)ncap * e->dstride * 4;
+  if (e->opts.device_budget && e->dev_bytes + bytes > e->opts.device_budget) return CSM_OK;
+  uint32_t* nd = nullptr;
+  int rc;
+  if ((rc = dalloc(e, &nd, (size_t)ncap * e->dstride))) return rc;
+  HIP_TRY(e, hipMemcpyAsync(nd, e->d_dir, (size_t)std::min<uint64_t>(top, e->dir_cap) * e->dstride * 4,
+                            hipMemcpyDeviceToDevice, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  dfree(e, e->d_dir);
+  e->d_dir = nd;
+  e->dir_cap = (uint32_t)ncap;
   return CSM_OK;
 }
 
@@ -561,6 +616,9 @@ void launch_search_mk(csm_engine* e, const SearchArgs& sa) {
                                                      SEARCH_WAVES * WAVE, 0) != hipSuccess ||
         per_cu < 1)
       per_cu = 2;
+    if (getenv("CSM_TRACE_ROUNDS"))
+      fprintf(stderr, "[csm] search kernel MK=%d: %d workgroups per CU, %d CUs\n", MK, per_cu,
+              e->num_cus);
   }
   k_search<MK, true, H, S><<<e->num_cus * per_cu, SEARCH_WAVES * WAVE, 0, e->stream>>>(sa);
 }
@@ -624,12 +682,33 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
     if (rc) return rc;
     uint64_t ns0 = e->stats[5];
     collect_search_time(e);
-    if (trace)
+    if (trace) {
       fprintf(stderr, "[csm] round %u: %s %u chunks, %.3f ms, spilled %u tasks / %u chunks\n",
               round, round ? "spilled" : "seed", round ? chunks_in : e->h_ctrl->nchunks,
               (e->stats[5] - ns0) / 1e6, e->h_ctrl->nspill, e->h_ctrl->nspill_chunks);
+      unsigned long long& life0 = e->trace_life0;
+      unsigned long long& waves0 = e->trace_waves0;
+      const Ctrl* c = e->h_ctrl;
+      if (c->prof[29] > waves0 && c->prof[31] > c->prof[30]) {
+        const double span = (double)(c->prof[31] - c->prof[30]);
+        const double nw = (double)(c->prof[29] - waves0);
+        fprintf(stderr, "[csm]   %.0f waves, span %.3f ms, busy fraction %.3f\n", nw, span * 1e-5,
+                (double)(c->prof[28] - life0) / (span * nw));
+        life0 = c->prof[28];
+        waves0 = c->prof[29];
+      }
+    }
     if (e->h_ctrl->nspill == 0) {
       e->max_rounds = std::max(e->max_rounds, round + 1);
+      if (trace && e->h_ctrl->prof[8]) {
+        static const char* names[8] = {"fetch", "enter", "prepare_last", "refill",
+                                       "refill_k-2", "leaves_shared", "leaves_own", "spill"};
+        for (int i = 0; i < 8; ++i)
+          fprintf(stderr, "[csm] phase %-13s %14llu cycles %12llu calls\n", names[i],
+                  e->h_ctrl->prof[i], e->h_ctrl->prof[8 + i]);
+        for (int i = 16; i < 32; ++i)
+          fprintf(stderr, "[csm] counter %2d %16llu\n", i - 16, e->h_ctrl->prof[i]);
+      }
       return CSM_OK;
     }
     k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl);
@@ -701,6 +780,8 @@ int symbi_batch_delete(csm_engine* e, uint32_t n);
 int symbi_add_vertex(csm_engine* e, uint32_t id);
 
 // The batch pipeline; updates already unpacked into d_type/d_add/d_v1/d_v2/d_lab.
+int dir_reserve(csm_engine* e, uint32_t n);
+
 int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
               uint8_t* safe) {
   int rc;
@@ -711,6 +792,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
     if ((uint64_t)e->h_ctrl->pool_top + 2 * live + 2ull * n + 1024 > e->pool_cap)
       return fail(e, CSM_E_CAPACITY, "adjacency pool too small");
   }
+  if ((rc = dir_reserve(e, n))) return rc;
   const size_t nq = e->queries.size();
   HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
   if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
@@ -738,6 +820,13 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   k_merge<APPLY_BS><<<2048, APPLY_BS, 0, e->stream>>>(
       e->d_vrec, e->d_adj[e->cur], e->d_ats[e->cur], e->d_keys2, e->d_vals2, 2 * n, e->d_runs,
       e->d_ctrl, e->d_dirty[1 - e->dcur], (uint32_t)e->pool_cap);
+  // key directories of the lists rewritten by compaction and merge
+  k_dir_build<DIR_LIST><<<1024, 256, 0, e->stream>>>(
+      e->d_vrec, e->d_adj[e->cur], e->d_dslot, e->d_dir, e->dstride, e->vbits, e->dir_threshold,
+      e->d_dirty[e->dcur], &e->d_ctrl->ndirty_cur, nullptr, 0, e->d_ctrl, e->dir_cap);
+  k_dir_build<DIR_RUNS><<<1024, 256, 0, e->stream>>>(
+      e->d_vrec, e->d_adj[e->cur], e->d_dslot, e->d_dir, e->dstride, e->vbits, e->dir_threshold,
+      e->d_runs, nullptr, e->d_keys2, 0, e->d_ctrl, e->dir_cap);
   // 3. deletions and duplicate insertions
   GraphView g = view(e);
   k_delete_mark<<<grid_for(n, 256), 256, 0, e->stream>>>(b, g, e->d_ats[e->cur], e->d_vrec,
@@ -1129,6 +1218,10 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   }
   e->vbits = vb;
   e->vmask = vb >= 32 ? 0xFFFFFFFFu : ((1u << vb) - 1u);
+  // a directory holds nkeys + 1 offsets; lists shorter than an eighth of that
+  // (or than 8) keep binary search
+  e->dstride = (uint32_t)keys + 1;
+  e->dir_threshold = std::max<uint32_t>(8u, e->dstride / 8);
   e->nv_cap = (uint32_t)std::min<uint64_t>(want_cap, 1ull << vb);
   if (e->nv_cap < e->nv) e->nv_cap = e->nv;
   e->d2_words = (e->nv_cap + 31) / 32;

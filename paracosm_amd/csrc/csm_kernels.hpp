@@ -13,6 +13,13 @@
 //              at position p iff ins <= p <= del.  Untouched entries hold
 //              TS_BASE (always visible); only lists flagged DIRTY hold others.
 //   vlab[v]  = dense vertex label (NO_LABEL when the vertex does not exist).
+//   dslot[v] = key directory slot of v (NONE: none); dir[slot*dstride + b] is
+//              the first index, relative to the list start, whose key is >= b
+//              (b = 0..nkeys, the last entry is the list length), so a segment
+//              lookup costs two dependent loads instead of two binary searches
+//              over the whole list.  Lists of at least a threshold length get
+//              a slot; every physical rewrite of a list with a slot rebuilds
+//              its directory before the next search.
 //
 // Lists are never edited in place: a list that changes is rewritten into
 // fresh space taken from the pool with one atomic bump, and the pool is
@@ -34,6 +41,8 @@ constexpr int MAXK = 16;                    // query vertices supported by the s
 constexpr int MAXNB = MAXK - 1;
 constexpr int WAVE = 64;
 constexpr int SEARCH_WAVES = 4;             // waves per search workgroup
+constexpr uint32_t FCAP = 512;              // LDS leaf filter entries per wave
+constexpr uint32_t LEAF_SPLIT = 8192;       // a parent with more leaves becomes a task
 
 // error bits in Ctrl::err
 constexpr uint32_t ERR_DELETE_MISSING = 1u;
@@ -55,11 +64,13 @@ struct Ctrl {
   uint32_t abort;         // host-set: stop searching
   uint32_t nspill;        // tasks spilled by the current search round
   uint32_t nspill_chunks; // their chunks
-  uint32_t pad[4];
+  uint32_t dir_top;       // key directory slots handed out
+  uint32_t pad[3];
   unsigned long long live;        // live entries (sum of list lengths)
   unsigned long long stat_tasks;
   unsigned long long stat_cands;
   unsigned long long stat_bytes;
+  unsigned long long prof[32];    // per-phase wave cycles, calls and sizes (CSM_PHASE_PROF builds)
 };
 
 // One search plan per (query, matching order): positions 0..k-1.
@@ -105,6 +116,9 @@ struct GraphView {
   const uint32_t* __restrict__ vlab;
   const uint32_t* __restrict__ adj;
   const uint32_t* __restrict__ ats;
+  const uint32_t* __restrict__ dslot;
+  const uint32_t* __restrict__ dir;
+  uint32_t dstride;
   uint32_t vbits;
   uint32_t vmask;
   uint32_t nel;
@@ -143,6 +157,27 @@ __device__ __forceinline__ uint2 segment(const uint32_t* __restrict__ adj, uint3
   uint64_t hk = (uint64_t)(key + 1) << vbits;
   uint32_t hi = hk > 0xFFFFFFFFull ? end : lower_bound(adj, lo, end, (uint32_t)hk);
   return make_uint2(lo, hi);
+}
+
+// segment of key `key` in v's list [off, off+len), through v's key directory
+// when it has one
+__device__ __forceinline__ uint2 seg_at(const GraphView& g, uint32_t v, uint32_t off,
+                                        uint32_t len, uint32_t key) {
+  const uint32_t s = g.dslot[v];
+  if (s != NONE) {
+    const uint32_t* __restrict__ D = g.dir + (size_t)s * g.dstride + key;
+    return make_uint2(off + D[0], off + D[1]);
+  }
+  return segment(g.adj, off, len, key, g.vbits);
+}
+
+// the same from v's record; *rec_y receives the list's length word (DIRTY in
+// bit 31)
+__device__ __forceinline__ uint2 seg_of(const GraphView& g, uint32_t v, uint32_t key,
+                                        uint32_t* rec_y) {
+  const uint2 r = g.vrec[v];
+  *rec_y = r.y;
+  return seg_at(g, v, r.x, r.y & LEN_MASK, key);
 }
 
 // is packed value x visible in [lo, hi) at update position pos
@@ -480,6 +515,52 @@ __global__ __launch_bounds__(BS) void k_merge(uint2* __restrict__ vrec, uint32_t
   }
 }
 
+// Rebuild the key directories of rewritten lists.  One wave per vertex; the
+// vertices are list[0 .. *count) (DIR_LIST), the heads of the merge runs
+// (DIR_RUNS) or every vertex below n (DIR_ALL).  A vertex without a slot gets
+// one when its list has at least `threshold` entries and slots are left;
+// otherwise it keeps using binary search.
+constexpr int DIR_LIST = 0, DIR_RUNS = 1, DIR_ALL = 2;
+template <int SRC>
+__global__ void k_dir_build(const uint2* __restrict__ vrec, const uint32_t* __restrict__ adj,
+                            uint32_t* __restrict__ dslot, uint32_t* __restrict__ dir,
+                            uint32_t dstride, uint32_t vbits, uint32_t threshold,
+                            const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                            const uint64_t* __restrict__ keys, uint32_t n,
+                            Ctrl* __restrict__ ctrl, uint32_t dir_cap) {
+  const uint32_t lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
+  const uint32_t total = SRC == DIR_LIST ? *count : SRC == DIR_RUNS ? ctrl->nruns : n;
+  const uint32_t nk = dstride - 1;
+  for (uint32_t it = wid; it < total; it += nw) {
+    uint32_t v = SRC == DIR_LIST ? list[it]
+               : SRC == DIR_RUNS ? (uint32_t)(keys[list[it]] >> 32) : it;
+    const uint2 r = vrec[v];
+    const uint32_t off = r.x, len = r.y & LEN_MASK;
+    uint32_t s = uniform(dslot[v]);
+    if (s == NONE) {
+      if (len < threshold) continue;
+      if (lane == 0) {
+        s = atomicAdd(&ctrl->dir_top, 1u);
+        if (s >= dir_cap) s = NONE;
+        else dslot[v] = s;
+      }
+      s = uniform(__shfl(s, 0));
+      if (s == NONE) continue;
+    }
+    uint32_t* __restrict__ D = dir + (size_t)s * dstride;
+    // entry i opens the keys (key(i-1), key(i)]
+    for (uint32_t i = lane; i < len; i += WAVE) {
+      const uint32_t kc = adj[off + i] >> vbits;
+      const uint32_t kb = i ? (adj[off + i - 1] >> vbits) + 1u : 0u;
+      for (uint32_t b = kb; b <= kc && b <= nk; ++b) D[b] = i;
+    }
+    const uint32_t kt = len ? (adj[off + len - 1] >> vbits) + 1u : 0u;
+    for (uint32_t b = kt + lane; b <= nk; b += WAVE) D[b] = len;
+  }
+}
+
 // Locate the live version of (a -> b) for the update at position pos: the
 // entry for neighbour b (any edge label) with the largest ins < pos
 // (mode 0), or any version visible just before pos (mode 1).  Returns the
@@ -641,8 +722,8 @@ __global__ void k_seeds(SeedArgs a) {
       for (uint32_t jj = 0; jj < P->nb[2]; ++jj) {
         uint32_t w = P->bw[2][jj];
         uint32_t vx = (w & 0xFFu) == 0 ? m0 : m1;
-        uint2 r = a.g.vrec[vx];
-        uint2 sg = segment(a.g.adj, r.x, r.y & LEN_MASK, w >> 8, a.g.vbits);
+        uint32_t ry;
+        uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
         uint32_t sz = sg.y - sg.x;
         if (best == NONE || sz < best) { best = sz; blo = sg.x; bhi = sg.y; bj = jj; }
       }
@@ -698,7 +779,23 @@ struct WaveState {
   uint32_t olo[WAVE], ohi[WAVE], pref[WAVE + 1];
   uint32_t src[WAVE];
   uint32_t ljf, ljp, lfsize, lC;
+  // F: the last position's valid vertices with respect to all fixed
+  // neighbours (ascending); lF = |F|, or NONE when F was not built
+  uint32_t lF;
+  uint32_t F[FCAP];
 };
+
+// first index in the ascending LDS array a[0, n) holding a value >= x
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t n, uint32_t x) {
+  uint32_t lo = 0;
+  while (n > 0) {
+    uint32_t half = n >> 1;
+    bool less = a[lo + half] < x;
+    lo = less ? lo + half + 1 : lo;
+    n = less ? n - half - 1 : half;
+  }
+  return lo;
+}
 
 // One wavefront runs one depth-2 chunk of a seed at a time: an iterative DFS
 // over positions 2..k-2 whose per-position candidate buffers (one wave-width
@@ -717,6 +814,22 @@ struct WaveState {
 //   * otherwise each lane finds its parent's own segment, the smaller of the
 //     own and the fixed segment is the candidate source, and the wave walks
 //     the concatenation of all parents' sources 64 entries at a time.
+// Optional per-phase cycle accounting of the search (built with
+// -DCSM_PHASE_PROF): wave cycles spent in each phase, summed over waves.
+#ifdef CSM_PHASE_PROF
+#define PROF_BEGIN(id) const uint64_t prof_t0_##id = __builtin_readcyclecounter()
+#define PROF_END(id, i) \
+  do { prof_acc[i] += __builtin_readcyclecounter() - prof_t0_##id; ++prof_n[i]; } while (0)
+#else
+#define PROF_BEGIN(id) (void)0
+#define PROF_END(id, i) (void)0
+#endif
+#ifdef CSM_PHASE_PROF
+#define PROF_ADD(i, x) (prof_x[i] += (x))
+#else
+#define PROF_ADD(i, x) (void)0
+#endif
+
 #ifndef CSM_SEARCH_WAVES_PER_EU
 #define CSM_SEARCH_WAVES_PER_EU 5
 #endif
@@ -731,8 +844,15 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
   const uint32_t* __restrict__ ats = a.g.ats;
   const uint32_t vbits = a.g.vbits, vmask = a.g.vmask;
   unsigned long long n_tasks = 0, n_cands = 0, n_bytes = 0;
+#ifdef CSM_PHASE_PROF
+  unsigned long long prof_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long prof_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long prof_x[16] = {0};
+  const uint64_t prof_kt0 = __builtin_amdgcn_s_memrealtime();
+#endif
 
   while (true) {
+    PROF_BEGIN(fetch);
     uint32_t c = 0;
     if (lane == 0) c = atomicAdd(&a.ctrl->chunk_head, 1u);
     c = uniform(__shfl(c, 0));
@@ -756,6 +876,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     }
     unsigned long long cnt = 0, hsum = 0, flushed = 0;
     uint32_t ticks = 0;
+    uint32_t work = 0;          // budget units used by this task (one per 64 candidates)
+    bool may_spill = false;
     bool stop = false;
     const uint32_t L1 = k - 1, L2 = k - 2;   // last and second-to-last positions
 
@@ -822,6 +944,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     // fixed neighbours of the last position, once per entry of position k-2;
     // false if some fixed segment is empty (then no embedding can complete)
     auto prepare_last = [&]() -> bool {
+      PROF_BEGIN(pl);
       const uint32_t nb = uniform(P->nb[L1]);
       uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu, is_par = 0;
       if (lane < nb) {
@@ -830,12 +953,12 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
         if (bp == L2) {
           is_par = 1;
         } else {
-          uint2 r = a.g.vrec[S.m[bp]];
-          uint2 sg = segment(adj, r.x, r.y & LEN_MASK, w >> 8, vbits);
+          uint32_t ry;
+          uint2 sg = seg_of(a.g, S.m[bp], w >> 8, &ry);
           my_lo = sg.x;
           my_hi = sg.y;
           my_size = sg.y - sg.x;
-          my_dirty = VERSIONED ? (r.y >> 31) : 0u;
+          my_dirty = VERSIONED ? (ry >> 31) : 0u;
           S.plo[L1][lane] = my_lo;
           S.phi[L1][lane] = my_hi;
         }
@@ -852,10 +975,15 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       n_bytes += 4ull * wave_sum((unsigned long long)(my_size == 0xFFFFFFFFu ? 0u : my_size));
       wave_sync();
       if (jf != NONE && fsize == 0) return false;
-      if (jp == NONE && !HASH) {
-        // C = valid leaves of the shared segment
+      PROF_ADD(10, jp == NONE ? 1u : 0u);
+      PROF_ADD(11, jf == NONE ? 0u : fsize);
+      S.lF = NONE;
+      if (jf != NONE && (fsize <= FCAP || (jp == NONE && !HASH))) {
+        // walk the smallest fixed segment once: F (when it fits in LDS) and
+        // C = |F|, the number of valid leaves shared by every parent
         const uint32_t lo = uniform(__shfl(my_lo, jf)), hi = uniform(__shfl(my_hi, jf));
         const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
+        const bool keep = fsize <= FCAP;
         uint32_t C = 0;
         for (uint32_t b = lo; b < hi; b += WAVE) {
           uint32_t e = b + lane;
@@ -866,11 +994,18 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
             if (fd) ok = ts_visible(ats[e], pos);
           }
           ok = ok && last_fixed_ok(w, jf);
-          C += popc64(ballot(ok));
+          const uint64_t m = ballot(ok);
+          if (keep && ok) S.F[C + popc64(m & lt_mask)] = w;
+          C += popc64(m);
           n_cands += (hi - b) < WAVE ? (hi - b) : WAVE;
         }
         S.lC = C;
+        if (keep) S.lF = C;
+        work += (hi - lo) / WAVE;
+        wave_sync();
+        if (C == 0) return false;   // no leaf can complete under this prefix
       }
+      PROF_END(pl, 2);
       return true;
     };
 
@@ -881,12 +1016,12 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       if (lane < nb) {
         uint32_t w = P->bw[lv][lane];
         uint32_t vx = S.m[w & 0xFFu];
-        uint2 r = a.g.vrec[vx];
-        uint2 sg = segment(adj, r.x, r.y & LEN_MASK, w >> 8, vbits);
+        uint32_t ry;
+        uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
         my_lo = sg.x;
         my_hi = sg.y;
         my_size = sg.y - sg.x;
-        my_dirty = VERSIONED ? (r.y >> 31) : 0u;
+        my_dirty = VERSIONED ? (ry >> 31) : 0u;
         S.plo[lv][lane] = my_lo;
         S.phi[lv][lane] = my_hi;
       }
@@ -915,45 +1050,116 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       return true;
     };
 
+    // write spill task ti: positions 0..depth-2 from m[], position depth-1 =
+    // extra, candidate range [lo, hi) of backward neighbour jm in chunks from ci
+    auto emit_task = [&](uint32_t& ti, uint32_t& ci, uint32_t depth, uint32_t jm, uint32_t lo,
+                         uint32_t hi, uint2 ch, uint32_t extra) {
+      const uint32_t nch = ch.x;
+      Task* o = a.spill + ti;
+      uint32_t word = 0;
+      if (lane < TASK_WORDS) {
+        if (lane == 0) word = t;
+        else if (lane == 1) word = qo;
+        else if (lane == 2) word = depth;
+        else if (lane == 3) word = jm;
+        else if (lane == 4) word = lo;
+        else if (lane == 5) word = hi;
+        else if (lane == 6) word = ci;
+        else if (lane == 7) word = ch.y;
+        else {
+          uint32_t p = lane - 8;
+          word = p + 1 < depth ? S.m[p] : (p + 1 == depth ? extra : 0u);
+        }
+        reinterpret_cast<uint32_t*>(o)[lane] = word;
+      }
+      for (uint32_t j = lane; j < nch; j += WAVE) a.spill_owner[ci + j] = ti;
+      ++ti;
+      ci += nch;
+    };
+
+    // the leaves of parent v (candidate range [lo, hi) of the last position's
+    // backward neighbour jm) as a task of the next round
+    auto spill_leaves = [&](uint32_t v, uint32_t jm, uint32_t lo, uint32_t hi) -> bool {
+      const uint2 ch = chunking(hi - lo, a.chunk);
+      uint32_t tb = 0, cb = 0;
+      if (lane == 0) {
+        tb = reserve(&a.ctrl->nspill, 1u, a.spill_cap);
+        cb = tb == NONE ? NONE : reserve(&a.ctrl->nspill_chunks, ch.x, a.spill_chunk_cap);
+      }
+      tb = uniform(__shfl(tb, 0));
+      cb = uniform(__shfl(cb, 0));
+      if (tb == NONE || cb == NONE) return false;  // (a task slot may leak; never read)
+      emit_task(tb, cb, L1, jm, lo, hi, ch, v);
+      return true;
+    };
+
     // all leaves under the bn parents now in cand[k-2]
     auto finish_parents = [&](uint32_t bn) {
+      PROF_BEGIN(fp);
       const uint32_t jp = S.ljp, jf = S.ljf;
+      const uint32_t nF = S.lF;          // F in LDS (NONE: not built)
+      const bool useF = nF != NONE;
       uint32_t v = lane < bn ? S.cand[L2][lane] : 0u;
       if (jp == NONE && !HASH) {
         // C - [parent is itself a valid leaf]
         bool self = false;
         if (lane < bn && !a.homo) {
-          const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
-          uint32_t bw = P->bw[L1][jf];
-          uint32_t x = ((bw >> 8) << vbits) | v;
-          self = contains(adj, ats, S.plo[L1][jf], S.phi[L1][jf], x, fd, pos) &&
-                 last_fixed_ok(v, jf);
+          if (useF) {
+            const uint32_t i = lds_lower_bound(S.F, nF, v);
+            self = i < nF && S.F[i] == v;
+          } else {
+            const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
+            uint32_t bw = P->bw[L1][jf];
+            uint32_t x = ((bw >> 8) << vbits) | v;
+            self = contains(adj, ats, S.plo[L1][jf], S.phi[L1][jf], x, fd, pos) &&
+                   last_fixed_ok(v, jf);
+          }
         }
         cnt += (unsigned long long)bn * S.lC - popc64(ballot(self));
         n_bytes += 4ull * bn * S.lfsize;
+        PROF_END(fp, 5);
         return;
       }
-      // per-parent source: own segment S(v, key_jp) or the fixed minimum
+      // per-parent source: own segment S(v, key_jp), or F / the smallest
+      // fixed segment
+      const uint32_t fsz = useF ? nF : S.lfsize;
       uint32_t olo = 0, ohi = 0, odirty = 0, size = 0, src = 0;
       if (lane < bn) {
         if (jp != NONE) {
-          uint2 r = a.g.vrec[v];
-          uint2 sg = segment(adj, r.x, r.y & LEN_MASK, P->bw[L1][jp] >> 8, vbits);
+          uint32_t ry;
+          uint2 sg = seg_of(a.g, v, P->bw[L1][jp] >> 8, &ry);
           olo = sg.x;
           ohi = sg.y;
-          odirty = VERSIONED ? (r.y >> 31) : 0u;
+          odirty = VERSIONED ? (ry >> 31) : 0u;
           uint32_t osz = sg.y - sg.x;
           if (osz == 0) size = 0;
-          else if (jf != NONE && S.lfsize < osz) { size = S.lfsize; src = 0; }
+          else if (jf != NONE && fsz < osz) { size = fsz; src = 0; }
           else { size = osz; src = 1; }
           n_bytes += 4ull * osz;
         } else {
-          size = S.lfsize;
+          size = fsz;
           src = 0;
         }
         S.olo[lane] = olo;
         S.ohi[lane] = ohi;
         S.src[lane] = src | (odirty << 1);
+      }
+      // parents with very many leaves go to the next round as tasks of their
+      // own, so that no chunk runs unboundedly long
+      if (may_spill) {
+        uint64_t big = ballot(lane < bn && size > LEAF_SPLIT);
+        while (big) {
+          const uint32_t p = (uint32_t)__builtin_ctzll(big);
+          big &= big - 1;
+          const bool own = (uniform(__shfl(src, p)) & 1u) != 0;
+          const uint32_t lo = own ? uniform(__shfl(olo, p)) : S.plo[L1][jf];
+          const uint32_t hi = own ? uniform(__shfl(ohi, p)) : S.phi[L1][jf];
+          if (!spill_leaves(S.cand[L2][p], own ? jp : jf, lo, hi)) {
+            may_spill = false;
+            break;
+          }
+          if (lane == p) size = 0;
+        }
       }
       // exclusive prefix of sizes over the parents
       uint32_t incl = size;
@@ -963,6 +1169,15 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
         if ((int)lane >= o) incl += y;
       }
       const uint32_t total = uniform(__shfl(incl, WAVE - 1));
+      work += total / WAVE;
+      PROF_ADD(0, total);
+      PROF_ADD(1, bn);
+      PROF_ADD(2, popc64(ballot(lane < bn && src == 1)));
+      PROF_ADD(3, popc64(ballot(lane < bn && src == 0 && size > 0)));
+      PROF_ADD(4, jf == NONE ? 1u : 0u);
+      PROF_ADD(5, uniform(P->nb[L1]));
+      PROF_ADD(6, uniform(popc64(ballot(lane < bn)) ? (uint32_t)wave_sum((unsigned long long)(lane < bn ? (ohi - olo) : 0u)) : 0u));
+      PROF_ADD(7, S.lfsize);
       S.pref[lane] = incl - size;
       if (lane == 0) S.pref[WAVE] = total;
       wave_sync();
@@ -982,15 +1197,24 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
           p = lo;
           const uint32_t sp = S.src[p];
           const uint32_t off = g - S.pref[p];
-          uint32_t e;
-          bool dirty;
-          if (sp & 1u) { e = S.olo[p] + off; dirty = (sp >> 1) & 1u; }
-          else { e = fsum_lo + off; dirty = VERSIONED && ((S.pdirty[L1] >> jf) & 1u); }
-          w = adj[e] & vmask;
-          if (VERSIONED && dirty) ok = ts_visible(ats[e], pos);
+          if ((sp & 1u) || !useF) {
+            uint32_t e;
+            bool dirty;
+            if (sp & 1u) { e = S.olo[p] + off; dirty = (sp >> 1) & 1u; }
+            else { e = fsum_lo + off; dirty = VERSIONED && ((S.pdirty[L1] >> jf) & 1u); }
+            w = adj[e] & vmask;
+            if (VERSIONED && dirty) ok = ts_visible(ats[e], pos);
+          } else {
+            w = S.F[off];   // already valid for the fixed neighbours
+          }
           const uint32_t vp = S.cand[L2][p];
           if (ok && !a.homo) ok = w != vp;
-          ok = ok && last_fixed_ok(w, (sp & 1u) ? NONE : jf);
+          if (!useF) {
+            ok = ok && last_fixed_ok(w, (sp & 1u) ? NONE : jf);
+          } else if (ok && (sp & 1u)) {
+            const uint32_t i = lds_lower_bound(S.F, nF, w);
+            ok = i < nF && S.F[i] == w;
+          }
           if (ok && jp != NONE && !(sp & 1u)) {
             uint32_t x = ((P->bw[L1][jp] >> 8) << vbits) | w;
             ok = contains(adj, ats, S.olo[p], S.ohi[p], x, VERSIONED && ((sp >> 1) & 1u), pos);
@@ -1012,6 +1236,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
         }
       }
       if (jp == NONE) n_bytes += 4ull * bn * S.lfsize;
+      PROF_END(fp, 6);
     };
 
     // spill the rest of this task's DFS as new tasks (one per remaining range
@@ -1035,36 +1260,13 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       cb = uniform(__shfl(cb, 0));
       if (tb == NONE || cb == NONE) return false;  // (a task slot may leak; never read)
       uint32_t ti = tb, ci = cb;
-      auto emit = [&](uint32_t depth, uint32_t jm, uint32_t lo, uint32_t hi, uint2 ch,
-                      uint32_t extra) {
-        const uint32_t nch = ch.x;
-        Task* o = a.spill + ti;
-        uint32_t word = 0;
-        if (lane < TASK_WORDS) {
-          if (lane == 0) word = t;
-          else if (lane == 1) word = qo;
-          else if (lane == 2) word = depth;
-          else if (lane == 3) word = jm;
-          else if (lane == 4) word = lo;
-          else if (lane == 5) word = hi;
-          else if (lane == 6) word = ci;
-          else if (lane == 7) word = ch.y;
-          else {
-            uint32_t p = lane - 8;
-            word = p + 1 < depth ? S.m[p] : (p + 1 == depth ? extra : 0u);
-          }
-          reinterpret_cast<uint32_t*>(o)[lane] = word;
-        }
-        for (uint32_t j = lane; j < nch; j += WAVE) a.spill_owner[ci + j] = ti;
-        ++ti;
-        ci += nch;
-      };
       for (uint32_t L = d0; L <= lv_now; ++L) {
         uint32_t cu = S.cur[L], en = S.end[L];
-        if (cu < en) emit(L, S.jmin[L], cu, en, chunking(en - cu, a.chunk), L > 0 ? S.m[L - 1] : 0u);
+        if (cu < en)
+          emit_task(ti, ci, L, S.jmin[L], cu, en, chunking(en - cu, a.chunk), L > 0 ? S.m[L - 1] : 0u);
         if (L < L2)
           for (uint32_t i = S.bi[L]; i < S.bn[L]; ++i)
-            emit(L + 1, NONE, 0u, NONE, make_uint2(1u, NONE), S.cand[L][i]);
+            emit_task(ti, ci, L + 1, NONE, 0u, NONE, make_uint2(1u, NONE), S.cand[L][i]);
       }
       return true;
     };
@@ -1080,12 +1282,18 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     }
     const uint32_t tj = uniform(TK->jmin);
     int lv = (int)d0;
+    work = 0;
+    may_spill = a.spill != nullptr;
+    PROF_END(fetch, 0);
+    PROF_BEGIN(e0);
     bool live = enter(d0, rlo, rhi, tj == NONE ? -1 : (int)tj);
-    uint32_t work = 0;
-    bool may_spill = a.spill != nullptr;
+    PROF_END(e0, 1);
     while (live && !stop) {
       if (may_spill && ++work > a.budget_iters && (uint32_t)lv < L1) {
-        if (spill((uint32_t)lv)) break;
+        PROF_BEGIN(sp);
+        bool ok_sp = spill((uint32_t)lv);
+        PROF_END(sp, 7);
+        if (ok_sp) break;
         may_spill = false;
       }
       const uint32_t L = (uint32_t)lv;
@@ -1116,6 +1324,9 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       uint32_t cur = S.cur[L], end = S.end[L];
       if (L == L2) {
         if (cur < end) {
+          PROF_BEGIN(t2);
+          PROF_ADD(8, uniform(P->nb[L]));
+          PROF_ADD(9, (end - cur) < WAVE ? (end - cur) : WAVE);
           uint32_t v;
           bool ok = test(L, cur + lane, end, v);
           uint64_t m = ballot(ok);
@@ -1123,6 +1334,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
           if (ok) S.cand[L][popc64(m & lt_mask)] = v;
           S.cur[L] = cur + WAVE;
           wave_sync();
+          PROF_END(t2, 4);
           if (m) finish_parents(popc64(m));
           continue;
         }
@@ -1137,10 +1349,14 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
         S.m[L] = v;
         if (HASH) S.mq[P->qv[L]] = v;
         ++lv;
-        if (!enter(L + 1, 0u, 0xFFFFFFFFu, -1)) --lv;
+        PROF_BEGIN(en);
+        bool entered = enter(L + 1, 0u, 0xFFFFFFFFu, -1);
+        PROF_END(en, 1);
+        if (!entered) --lv;
         continue;
       }
       if (cur < end) {
+        PROF_BEGIN(t1);
         uint32_t v;
         bool ok = test(L, cur + lane, end, v);
         uint64_t m = ballot(ok);
@@ -1150,6 +1366,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
         S.bi[L] = 0;
         S.cur[L] = cur + WAVE;
         wave_sync();
+        PROF_END(t1, 3);
         continue;
       }
       --lv;
@@ -1162,11 +1379,26 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     atomicAdd(&a.ctrl->stat_tasks, n_tasks);
     atomicAdd(&a.ctrl->stat_cands, n_cands);
     atomicAdd(&a.ctrl->stat_bytes, n_bytes + 4ull * n_cands);
+#ifdef CSM_PHASE_PROF
+    for (int i = 0; i < 8; ++i) {
+      atomicAdd(&a.ctrl->prof[i], prof_acc[i]);
+      atomicAdd(&a.ctrl->prof[8 + i], prof_n[i]);
+    }
+    const uint64_t prof_kt1 = __builtin_amdgcn_s_memrealtime();
+    prof_x[12] += prof_kt1 - prof_kt0;
+    prof_x[13] += 1;
+    for (int i = 0; i < 14; ++i) atomicAdd(&a.ctrl->prof[16 + i], prof_x[i]);
+    atomicMin(&a.ctrl->prof[30], (unsigned long long)prof_kt0);
+    atomicMax(&a.ctrl->prof[31], (unsigned long long)prof_kt1);
+#endif
   }
 }
 
 // ------------------------------------------------------------ batch control
 __global__ void k_batch_begin(Ctrl* c) {
+#ifdef CSM_PHASE_PROF
+  if (threadIdx.x == 0) { c->prof[30] = ~0ull; c->prof[31] = 0; }
+#endif
   if (threadIdx.x == 0) {
     c->nseeds = 0;
     c->nchunks = 0;
@@ -1182,6 +1414,9 @@ __global__ void k_batch_begin(Ctrl* c) {
 
 // the spilled tasks of the last round become the input of the next
 __global__ void k_next_round(Ctrl* c) {
+#ifdef CSM_PHASE_PROF
+  if (threadIdx.x == 0) { c->prof[30] = ~0ull; c->prof[31] = 0; }
+#endif
   if (threadIdx.x == 0) {
     c->nchunks = c->nspill_chunks;
     c->chunk_head = 0;
@@ -1211,9 +1446,9 @@ __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, 
        v += gridDim.x * blockDim.x) {
     if (g.vlab[v] != la) continue;
     if (symbi && !d2_bit(d2, P->d2row[0], d2_words, v)) continue;
-    uint2 r = g.vrec[v];
-    uint2 sg = segment(g.adj, r.x, r.y & LEN_MASK, key01, g.vbits);
-    bool dirty = (r.y & DIRTY) != 0;
+    uint32_t ry;
+    uint2 sg = seg_of(g, v, key01, &ry);
+    bool dirty = (ry & DIRTY) != 0;
     for (uint32_t i = sg.x; i < sg.y; ++i) {
       if (dirty && !ts_visible(g.ats[i], 0xFFFFu)) continue;
       uint32_t w = g.adj[i] & g.vmask;
@@ -1223,8 +1458,8 @@ __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, 
       for (uint32_t jj = 0; jj < P->nb[2]; ++jj) {
         uint32_t bw = P->bw[2][jj];
         uint32_t vx = (bw & 0xFFu) == 0 ? v : w;
-        uint2 rr = g.vrec[vx];
-        uint2 s2 = segment(g.adj, rr.x, rr.y & LEN_MASK, bw >> 8, g.vbits);
+        uint32_t ry2;
+        uint2 s2 = seg_of(g, vx, bw >> 8, &ry2);
         uint32_t sz = s2.y - s2.x;
         if (best == NONE || sz < best) { best = sz; blo = s2.x; bhi = s2.y; bj = jj; }
       }

@@ -110,7 +110,7 @@ __global__ void k_sx_build(GraphView g, SxView x, uint32_t r, int mode) {
     if (mode == 0) {
       uint32_t np = 0;
       for (uint32_t j = 0; j < R->np; ++j) {
-        uint2 sg = segment(g.adj, off, len, R->par_key[j], g.vbits);
+        uint2 sg = seg_at(g, v, off, len, R->par_key[j]);
         uint32_t c = 0, pr = R->par_row[j];
         for (uint32_t i = sg.x; i < sg.y; ++i)
           if (present(g, i, sg.x, dirty, true) && bit_get(x.d1, x.words, pr, g.adj[i] & g.vmask)) ++c;
@@ -122,7 +122,7 @@ __global__ void k_sx_build(GraphView g, SxView x, uint32_t r, int mode) {
     } else if (mode == 1) {
       uint32_t nc = 0;
       for (uint32_t j = 0; j < R->nc; ++j) {
-        uint2 sg = segment(g.adj, off, len, R->ch_key[j], g.vbits);
+        uint2 sg = seg_at(g, v, off, len, R->ch_key[j]);
         uint32_t c = 0, cr = R->ch_row[j];
         for (uint32_t i = sg.x; i < sg.y; ++i)
           if (present(g, i, sg.x, dirty, true) && bit_get(x.d2, x.words, cr, g.adj[i] & g.vmask)) ++c;
@@ -133,7 +133,7 @@ __global__ void k_sx_build(GraphView g, SxView x, uint32_t r, int mode) {
       if (nc == R->nc && bit_get(x.d1, x.words, r, v)) bit_set(x.d2, x.words, r, v);
     } else {
       for (uint32_t j = 0; j < R->np; ++j) {
-        uint2 sg = segment(g.adj, off, len, R->par_key[j], g.vbits);
+        uint2 sg = seg_at(g, v, off, len, R->par_key[j]);
         uint32_t c = 0, pr = R->par_row[j];
         for (uint32_t i = sg.x; i < sg.y; ++i)
           if (present(g, i, sg.x, dirty, true) && bit_get(x.d2, x.words, pr, g.adj[i] & g.vmask)) ++c;
@@ -310,7 +310,7 @@ __global__ void k_sx_propagate(GraphView g, SxView x, SxList front, bool ins, bo
     const bool dirty = (rec.y & DIRTY) != 0;
     if (stage == 0) {
       for (uint32_t j = 0; j < R->nc; ++j) {
-        uint2 sg = segment(g.adj, off, len, R->ch_key[j], g.vbits);
+        uint2 sg = seg_at(g, v, off, len, R->ch_key[j]);
         const uint32_t cr = R->ch_row[j], nprow = R->ch_n1[j];
         const uint32_t need = x.rows[cr].np;
         for (uint32_t i = sg.x + lane; i < sg.y; i += WAVE) {
@@ -328,7 +328,7 @@ __global__ void k_sx_propagate(GraphView g, SxView x, SxList front, bool ins, bo
       }
     } else {
       for (uint32_t j = 0; j < R->np; ++j) {
-        uint2 sg = segment(g.adj, off, len, R->par_key[j], g.vbits);
+        uint2 sg = seg_at(g, v, off, len, R->par_key[j]);
         const uint32_t pr = R->par_row[j], n2row = R->par_n2dn[j];
         const uint32_t need = x.rows[pr].nc;
         for (uint32_t i = sg.x + lane; i < sg.y; i += WAVE) {
@@ -347,7 +347,7 @@ __global__ void k_sx_propagate(GraphView g, SxView x, SxList front, bool ins, bo
         }
       }
       for (uint32_t j = 0; j < R->nc; ++j) {
-        uint2 sg = segment(g.adj, off, len, R->ch_key[j], g.vbits);
+        uint2 sg = seg_at(g, v, off, len, R->ch_key[j]);
         const uint32_t n2row = R->ch_n2up[j];
         for (uint32_t i = sg.x + lane; i < sg.y; i += WAVE) {
           if (!present(g, i, sg.x, dirty, end_view)) continue;
