@@ -655,6 +655,7 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
   sa.plans = e->d_plans;
   sa.tasks = e->d_seeds;
   sa.chunk_owner = e->d_chunk_owner;
+  sa.chunk_cap = e->chunk_cap;
   sa.ctrl = e->d_ctrl;
   sa.counts = counts;
   sa.hashes = hashes;
@@ -714,6 +715,7 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
     k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl);
     sa.tasks = e->d_spill[out];
     sa.chunk_owner = e->d_spill_owner[out];
+    sa.chunk_cap = e->spill_chunk_cap;
     out = 1 - out;
     if (round > 100000) return fail(e, CSM_E_DEVICE, "search did not converge");
   }

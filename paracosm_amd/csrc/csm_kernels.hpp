@@ -682,6 +682,7 @@ struct SeedArgs {
 // parallel_graphflow.cpp:1990-2016).
 __global__ void k_seeds(SeedArgs a) {
   const uint32_t lane = lane_id();
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
   for (uint32_t i = wid; i < a.b.n; i += nw) {
@@ -697,46 +698,83 @@ __global__ void k_seeds(SeedArgs a) {
         e = a.tab_off[bucket + 1];
       }
     }
+    s = uniform(s);
+    e = uniform(e);
     if (lane == 0 && a.safe) a.safe[i] = (s == e);
-    for (uint32_t j = s + lane; j < e; j += WAVE) {
-      uint32_t ent = a.tab_ent[j];
-      uint32_t qo = ent >> 1;
-      const PlanRec* P = a.plans + qo;
-      uint32_t m0 = (ent & 1) ? y : x, m1 = (ent & 1) ? x : y;
-      if (P->symbi && a.d2 &&
-          !(d2_bit(a.d2, P->d2row[0], a.d2_words, m0) && d2_bit(a.d2, P->d2row[1], a.d2_words, m1)))
-        continue;
-      if (P->k == 2) {
-        atomicAdd(&a.counts[(size_t)i * a.nq + P->q], 1ull);
-        if (a.want_hash) {
-          uint32_t mq[2];
-          mq[P->qv[0]] = m0;
-          mq[P->qv[1]] = m1;
-          uint64_t h = mix64(mix64((uint64_t)mq[0] + 1) + (uint64_t)mq[1] + 1);
-          atomicAdd(&a.hashes[(size_t)i * a.nq + P->q], (unsigned long long)h);
+    for (uint32_t j0 = s; j0 < e; j0 += WAVE) {
+      const uint32_t j = j0 + lane;
+      bool have = false;
+      uint32_t qo = 0, m0 = 0, m1 = 0, blo = 0, bhi = 0, bj = 0;
+      uint2 ch = make_uint2(0u, 0u);
+      if (j < e) {
+        uint32_t ent = a.tab_ent[j];
+        qo = ent >> 1;
+        const PlanRec* P = a.plans + qo;
+        m0 = (ent & 1) ? y : x;
+        m1 = (ent & 1) ? x : y;
+        bool ok = !(P->symbi && a.d2 &&
+                    !(d2_bit(a.d2, P->d2row[0], a.d2_words, m0) &&
+                      d2_bit(a.d2, P->d2row[1], a.d2_words, m1)));
+        if (ok && P->k == 2) {
+          atomicAdd(&a.counts[(size_t)i * a.nq + P->q], 1ull);
+          if (a.want_hash) {
+            uint32_t mq[2];
+            mq[P->qv[0]] = m0;
+            mq[P->qv[1]] = m1;
+            uint64_t h = mix64(mix64((uint64_t)mq[0] + 1) + (uint64_t)mq[1] + 1);
+            atomicAdd(&a.hashes[(size_t)i * a.nq + P->q], (unsigned long long)h);
+          }
+          ok = false;
         }
-        continue;
+        if (ok) {
+          // depth-2 segment: smallest label-filtered partner segment
+          uint32_t best = NONE;
+          for (uint32_t jj = 0; jj < P->nb[2]; ++jj) {
+            uint32_t w = P->bw[2][jj];
+            uint32_t vx = (w & 0xFFu) == 0 ? m0 : m1;
+            uint32_t ry;
+            uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
+            uint32_t sz = sg.y - sg.x;
+            if (best == NONE || sz < best) { best = sz; blo = sg.x; bhi = sg.y; bj = jj; }
+          }
+          if (best != 0 && best != NONE) {
+            have = true;
+            ch = chunking(best, a.chunk);
+          }
+        }
       }
-      // depth-2 segment: smallest label-filtered partner segment
-      uint32_t best = NONE, blo = 0, bhi = 0, bj = 0;
-      for (uint32_t jj = 0; jj < P->nb[2]; ++jj) {
-        uint32_t w = P->bw[2][jj];
-        uint32_t vx = (w & 0xFFu) == 0 ? m0 : m1;
-        uint32_t ry;
-        uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
-        uint32_t sz = sg.y - sg.x;
-        if (best == NONE || sz < best) { best = sz; blo = sg.x; bhi = sg.y; bj = jj; }
+      // one reservation per wave: task slots and chunk ids
+      const uint64_t hm = ballot(have);
+      if (!hm) continue;
+      const uint32_t nt = popc64(hm), ti = popc64(hm & lt_mask);
+      uint32_t incl = have ? ch.x : 0u;
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o);
+        if ((int)lane >= o) incl += t;
       }
-      if (best == 0 || best == NONE) continue;
-      uint2 ch = chunking(best, a.chunk);
-      uint32_t sidx = reserve(&a.ctrl->nseeds, 1u, a.seed_cap);
-      if (sidx == NONE) { atomicOr(&a.ctrl->err, ERR_SEED_OVERFLOW); continue; }
-      uint32_t base = reserve(&a.ctrl->nchunks, ch.x, a.chunk_cap);
-      if (base == NONE) { atomicOr(&a.ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
-      Task* tk = a.seeds + sidx;
-      tk->t = i; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
-      tk->base = base; tk->csize = ch.y; tk->m[0] = m0; tk->m[1] = m1;
-      for (uint32_t c = 0; c < ch.x; ++c) a.chunk_owner[base + c] = sidx;
+      const uint32_t nc = uniform(__shfl(incl, WAVE - 1));
+      uint32_t tb = 0, cb = 0;
+      if (lane == 0) {
+        tb = atomicAdd(&a.ctrl->nseeds, nt);
+        cb = atomicAdd(&a.ctrl->nchunks, nc);
+        if ((uint64_t)tb + nt > a.seed_cap) atomicOr(&a.ctrl->err, ERR_SEED_OVERFLOW);
+        if ((uint64_t)cb + nc > a.chunk_cap) atomicOr(&a.ctrl->err, ERR_CHUNK_OVERFLOW);
+      }
+      tb = uniform(__shfl(tb, 0));
+      cb = uniform(__shfl(cb, 0));
+      if (have) {
+        const uint32_t sidx = tb + ti;
+        const bool fits = (uint64_t)sidx < a.seed_cap;
+        if (fits) {
+          Task* tk = a.seeds + sidx;
+          tk->t = i; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
+          tk->base = cb + incl - ch.x; tk->csize = ch.y; tk->m[0] = m0; tk->m[1] = m1;
+        }
+        // every reserved chunk id below the cap gets an owner (NONE: skipped)
+        for (uint32_t c = 0, id = cb + incl - ch.x; c < ch.x; ++c, ++id)
+          if ((uint64_t)id < a.chunk_cap) a.chunk_owner[id] = fits ? sidx : NONE;
+      }
     }
   }
 }
@@ -746,7 +784,8 @@ struct SearchArgs {
   GraphView g;
   const PlanRec* __restrict__ plans;
   const Task* __restrict__ tasks;
-  const uint32_t* __restrict__ chunk_owner;
+  const uint32_t* __restrict__ chunk_owner;   // NONE: reserved id without a task
+  uint32_t chunk_cap;                         // ids beyond it were never written
   Task* spill;                // next round's tasks (nullable: never spill)
   uint32_t* spill_owner;      // next round's chunk owners
   uint32_t spill_cap, spill_chunk_cap;
@@ -783,6 +822,10 @@ struct WaveState {
   // neighbours (ascending); lF = |F|, or NONE when F was not built
   uint32_t lF;
   uint32_t F[FCAP];
+  // the current plan's per-position fields (plan index pqo)
+  uint32_t pqo;
+  uint32_t pnb[MK], pd2[MK], pqv[MK];
+  uint32_t pbw[MK][MK - 1];
 };
 
 // first index in the ascending LDS array a[0, n) holding a value >= x
@@ -840,6 +883,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
   const uint32_t lane = lane_id();
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   WaveState<MK>& S = S_all[threadIdx.x / WAVE];
+  if (lane_id() == 0) S.pqo = NONE;
+  wave_sync();
   const uint32_t* __restrict__ adj = a.g.adj;
   const uint32_t* __restrict__ ats = a.g.ats;
   const uint32_t vbits = a.g.vbits, vmask = a.g.vmask;
@@ -857,9 +902,11 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     if (lane == 0) c = atomicAdd(&a.ctrl->chunk_head, 1u);
     c = uniform(__shfl(c, 0));
     uint32_t nch = uniform(__atomic_load_n(&a.ctrl->nchunks, __ATOMIC_RELAXED));
+    nch = nch < a.chunk_cap ? nch : a.chunk_cap;
     if (c >= nch) break;
     if (__atomic_load_n(&a.ctrl->abort, __ATOMIC_RELAXED)) break;
     const uint32_t sidx = uniform(a.chunk_owner[c]);
+    if (sidx == NONE) continue;
     const Task* __restrict__ TK = a.tasks + sidx;
     const uint32_t t = uniform(TK->t), qo = uniform(TK->qo);
     const uint32_t d0 = uniform(TK->depth);
@@ -869,11 +916,23 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     const uint32_t q = uniform(P->q);
     if (a.qabort && __atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) continue;
     ++n_tasks;
-    for (uint32_t i = 0; i < d0; ++i) {
-      uint32_t v = TK->m[i];
-      S.m[i] = v;
-      if (HASH) S.mq[P->qv[i]] = v;
+    if (qo != S.pqo) {
+      if (lane < MK) {
+        S.pnb[lane] = P->nb[lane];
+        S.pd2[lane] = P->d2row[lane];
+        S.pqv[lane] = P->qv[lane];
+      }
+      for (uint32_t i = lane; i < MK * (MK - 1); i += WAVE)
+        S.pbw[i / (MK - 1)][i % (MK - 1)] = P->bw[i / (MK - 1)][i % (MK - 1)];
+      if (lane == 0) S.pqo = qo;
+      wave_sync();
     }
+    if (lane < d0) {
+      const uint32_t v = TK->m[lane];
+      S.m[lane] = v;
+      if (HASH) S.mq[S.pqv[lane]] = v;
+    }
+    wave_sync();
     unsigned long long cnt = 0, hsum = 0, flushed = 0;
     uint32_t ticks = 0;
     uint32_t work = 0;          // budget units used by this task (one per 64 candidates)
@@ -908,13 +967,13 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       if (ok && !a.homo) {
         for (uint32_t i = 0; i < lv; ++i) ok = ok && (S.m[i] != v);
       }
-      if (SYMBI && ok) ok = d2_bit(a.d2, P->d2row[lv], a.d2_words, v);
-      const uint32_t nb = uniform(P->nb[lv]);
+      if (SYMBI && ok) ok = d2_bit(a.d2, S.pd2[lv], a.d2_words, v);
+      const uint32_t nb = uniform(S.pnb[lv]);
       const uint32_t jm = S.jmin[lv];
       for (uint32_t j = 0; j < nb; ++j) {
         if (j == jm) continue;
         if (!ok) continue;
-        uint32_t w = P->bw[lv][j];
+        uint32_t w = S.pbw[lv][j];
         uint32_t x = ((w >> 8) << vbits) | v;
         bool vd = VERSIONED && ((S.pdirty[lv] >> j) & 1u);
         ok = contains(adj, ats, S.plo[lv][j], S.phi[lv][j], x, vd, pos);
@@ -929,11 +988,11 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       bool ok = true;
       if (!a.homo)
         for (uint32_t i = 0; i < L2; ++i) ok = ok && (S.m[i] != w);
-      if (SYMBI && ok) ok = d2_bit(a.d2, P->d2row[L1], a.d2_words, w);
-      const uint32_t nb = uniform(P->nb[L1]);
+      if (SYMBI && ok) ok = d2_bit(a.d2, S.pd2[L1], a.d2_words, w);
+      const uint32_t nb = uniform(S.pnb[L1]);
       for (uint32_t j = 0; j < nb; ++j) {
         if (j == skip || j == S.ljp || !ok) continue;
-        uint32_t bw = P->bw[L1][j];
+        uint32_t bw = S.pbw[L1][j];
         uint32_t x = ((bw >> 8) << vbits) | w;
         bool vd = VERSIONED && ((S.pdirty[L1] >> j) & 1u);
         ok = contains(adj, ats, S.plo[L1][j], S.phi[L1][j], x, vd, pos);
@@ -945,10 +1004,10 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     // false if some fixed segment is empty (then no embedding can complete)
     auto prepare_last = [&]() -> bool {
       PROF_BEGIN(pl);
-      const uint32_t nb = uniform(P->nb[L1]);
+      const uint32_t nb = uniform(S.pnb[L1]);
       uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu, is_par = 0;
       if (lane < nb) {
-        uint32_t w = P->bw[L1][lane];
+        uint32_t w = S.pbw[L1][lane];
         uint32_t bp = w & 0xFFu;
         if (bp == L2) {
           is_par = 1;
@@ -1011,10 +1070,10 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
 
     // enter position lv: partner segments, u_min, candidate range
     auto enter = [&](uint32_t lv, uint32_t rlo, uint32_t rhi, int forced_j) -> bool {
-      const uint32_t nb = uniform(P->nb[lv]);
+      const uint32_t nb = uniform(S.pnb[lv]);
       uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu;
       if (lane < nb) {
-        uint32_t w = P->bw[lv][lane];
+        uint32_t w = S.pbw[lv][lane];
         uint32_t vx = S.m[w & 0xFFu];
         uint32_t ry;
         uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
@@ -1050,6 +1109,24 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       return true;
     };
 
+    // reserve n_t spill tasks and n_c chunk ids with one atomic each; when
+    // they do not fit, the chunk ids below the cap are marked ownerless and
+    // the caller keeps the work
+    auto reserve_spill = [&](uint32_t n_t, uint32_t n_c, uint32_t& tb, uint32_t& cb) -> bool {
+      tb = 0;
+      cb = 0;
+      if (lane == 0) {
+        tb = atomicAdd(&a.ctrl->nspill, n_t);
+        cb = atomicAdd(&a.ctrl->nspill_chunks, n_c);
+      }
+      tb = uniform(__shfl(tb, 0));
+      cb = uniform(__shfl(cb, 0));
+      if ((uint64_t)tb + n_t <= a.spill_cap && (uint64_t)cb + n_c <= a.spill_chunk_cap) return true;
+      for (uint64_t id = (uint64_t)cb + lane; id < (uint64_t)cb + n_c && id < a.spill_chunk_cap; id += WAVE)
+        a.spill_owner[id] = NONE;
+      return false;
+    };
+
     // write spill task ti: positions 0..depth-2 from m[], position depth-1 =
     // extra, candidate range [lo, hi) of backward neighbour jm in chunks from ci
     auto emit_task = [&](uint32_t& ti, uint32_t& ci, uint32_t depth, uint32_t jm, uint32_t lo,
@@ -1081,14 +1158,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
     // backward neighbour jm) as a task of the next round
     auto spill_leaves = [&](uint32_t v, uint32_t jm, uint32_t lo, uint32_t hi) -> bool {
       const uint2 ch = chunking(hi - lo, a.chunk);
-      uint32_t tb = 0, cb = 0;
-      if (lane == 0) {
-        tb = reserve(&a.ctrl->nspill, 1u, a.spill_cap);
-        cb = tb == NONE ? NONE : reserve(&a.ctrl->nspill_chunks, ch.x, a.spill_chunk_cap);
-      }
-      tb = uniform(__shfl(tb, 0));
-      cb = uniform(__shfl(cb, 0));
-      if (tb == NONE || cb == NONE) return false;  // (a task slot may leak; never read)
+      uint32_t tb, cb;
+      if (!reserve_spill(1u, ch.x, tb, cb)) return false;
       emit_task(tb, cb, L1, jm, lo, hi, ch, v);
       return true;
     };
@@ -1109,7 +1180,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
             self = i < nF && S.F[i] == v;
           } else {
             const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
-            uint32_t bw = P->bw[L1][jf];
+            uint32_t bw = S.pbw[L1][jf];
             uint32_t x = ((bw >> 8) << vbits) | v;
             self = contains(adj, ats, S.plo[L1][jf], S.phi[L1][jf], x, fd, pos) &&
                    last_fixed_ok(v, jf);
@@ -1127,7 +1198,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       if (lane < bn) {
         if (jp != NONE) {
           uint32_t ry;
-          uint2 sg = seg_of(a.g, v, P->bw[L1][jp] >> 8, &ry);
+          uint2 sg = seg_of(a.g, v, S.pbw[L1][jp] >> 8, &ry);
           olo = sg.x;
           ohi = sg.y;
           odirty = VERSIONED ? (ry >> 31) : 0u;
@@ -1175,13 +1246,13 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       PROF_ADD(2, popc64(ballot(lane < bn && src == 1)));
       PROF_ADD(3, popc64(ballot(lane < bn && src == 0 && size > 0)));
       PROF_ADD(4, jf == NONE ? 1u : 0u);
-      PROF_ADD(5, uniform(P->nb[L1]));
+      PROF_ADD(5, uniform(S.pnb[L1]));
       PROF_ADD(6, uniform(popc64(ballot(lane < bn)) ? (uint32_t)wave_sum((unsigned long long)(lane < bn ? (ohi - olo) : 0u)) : 0u));
       PROF_ADD(7, S.lfsize);
       S.pref[lane] = incl - size;
       if (lane == 0) S.pref[WAVE] = total;
       wave_sync();
-      const uint32_t nfb = uniform(P->nb[L1]);
+      const uint32_t nfb = uniform(S.pnb[L1]);
       const uint32_t fsum_lo = jf != NONE ? S.plo[L1][jf] : 0u;
       for (uint32_t b = 0; b < total; b += WAVE) {
         const uint32_t g = b + lane;
@@ -1216,7 +1287,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
             ok = i < nF && S.F[i] == w;
           }
           if (ok && jp != NONE && !(sp & 1u)) {
-            uint32_t x = ((P->bw[L1][jp] >> 8) << vbits) | w;
+            uint32_t x = ((S.pbw[L1][jp] >> 8) << vbits) | w;
             ok = contains(adj, ats, S.olo[p], S.ohi[p], x, VERSIONED && ((sp >> 1) & 1u), pos);
           }
         }
@@ -1228,7 +1299,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
           if (ok) {
             const uint32_t vp = S.cand[L2][p];
             for (uint32_t i = 0; i < k; ++i) {
-              uint32_t mv = (i == P->qv[L1]) ? w : (i == P->qv[L2]) ? vp : S.mq[i];
+              uint32_t mv = (i == S.pqv[L1]) ? w : (i == S.pqv[L2]) ? vp : S.mq[i];
               h = mix64(h + (uint64_t)mv + 1);
             }
           }
@@ -1251,14 +1322,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       n_t = uniform(n_t);
       n_c = uniform(n_c);
       if (n_t == 0) return true;
-      uint32_t tb = 0, cb = 0;
-      if (lane == 0) {
-        tb = reserve(&a.ctrl->nspill, n_t, a.spill_cap);
-        cb = tb == NONE ? NONE : reserve(&a.ctrl->nspill_chunks, n_c, a.spill_chunk_cap);
-      }
-      tb = uniform(__shfl(tb, 0));
-      cb = uniform(__shfl(cb, 0));
-      if (tb == NONE || cb == NONE) return false;  // (a task slot may leak; never read)
+      uint32_t tb, cb;
+      if (!reserve_spill(n_t, n_c, tb, cb)) return false;
       uint32_t ti = tb, ci = cb;
       for (uint32_t L = d0; L <= lv_now; ++L) {
         uint32_t cu = S.cur[L], en = S.end[L];
@@ -1309,7 +1374,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
             uint64_t h = 0;
             if (ok)
               for (uint32_t i = 0; i < k; ++i) {
-                uint32_t mv = (i == P->qv[L]) ? v : S.mq[i];
+                uint32_t mv = (i == S.pqv[L]) ? v : S.mq[i];
                 h = mix64(h + (uint64_t)mv + 1);
               }
             hsum += wave_sum(h);
@@ -1325,7 +1390,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
       if (L == L2) {
         if (cur < end) {
           PROF_BEGIN(t2);
-          PROF_ADD(8, uniform(P->nb[L]));
+          PROF_ADD(8, uniform(S.pnb[L]));
           PROF_ADD(9, (end - cur) < WAVE ? (end - cur) : WAVE);
           uint32_t v;
           bool ok = test(L, cur + lane, end, v);
@@ -1347,7 +1412,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) v
         uint32_t v = S.cand[L][bi];
         S.bi[L] = bi + 1;
         S.m[L] = v;
-        if (HASH) S.mq[P->qv[L]] = v;
+        if (HASH) S.mq[S.pqv[L]] = v;
         ++lv;
         PROF_BEGIN(en);
         bool entered = enter(L + 1, 0u, 0xFFFFFFFFu, -1);
