@@ -31,6 +31,28 @@ METRIC = "updates/s + p99 update latency, served vs per-update GPU path, exact c
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_search_c2_hbm_traffic.md")
+
+
+def profiled_traffic(config, algo, batch, avg_launch_us):
+    """HBM bytes per search launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it describes this workload and its average launch
+    duration is within 25 % of the live one (otherwise the kernel changed
+    since it was profiled: None)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            line = [x for x in f if x.startswith("machine-readable:")][-1]
+    except (OSError, IndexError):
+        return None
+    kv = dict(tok.split("=", 1) for tok in line.split()[1:])
+    if (kv.get("config"), kv.get("algo"), int(kv.get("batch", 0))) != (config, algo, batch):
+        return None
+    prof_us = float(kv["avg_launch_us"])
+    if not avg_launch_us or abs(prof_us - avg_launch_us) > 0.25 * avg_launch_us:
+        return None
+    return int(kv["traffic_bytes_per_launch"])
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -190,8 +212,10 @@ def main() -> int:
     per_launch_bytes = alg / max(launches, 1)
     avg_ns = k_ns / max(launches, 1)
     achieved = (per_launch_bytes / avg_ns) if avg_ns else 0.0  # bytes/ns == GB/s
+    traffic = profiled_traffic(args.config, args.algo, args.batch, avg_ns / 1e3)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "traffic_source": (os.path.relpath(TRAFFIC_PROFILE, HERE) if traffic else None),
             "kernel": "k_search", "avg_launch_us": round(avg_ns / 1e3, 2),
             "alg_bytes_per_launch": int(per_launch_bytes), "launches": int(launches),
             "kernel_share_of_step": round(k_ns / 1e9 / elapsed, 4) if elapsed else None}

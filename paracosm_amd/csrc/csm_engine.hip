@@ -1138,7 +1138,7 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
     uint32_t v1 = (uint32_t)std::min<uint64_t>((uint64_t)v0 + step, e->nv);
     k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
     GraphView g = view(e);
-    k_initial_seeds<<<grid_for(v1 - v0, 256), 256, 0, e->stream>>>(
+    k_initial_seeds<<<grid_for(v1 - v0, 256 / WAVE, 8192), 256, 0, e->stream>>>(
         g, e->d_plans, q.plan_base, la, key01, v0, v1, e->opts.split_chunk, e->d_seeds,
         e->seed_cap, e->d_chunk_owner, e->chunk_cap, e->d_ctrl, e->d_d2, e->d2_words,
         (e->algo == ALGO_SYMBI && e->d_d2) ? 1u : 0u, e->d_counts);

@@ -191,17 +191,6 @@ __device__ __forceinline__ bool contains(const uint32_t* __restrict__ adj,
   return false;
 }
 
-// reserve n slots below cap without leaving holes; NONE if they do not fit
-__device__ __forceinline__ uint32_t reserve(uint32_t* counter, uint32_t n, uint32_t cap) {
-  uint32_t cur = __atomic_load_n(counter, __ATOMIC_RELAXED);
-  while (true) {
-    if ((uint64_t)cur + n > cap) return NONE;
-    uint32_t prev = atomicCAS(counter, cur, cur + n);
-    if (prev == cur) return cur;
-    cur = prev;
-  }
-}
-
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -675,6 +664,48 @@ struct SeedArgs {
   uint8_t* safe;
 };
 
+// Emit the depth-2 seeds of the lanes with `have` set (one task each, cut into
+// ch.x chunks) with one reservation per wave.  Chunk ids reserved past the
+// capacity are flagged; ids below it whose task did not fit are marked
+// ownerless (NONE), so every id the search can reach has been written.
+__device__ __forceinline__ void emit_seed_wave(bool have, uint32_t t, uint32_t qo, uint32_t m0,
+                                               uint32_t m1, uint32_t bj, uint32_t blo,
+                                               uint32_t bhi, uint2 ch, Task* seeds,
+                                               uint32_t seed_cap, uint32_t* chunk_owner,
+                                               uint32_t chunk_cap, Ctrl* ctrl) {
+  const uint32_t lane = lane_id();
+  const uint64_t hm = ballot(have);
+  if (!hm) return;
+  const uint32_t nt = popc64(hm), ti = popc64(hm & ((1ull << lane) - 1ull));
+  uint32_t incl = have ? ch.x : 0u;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    uint32_t y = __shfl_up(incl, o);
+    if ((int)lane >= o) incl += y;
+  }
+  const uint32_t nc = uniform(__shfl(incl, WAVE - 1));
+  uint32_t tb = 0, cb = 0;
+  if (lane == 0) {
+    tb = atomicAdd(&ctrl->nseeds, nt);
+    cb = atomicAdd(&ctrl->nchunks, nc);
+    if ((uint64_t)tb + nt > seed_cap) atomicOr(&ctrl->err, ERR_SEED_OVERFLOW);
+    if ((uint64_t)cb + nc > chunk_cap) atomicOr(&ctrl->err, ERR_CHUNK_OVERFLOW);
+  }
+  tb = uniform(__shfl(tb, 0));
+  cb = uniform(__shfl(cb, 0));
+  if (!have) return;
+  const uint32_t sidx = tb + ti;
+  const bool fits = (uint64_t)sidx < seed_cap;
+  const uint32_t first = cb + incl - ch.x;
+  if (fits) {
+    Task* tk = seeds + sidx;
+    tk->t = t; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
+    tk->base = first; tk->csize = ch.y; tk->m[0] = m0; tk->m[1] = m1;
+  }
+  for (uint32_t c = 0, id = first; c < ch.x; ++c, ++id)
+    if ((uint64_t)id < chunk_cap) chunk_owner[id] = fits ? sidx : NONE;
+}
+
 // One wavefront per update, one lane per seed template: fix m[0], m[1], find
 // the depth-2 candidate segment (the smaller label-filtered segment of the
 // two endpoints) and emit the seed as a task cut into chunks.  Updates with
@@ -682,7 +713,6 @@ struct SeedArgs {
 // parallel_graphflow.cpp:1990-2016).
 __global__ void k_seeds(SeedArgs a) {
   const uint32_t lane = lane_id();
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
   for (uint32_t i = wid; i < a.b.n; i += nw) {
@@ -743,38 +773,8 @@ __global__ void k_seeds(SeedArgs a) {
           }
         }
       }
-      // one reservation per wave: task slots and chunk ids
-      const uint64_t hm = ballot(have);
-      if (!hm) continue;
-      const uint32_t nt = popc64(hm), ti = popc64(hm & lt_mask);
-      uint32_t incl = have ? ch.x : 0u;
-#pragma unroll
-      for (int o = 1; o < WAVE; o <<= 1) {
-        uint32_t t = __shfl_up(incl, o);
-        if ((int)lane >= o) incl += t;
-      }
-      const uint32_t nc = uniform(__shfl(incl, WAVE - 1));
-      uint32_t tb = 0, cb = 0;
-      if (lane == 0) {
-        tb = atomicAdd(&a.ctrl->nseeds, nt);
-        cb = atomicAdd(&a.ctrl->nchunks, nc);
-        if ((uint64_t)tb + nt > a.seed_cap) atomicOr(&a.ctrl->err, ERR_SEED_OVERFLOW);
-        if ((uint64_t)cb + nc > a.chunk_cap) atomicOr(&a.ctrl->err, ERR_CHUNK_OVERFLOW);
-      }
-      tb = uniform(__shfl(tb, 0));
-      cb = uniform(__shfl(cb, 0));
-      if (have) {
-        const uint32_t sidx = tb + ti;
-        const bool fits = (uint64_t)sidx < a.seed_cap;
-        if (fits) {
-          Task* tk = a.seeds + sidx;
-          tk->t = i; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
-          tk->base = cb + incl - ch.x; tk->csize = ch.y; tk->m[0] = m0; tk->m[1] = m1;
-        }
-        // every reserved chunk id below the cap gets an owner (NONE: skipped)
-        for (uint32_t c = 0, id = cb + incl - ch.x; c < ch.x; ++c, ++id)
-          if ((uint64_t)id < a.chunk_cap) a.chunk_owner[id] = fits ? sidx : NONE;
-      }
+      emit_seed_wave(have, i, qo, m0, m1, bj, blo, bhi, ch, a.seeds, a.seed_cap, a.chunk_owner,
+                     a.chunk_cap, a.ctrl);
     }
   }
 }
@@ -1506,38 +1506,50 @@ __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, 
                                 uint32_t* chunk_owner, uint32_t chunk_cap, Ctrl* ctrl,
                                 const uint32_t* __restrict__ d2, uint32_t d2_words, uint32_t symbi,
                                 unsigned long long* counts) {
+  const uint32_t lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
   const PlanRec* P = plans + qo;
-  for (uint32_t v = v0 + blockIdx.x * blockDim.x + threadIdx.x; v < v1;
-       v += gridDim.x * blockDim.x) {
+  const uint32_t k = P->k, nb2 = k > 2 ? P->nb[2] : 0u;
+  // one wave per first-position vertex v, one lane per entry of its segment
+  for (uint32_t v = v0 + wid; v < v1; v += nw) {
     if (g.vlab[v] != la) continue;
     if (symbi && !d2_bit(d2, P->d2row[0], d2_words, v)) continue;
     uint32_t ry;
-    uint2 sg = seg_of(g, v, key01, &ry);
-    bool dirty = (ry & DIRTY) != 0;
-    for (uint32_t i = sg.x; i < sg.y; ++i) {
-      if (dirty && !ts_visible(g.ats[i], 0xFFFFu)) continue;
-      uint32_t w = g.adj[i] & g.vmask;
-      if (symbi && !d2_bit(d2, P->d2row[1], d2_words, w)) continue;
-      if (P->k == 2) { atomicAdd(&counts[P->q], 1ull); continue; }
-      uint32_t best = NONE, blo = 0, bhi = 0, bj = 0;
-      for (uint32_t jj = 0; jj < P->nb[2]; ++jj) {
-        uint32_t bw = P->bw[2][jj];
-        uint32_t vx = (bw & 0xFFu) == 0 ? v : w;
-        uint32_t ry2;
-        uint2 s2 = seg_of(g, vx, bw >> 8, &ry2);
-        uint32_t sz = s2.y - s2.x;
-        if (best == NONE || sz < best) { best = sz; blo = s2.x; bhi = s2.y; bj = jj; }
+    const uint2 sg = seg_of(g, v, key01, &ry);
+    const bool dirty = (ry & DIRTY) != 0;
+    for (uint32_t i0 = sg.x; i0 < sg.y; i0 += WAVE) {
+      const uint32_t i = i0 + lane;
+      bool have = false, ok = i < sg.y;
+      uint32_t w = 0, blo = 0, bhi = 0, bj = 0;
+      uint2 ch = make_uint2(0u, 0u);
+      if (ok) {
+        if (dirty) ok = ts_visible(g.ats[i], 0xFFFFu);
+        w = g.adj[i] & g.vmask;
+        if (ok && symbi) ok = d2_bit(d2, P->d2row[1], d2_words, w);
       }
-      if (best == 0 || best == NONE) continue;
-      uint2 ch = chunking(best, chunk);
-      uint32_t sidx = reserve(&ctrl->nseeds, 1u, seed_cap);
-      if (sidx == NONE) { atomicOr(&ctrl->err, ERR_SEED_OVERFLOW); continue; }
-      uint32_t base = reserve(&ctrl->nchunks, ch.x, chunk_cap);
-      if (base == NONE) { atomicOr(&ctrl->err, ERR_CHUNK_OVERFLOW); continue; }
-      Task* tk = seeds + sidx;
-      tk->t = 0; tk->qo = qo; tk->depth = 2; tk->jmin = bj; tk->lo = blo; tk->hi = bhi;
-      tk->base = base; tk->csize = ch.y; tk->m[0] = v; tk->m[1] = w;
-      for (uint32_t c = 0; c < ch.x; ++c) chunk_owner[base + c] = sidx;
+      if (k == 2) {
+        const uint32_t n2 = popc64(ballot(ok));
+        if (lane == 0 && n2) atomicAdd(&counts[P->q], (unsigned long long)n2);
+        continue;
+      }
+      if (ok) {
+        uint32_t best = NONE;
+        for (uint32_t jj = 0; jj < nb2; ++jj) {
+          uint32_t bw = P->bw[2][jj];
+          uint32_t vx = (bw & 0xFFu) == 0 ? v : w;
+          uint32_t ry2;
+          uint2 s2 = seg_of(g, vx, bw >> 8, &ry2);
+          uint32_t sz = s2.y - s2.x;
+          if (best == NONE || sz < best) { best = sz; blo = s2.x; bhi = s2.y; bj = jj; }
+        }
+        if (best != 0 && best != NONE) {
+          have = true;
+          ch = chunking(best, chunk);
+        }
+      }
+      emit_seed_wave(have, 0u, qo, v, w, bj, blo, bhi, ch, seeds, seed_cap, chunk_owner,
+                     chunk_cap, ctrl);
     }
   }
 }
