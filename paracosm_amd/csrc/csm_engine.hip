@@ -613,20 +613,21 @@ void launch_search_mk(csm_engine* e, const SearchArgs& sa) {
   static int per_cu = -1;
   if (per_cu < 0) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search<MK, true, H, S>,
-                                                     SEARCH_WAVES * WAVE, 0) != hipSuccess ||
+                                                     search_waves<MK>() * WAVE, 0) != hipSuccess ||
         per_cu < 1)
       per_cu = 2;
     if (getenv("CSM_TRACE_ROUNDS"))
       fprintf(stderr, "[csm] search kernel MK=%d: %d workgroups per CU, %d CUs\n", MK, per_cu,
               e->num_cus);
   }
-  k_search<MK, true, H, S><<<e->num_cus * per_cu, SEARCH_WAVES * WAVE, 0, e->stream>>>(sa);
+  k_search<MK, true, H, S><<<e->num_cus * per_cu, search_waves<MK>() * WAVE, 0, e->stream>>>(sa);
 }
 
 // the smallest LDS state that holds the largest registered query
 template <bool H, bool S>
 void launch_search(csm_engine* e, const SearchArgs& sa) {
   if (e->max_k <= 8) launch_search_mk<8, H, S>(e, sa);
+  else if (e->max_k <= 16) launch_search_mk<16, H, S>(e, sa);
   else launch_search_mk<MAXK, H, S>(e, sa);
 }
 

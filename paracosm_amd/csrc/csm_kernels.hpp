@@ -37,10 +37,14 @@ constexpr uint32_t TS_DEAD = 0xFFFF0000u;   // never visible
 constexpr uint32_t POS_MAX = 0xFFFEu;       // largest update position in a batch
 constexpr uint32_t NO_LABEL = 0xFFFFFFFFu;
 constexpr uint32_t NONE = 0xFFFFFFFFu;
-constexpr int MAXK = 16;                    // query vertices supported by the search
+constexpr int MAXK = 32;                    // query vertices supported by the search
 constexpr int MAXNB = MAXK - 1;
 constexpr int WAVE = 64;
-constexpr int SEARCH_WAVES = 4;             // waves per search workgroup
+constexpr int SEARCH_WAVES = 4;             // waves per search workgroup (MK <= 16)
+// waves per search workgroup for queries of up to MK vertices: the per-wave
+// LDS state grows with MK^2, so the largest layout runs two waves per group
+template <int MK>
+constexpr int search_waves() { return MK > 16 ? 2 : SEARCH_WAVES; }
 constexpr uint32_t FCAP = 512;              // LDS leaf filter entries per wave
 constexpr uint32_t LEAF_SPLIT = 8192;       // a parent with more leaves becomes a task
 
@@ -877,9 +881,9 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t 
 #define CSM_SEARCH_WAVES_PER_EU 5
 #endif
 template <int MK, bool VERSIONED, bool HASH, bool SYMBI>
-__global__ __launch_bounds__(SEARCH_WAVES * WAVE)
-__attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : 5, 8))) void k_search(SearchArgs a) {
-  __shared__ WaveState<MK> S_all[SEARCH_WAVES];
+__global__ __launch_bounds__(search_waves<MK>() * WAVE)
+__attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 ? 3 : 1, 8))) void k_search(SearchArgs a) {
+  __shared__ WaveState<MK> S_all[search_waves<MK>()];
   const uint32_t lane = lane_id();
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   WaveState<MK>& S = S_all[threadIdx.x / WAVE];

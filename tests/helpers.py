@@ -87,3 +87,57 @@ def query_star(leaves, label=0):
                  src=np.zeros(leaves, np.uint32),
                  dst=np.arange(1, leaves + 1, dtype=np.uint32),
                  elabels=np.zeros(leaves, np.uint32))
+
+
+def walk_query(vl, src, dst, el, k, rng, return_vertices=False):
+    """The subgraph induced by the first k distinct vertices of a random walk
+    on the data graph (so the query has at least one embedding), relabelled
+    0..k-1."""
+    adj = {}
+    for a, b in zip(src.tolist(), dst.tolist()):
+        adj.setdefault(a, []).append(b)
+        adj.setdefault(b, []).append(a)
+    starts = sorted(adj)
+    cur = starts[int(rng.integers(0, len(starts)))]
+    seen = [cur]
+    for _ in range(100000):
+        if len(seen) == k:
+            break
+        nb = adj[cur]
+        cur = nb[int(rng.integers(0, len(nb)))]
+        if cur not in seen:
+            seen.append(cur)
+    assert len(seen) == k, "the walk's component is smaller than k"
+    idx = {v: i for i, v in enumerate(seen)}
+    es = [(idx[a], idx[b], l) for a, b, l in zip(src.tolist(), dst.tolist(), el.tolist())
+          if a in idx and b in idx]
+    q = Query(vlabels=np.asarray(vl[seen], np.uint32),
+              src=np.asarray([e[0] for e in es], np.uint32),
+              dst=np.asarray([e[1] for e in es], np.uint32),
+              elabels=np.asarray([e[2] for e in es], np.uint32))
+    return (q, seen) if return_vertices else q
+
+
+def walk_case(seed, k, nv=150, p=0.05, n_vl=3, n_el=2):
+    """A data graph, a k-vertex query induced by a random walk on it, and a
+    stream that inserts every edge among the walk's vertices (plus a fifth of
+    the others, held out of the initial graph) in random order and then
+    deletes half of the inserted edges: the walk's embedding appears during
+    the stream and may disappear again."""
+    rng = np.random.default_rng(seed)
+    vl, s, d, el = random_graph(nv, p, n_vl, n_el, rng)
+    q, verts = walk_query(vl, s, d, el, k, rng, return_vertices=True)
+    vset = set(verts)
+    inq = np.array([a in vset and b in vset for a, b in zip(s.tolist(), d.tolist())])
+    other = np.flatnonzero(~inq)
+    held = np.concatenate([np.flatnonzero(inq),
+                           rng.choice(other, size=len(other) // 5, replace=False)])
+    keep = np.setdiff1d(np.arange(len(s)), held)
+    ins = rng.permutation(held)
+    dels = rng.choice(ins, size=len(ins) // 2, replace=False)
+    add = np.concatenate([np.ones(len(ins), np.uint8), np.zeros(len(dels), np.uint8)])
+    e = np.concatenate([ins, dels])
+    sw = rng.random(len(e)) < 0.5
+    v1 = np.where(sw, d[e], s[e]).astype(np.uint32)
+    v2 = np.where(sw, s[e], d[e]).astype(np.uint32)
+    return vl, (s[keep], d[keep], el[keep]), (add, v1, v2, el[e].astype(np.uint32)), q

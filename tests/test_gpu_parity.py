@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from tests.helpers import (mixed_stream_small, query_path2, query_star, query_triangle,
-                           random_connected_query, random_graph)
+                           random_connected_query, random_graph, walk_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -296,3 +296,33 @@ def test_spilled_subtrees_are_exact(oracle_lib, budget, algo):
         np.testing.assert_array_equal(neg[:, 0], ref.neg)
         np.testing.assert_array_equal(hsh[:, 0], ref.hash)
         assert budget > 5 or rounds > 1
+
+
+@pytest.mark.parametrize("k", [12, 17, 24])
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_large_queries(oracle_lib, k, algo):
+    """Queries beyond 8 and 16 vertices run the larger LDS layouts of the
+    search; a small spill budget also cuts their deep prefixes into tasks."""
+    from paracosm_amd.engine import make_updates
+    for seed in (0, 1):
+        vl, init, ups, q = walk_case(700 + 10 * k + seed, k)
+        ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+        assert ref.pos.sum() > 0
+        for budget in (0, 2):
+            kw = {"spill_budget": budget, "split_chunk": 4} if budget else {}
+            with _engine(algo, q, vl, init, want_hash=True, **kw) as m:
+                pos, neg, _, hsh = m.apply_batch(make_updates(*ups))
+            print("large", algo, k, seed, budget, "edges", len(q.src), "total",
+                  int(ref.pos.sum() + ref.neg.sum()), flush=True)
+            np.testing.assert_array_equal(pos[:, 0], ref.pos)
+            np.testing.assert_array_equal(neg[:, 0], ref.neg)
+            np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+
+
+def test_query_beyond_32_vertices_is_rejected():
+    from paracosm_amd.engine import CsmError, Matching
+    rng = np.random.default_rng(5)
+    vl, s, d, el = random_graph(60, 0.2, 2, 1, rng)
+    q = random_connected_query(33, 0, 2, 1, rng)
+    with pytest.raises(CsmError):
+        Matching("parallel_graphflow", q, vl, s, d, el).close()
