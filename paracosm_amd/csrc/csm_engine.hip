@@ -766,6 +766,10 @@ int check_errors(csm_engine* e) {
   if (!err) return CSM_OK;
   if (err & ERR_DELETE_MISSING)
     return fail(e, CSM_E_DELETE_MISSING, "deletion error (update %u)", e->h_ctrl->err_at);
+  if (err & ERR_VERTEX_UPDATE)
+    return fail(e, CSM_E_UNSUPPORTED,
+                "vertex record at %u in a device batch; vertex updates go through "
+                "csm_apply_batch, csm_add_vertex or csm_remove_vertex", e->h_ctrl->err_at);
   if (err & ERR_BAD_UPDATE)
     return fail(e, CSM_E_ARG, "invalid update (unknown vertex or label, or self-loop) at %u",
                 e->h_ctrl->err_at);
@@ -782,11 +786,13 @@ int symbi_batch_insert(csm_engine* e, uint32_t n);
 int symbi_batch_delete(csm_engine* e, uint32_t n);
 int symbi_add_vertex(csm_engine* e, uint32_t id);
 
-// The batch pipeline; updates already unpacked into d_type/d_add/d_v1/d_v2/d_lab.
+// The batch pipeline; k_batch_begin has run and the updates are unpacked into
+// d_type/d_add/d_v1/d_v2/d_lab.  With search == false the graph and the SymBi
+// index are updated but no match is searched (vertex deletion).
 int dir_reserve(csm_engine* e, uint32_t n);
 
 int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
-              uint8_t* safe) {
+              uint8_t* safe, bool search_matches = true) {
   int rc;
   // room for this batch's rewrites: at most every list twice plus the new entries
   uint64_t live = e->h_ctrl->live;
@@ -799,7 +805,6 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   const size_t nq = e->queries.size();
   HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
   if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
-  k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
   if (e->opts.match_budget) {
     HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
     HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
@@ -840,8 +845,10 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   if (e->algo == ALGO_SYMBI && e->d_d2)
     if ((rc = symbi_batch_insert(e, n))) return rc;
   // 5. seeds + search
-  seeds(e, n, counts, hashes, safe, true);
-  if ((rc = search(e, true, counts, hashes))) return rc;
+  if (search_matches) {
+    seeds(e, n, counts, hashes, safe, true);
+    if ((rc = search(e, true, counts, hashes))) return rc;
+  }
   // 6. ... and deletions (after it)
   if (e->algo == ALGO_SYMBI && e->d_d2)
     if ((rc = symbi_batch_delete(e, n))) return rc;
@@ -1160,6 +1167,55 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
   return CSM_OK;
 }
 
+// Vertex deletion (Graph::RemoveVertex, graph.cpp:51-56, called by the
+// plugins' RemoveVertex, e.g. parallel_graphflow.cpp:2311-2315 and
+// symbi.cpp:992-1021): the vertex loses its label and its edges, and no match
+// is counted, as in the reference.  The reference leaves the vertex's entries
+// in its neighbours' lists (they no longer match its label); here every
+// incident edge is deleted through the batch pipeline with the search off, so
+// no list keeps an entry for it and the SymBi index stays the fix-point of the
+// remaining graph.
+int remove_vertex(csm_engine* e, uint32_t id) {
+  if (id >= e->nv || e->h_vlab[id] == NO_LABEL) return CSM_OK;
+  uint2 rec;
+  HIP_TRY(e, hipMemcpy(&rec, e->d_vrec + id, sizeof(uint2), hipMemcpyDeviceToHost));
+  const uint32_t len = rec.y & LEN_MASK;
+  std::vector<uint32_t> ent(len), ver(len);
+  if (len) {
+    HIP_TRY(e, hipMemcpy(ent.data(), e->d_adj[e->cur] + rec.x, len * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(e, hipMemcpy(ver.data(), e->d_ats[e->cur] + rec.x, len * 4, hipMemcpyDeviceToHost));
+  }
+  std::vector<UpdateRec> dels;
+  for (uint32_t i = 0; i < len; ++i) {
+    if ((ver[i] & 0xFFFFu) != 0xFFFFu) continue;  // deleted in an earlier batch
+    const uint32_t nbr = ent[i] & e->vmask, key = ent[i] >> e->vbits;
+    dels.push_back(UpdateRec{'e', 0, 0, id, nbr, e->el_map[key % e->nel]});
+  }
+  for (size_t done = 0; done < dels.size();) {
+    const uint32_t m = (uint32_t)std::min<size_t>(dels.size() - done, e->max_batch);
+    std::memcpy(e->h_upd, dels.data() + done, m * sizeof(UpdateRec));
+    HIP_TRY(e, hipMemcpyAsync(e->d_upd, e->h_upd, m * sizeof(UpdateRec), hipMemcpyHostToDevice,
+                              e->stream));
+    k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+    k_unpack<<<grid_for(m, 256), 256, 0, e->stream>>>(e->d_upd, m, e->d_type, e->d_add, e->d_v1,
+                                                      e->d_v2, e->d_lab, e->d_elmap,
+                                                      (uint32_t)e->el_map.size(), e->d_ctrl);
+    int rc = run_batch(e, m, e->d_counts, nullptr, e->d_safe, false);
+    if (rc) return rc;
+    if ((rc = read_ctrl(e))) return rc;
+    if ((rc = check_errors(e))) return rc;
+    done += m;
+  }
+  e->num_edges -= dels.size();
+  e->h_vlab[id] = NO_LABEL;
+  HIP_TRY(e, hipMemcpyAsync(e->d_vlab + id, &e->h_vlab[id], 4, hipMemcpyHostToDevice, e->stream));
+  if (e->algo == ALGO_SYMBI && e->d_sxrows)
+    k_sx_remove_vertex<<<1, 256, 0, e->stream>>>(sx_view(e), e->d2_rows, e->sx_n1rows,
+                                                 e->sx_n2rows, id);
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CSM_OK;
+}
+
 }  // namespace
 
 // ================================================================== C-ABI
@@ -1409,8 +1465,10 @@ int csm_add_vertex(csm_engine* e, uint32_t id, uint32_t label) {
 }
 
 int csm_remove_vertex(csm_engine* e, uint32_t id) {
-  (void)id;
-  return fail(e, CSM_E_UNSUPPORTED, "vertex deletion is not supported on the device");
+  if (!e) return CSM_E_ARG;
+  if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
+  HIP_TRY(e, hipSetDevice(e->dev));
+  return remove_vertex(e, id);
 }
 
 int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
@@ -1421,9 +1479,10 @@ int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
   if (d_hash && !e->opts.want_hash) d_hash = nullptr;
   if (n == 0) return CSM_OK;
   HIP_TRY(e, hipSetDevice(e->dev));
+  k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
   k_unpack<<<grid_for(n, 256), 256, 0, e->stream>>>(
       reinterpret_cast<const UpdateRec*>(d_updates), (uint32_t)n, e->d_type, e->d_add, e->d_v1,
-      e->d_v2, e->d_lab, e->d_elmap, (uint32_t)e->el_map.size());
+      e->d_v2, e->d_lab, e->d_elmap, (uint32_t)e->el_map.size(), e->d_ctrl);
   int rc = run_batch(e, (uint32_t)n, reinterpret_cast<unsigned long long*>(d_counts),
                      reinterpret_cast<unsigned long long*>(d_hash), d_safe);
   if (rc) return rc;
@@ -1455,16 +1514,33 @@ int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t
   const size_t nq = e->queries.size();
   size_t done = 0;
   while (done < n) {
+    // a vertex deletion is applied on its own, in stream order
+    if (updates[done].type == 'v' && !updates[done].is_add) {
+      int rc = remove_vertex(e, updates[done].id1);
+      if (rc) return rc;
+      for (size_t q = 0; q < nq; ++q) {
+        size_t o = done * nq + q;
+        if (pos) pos[o] = 0;
+        if (neg) neg[o] = 0;
+        if (emb_hash) emb_hash[o] = 0;
+      }
+      if (safe) safe[done] = 0;
+      ++done;
+      continue;
+    }
     size_t m = std::min<size_t>(n - done, e->max_batch);
-    // vertex insertions take effect before the slice's edges
-    size_t j = 0;
+    for (size_t i = 1; i < m; ++i)
+      if (updates[done + i].type == 'v' && !updates[done + i].is_add) { m = i; break; }
+    // vertex insertions take effect before the slice's edges; vertex records
+    // travel to the device as no-ops so that positions are kept
     for (size_t i = 0; i < m; ++i) {
       const csm_update& u = updates[done + i];
+      std::memcpy(&e->h_upd[i], &u, sizeof(UpdateRec));
       if (u.type == 'v') {
-        if (u.is_add) { int rc = csm_add_vertex(e, u.id1, u.label); if (rc) return rc; }
-        else return fail(e, CSM_E_UNSUPPORTED, "vertex deletion is not supported on the device");
+        int rc = csm_add_vertex(e, u.id1, u.label);
+        if (rc) return rc;
+        e->h_upd[i].type = 0;
       }
-      std::memcpy(&e->h_upd[j++], &u, sizeof(UpdateRec));
     }
     HIP_TRY(e, hipMemcpyAsync(e->d_upd, e->h_upd, m * sizeof(UpdateRec), hipMemcpyHostToDevice,
                               e->stream));

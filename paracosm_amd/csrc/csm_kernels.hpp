@@ -54,6 +54,7 @@ constexpr uint32_t ERR_SEED_OVERFLOW = 2u;
 constexpr uint32_t ERR_CHUNK_OVERFLOW = 4u;
 constexpr uint32_t ERR_POOL_OVERFLOW = 8u;
 constexpr uint32_t ERR_BAD_UPDATE = 16u;
+constexpr uint32_t ERR_VERTEX_UPDATE = 64u;   // a vertex record inside a device batch
 
 struct Ctrl {
   uint32_t pool_top;      // bump pointer of the current pool (entries)
@@ -306,12 +307,19 @@ struct BatchView {
 // Unpack csm_update records (16 B) into structure-of-arrays.
 struct UpdateRec { uint8_t type, add; uint16_t pad; uint32_t id1, id2, label; };
 
+// Vertex records are applied on the host (the host entry points turn them into
+// no-op records of type 0); one reaching a device batch is an error.
 __global__ void k_unpack(const UpdateRec* __restrict__ u, uint32_t n, uint8_t* __restrict__ type,
                          uint8_t* __restrict__ add, uint32_t* __restrict__ v1,
                          uint32_t* __restrict__ v2, uint32_t* __restrict__ lab,
-                         const uint32_t* __restrict__ elab_map, uint32_t elab_map_n) {
+                         const uint32_t* __restrict__ elab_map, uint32_t elab_map_n,
+                         Ctrl* __restrict__ ctrl) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     UpdateRec r = u[i];
+    if (r.type == 'v') {
+      atomicOr(&ctrl->err, ERR_VERTEX_UPDATE);
+      atomicMin(&ctrl->err_at, i);
+    }
     type[i] = r.type;
     add[i] = r.add;
     v1[i] = r.id1;

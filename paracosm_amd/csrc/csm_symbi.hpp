@@ -155,6 +155,21 @@ __global__ void k_sx_add_vertex(SxView x, uint32_t nrows, uint32_t v, uint32_t l
   }
 }
 
+// Vertex deletion: v leaves every index row.  Its incident edges are already
+// deleted, so what remains are the unconditional d1/d2 bits of rows without
+// parents (and children) and counters that are zero.
+__global__ void k_sx_remove_vertex(SxView x, uint32_t nrows, uint32_t n1rows, uint32_t n2rows,
+                                   uint32_t v) {
+  for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+    bit_clear(x.d1, x.words, r, v);
+    bit_clear(x.d2, x.words, r, v);
+    *cnt_at(x.np1, x.nvc, r, v) = 0;
+    *cnt_at(x.nc2, x.nvc, r, v) = 0;
+  }
+  for (uint32_t r = threadIdx.x; r < n1rows; r += blockDim.x) *cnt_at(x.n1, x.nvc, r, v) = 0;
+  for (uint32_t r = threadIdx.x; r < n2rows; r += blockDim.x) *cnt_at(x.n2, x.nvc, r, v) = 0;
+}
+
 // ---------------------------------------------------- changed-edge marks
 // Per update: 1 = insertion of an edge absent before the batch (first version
 // of its value in the list), 2 = deletion leaving the edge absent after it

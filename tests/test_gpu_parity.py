@@ -326,3 +326,86 @@ def test_query_beyond_32_vertices_is_rejected():
     q = random_connected_query(33, 0, 2, 1, rng)
     with pytest.raises(CsmError):
         Matching("parallel_graphflow", q, vl, s, d, el).close()
+
+
+def _present_after(init, ups, n):
+    """Edge set {(min, max): label} after the first n updates."""
+    present = {(min(a, b), max(a, b)): l for a, b, l in zip(*[x.tolist() for x in init])}
+    add, v1, v2, lab = [x.tolist() for x in ups]
+    for i in range(n):
+        key = (min(v1[i], v2[i]), max(v1[i], v2[i]))
+        if add[i]:
+            present[key] = lab[i]
+        else:
+            present.pop(key, None)
+    return present
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_remove_vertex_matches_oracle(oracle_lib, algo):
+    """A vertex deletion inside a stream removes the vertex and its edges
+    without counting matches (as the reference's RemoveVertex); the updates
+    after it are exact against the oracle run on the graph without those
+    edges, and the SymBi index is the fix-point of that graph."""
+    O = oracle_lib
+    from paracosm_amd.engine import make_updates
+    for seed in (0, 1, 2):
+        vl, init, ups, q = _case(900 + seed, nv=45, p=0.3, n_vl=2, n_el=2, n_upd=140, k=4,
+                                 extra=seed % 2)
+        cut = 60
+        present = _present_after(init, ups, cut)
+        deg = {}
+        for a, b in present:
+            deg[a] = deg.get(a, 0) + 1
+            deg[b] = deg.get(b, 0) + 1
+        h = max(deg, key=lambda v: (deg[v], -v))
+        inc = sorted(k for k in present if h in k)
+        # the rest of the stream without updates that touch h
+        add, v1, v2, lab = ups
+        rest = [i for i in range(cut, len(v1)) if v1[i] != h and v2[i] != h]
+        B = tuple(x[rest] for x in ups)
+        A = tuple(x[:cut] for x in ups)
+        D = (np.zeros(len(inc), np.uint8), np.asarray([a for a, _ in inc], np.uint32),
+             np.asarray([b for _, b in inc], np.uint32),
+             np.asarray([present[k] for k in inc], np.uint32))
+        ref = _oracle(O, q, vl, init, tuple(np.concatenate(z) for z in zip(A, D, B)))
+        stream = make_updates(*(np.concatenate(z) for z in zip(A, ([0], [h], [0], [0]), B)),
+                              vertex=np.r_[np.zeros(cut, bool), [True], np.zeros(len(rest), bool)])
+        with _engine(algo, q, vl, init) as m:
+            pos, neg, safe, _ = m.apply_batch(stream)
+            print("remove_vertex", algo, seed, "vertex", h, "edges", len(inc), "total",
+                  int(ref.pos.sum() + ref.neg.sum()), flush=True)
+            np.testing.assert_array_equal(pos[:cut, 0], ref.pos[:cut])
+            np.testing.assert_array_equal(neg[:cut, 0], ref.neg[:cut])
+            assert pos[cut, 0] == 0 and neg[cut, 0] == 0
+            np.testing.assert_array_equal(pos[cut + 1:, 0], ref.pos[cut + len(inc):])
+            np.testing.assert_array_equal(neg[cut + 1:, 0], ref.neg[cut + len(inc):])
+            if algo == "parallel_symbi":
+                nv = len(vl)
+                vl2 = vl.copy()
+                vl2[h] = 1000  # a label no query vertex carries: h is in no index row
+                fin = _present_after(init, tuple(np.concatenate(z) for z in zip(A, D, B)),
+                                     cut + len(inc) + len(rest))
+                es = sorted(fin)
+                dg = O.OGraph(vl2, np.asarray([a for a, _ in es], np.uint32),
+                              np.asarray([b for _, b in es], np.uint32),
+                              np.asarray([fin[k] for k in es], np.uint32))
+                qg = O.OGraph.from_query(q)
+                for what, rows in ((1, q.k), (2, q.k), (3, q.k), (4, q.k), (5, q.num_edges),
+                                   (6, 2 * q.num_edges)):
+                    np.testing.assert_array_equal(m.index_dump(0, what, nv, rows),
+                                                  O.symbi_dump(dg, qg, what, nv, rows),
+                                                  err_msg=f"what {what}")
+
+
+def test_vertex_record_in_a_device_batch_is_an_error():
+    from paracosm_amd.engine import CsmError, make_updates
+    vl, init, ups, q = _case(950)
+    u = make_updates(*(x[:8] for x in ups))
+    u["type"][3] = ord("v")
+    with _engine("parallel_graphflow", q, vl, init) as m:
+        d_u = m.device_buffer(u.nbytes)
+        d_c = m.device_buffer(8 * len(u))
+        d_u.upload(u.view(np.uint8))
+        with pytest.raises(CsmError):
+            m.apply_batch_device(d_u.ptr, len(u), d_c.ptr)
