@@ -833,7 +833,12 @@ struct WaveState {
   // F: the last position's valid vertices with respect to all fixed
   // neighbours (ascending); lF = |F|, or NONE when F was not built
   uint32_t lF;
+  uint32_t lCeff;             // C without the vertex at position k-3
+  uint32_t fvalid;            // F still holds for the current prefix
   uint32_t F[FCAP];
+  // data vertex whose segment plo/phi[p][j] currently hold (NONE: none); the
+  // graph does not change during a launch, so a hit needs no lookup
+  uint32_t pvx[MK][MK - 1];
   // the current plan's per-position fields (plan index pqo)
   uint32_t pqo;
   uint32_t pnb[MK], pd2[MK], pqv[MK];
@@ -895,7 +900,11 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
   const uint32_t lane = lane_id();
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   WaveState<MK>& S = S_all[threadIdx.x / WAVE];
-  if (lane_id() == 0) S.pqo = NONE;
+  if (lane_id() == 0) {
+    S.pqo = NONE;
+    S.fvalid = 0;
+  }
+  for (uint32_t i = lane; i < MK * (MK - 1); i += WAVE) S.pvx[i / (MK - 1)][i % (MK - 1)] = NONE;
   wave_sync();
   const uint32_t* __restrict__ adj = a.g.adj;
   const uint32_t* __restrict__ ats = a.g.ats;
@@ -934,8 +943,10 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
         S.pd2[lane] = P->d2row[lane];
         S.pqv[lane] = P->qv[lane];
       }
-      for (uint32_t i = lane; i < MK * (MK - 1); i += WAVE)
+      for (uint32_t i = lane; i < MK * (MK - 1); i += WAVE) {
         S.pbw[i / (MK - 1)][i % (MK - 1)] = P->bw[i / (MK - 1)][i % (MK - 1)];
+        S.pvx[i / (MK - 1)][i % (MK - 1)] = NONE;
+      }
       if (lane == 0) S.pqo = qo;
       wave_sync();
     }
@@ -944,6 +955,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       S.m[lane] = v;
       if (HASH) S.mq[S.pqv[lane]] = v;
     }
+    if (lane == 0) S.fvalid = 0;
     wave_sync();
     unsigned long long cnt = 0, hsum = 0, flushed = 0;
     uint32_t ticks = 0;
@@ -995,11 +1007,11 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     };
 
     // is w a valid last-position vertex w.r.t. the fixed neighbours (all but
-    // `skip`), injectivity against m[0..k-3] and the d2 bit
-    auto last_fixed_ok = [&](uint32_t w, uint32_t skip) -> bool {
+    // `skip`), injectivity against m[0..ninj-1] and the d2 bit
+    auto last_fixed_ok = [&](uint32_t w, uint32_t skip, uint32_t ninj) -> bool {
       bool ok = true;
       if (!a.homo)
-        for (uint32_t i = 0; i < L2; ++i) ok = ok && (S.m[i] != w);
+        for (uint32_t i = 0; i < ninj; ++i) ok = ok && (S.m[i] != w);
       if (SYMBI && ok) ok = d2_bit(a.d2, S.pd2[L1], a.d2_words, w);
       const uint32_t nb = uniform(S.pnb[L1]);
       for (uint32_t j = 0; j < nb; ++j) {
@@ -1013,48 +1025,85 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     };
 
     // fixed neighbours of the last position, once per entry of position k-2;
-    // false if some fixed segment is empty (then no embedding can complete)
+    // false if no leaf can complete (an empty fixed segment or an empty F).
+    // F is built without the injectivity test against position k-3, which the
+    // leaves make themselves, so it is reused across the entries of position
+    // k-2 that share the fixed vertices and the prefix m[0..k-4].
     auto prepare_last = [&]() -> bool {
       PROF_BEGIN(pl);
       const uint32_t nb = uniform(S.pnb[L1]);
-      uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu, is_par = 0;
+      uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu, is_par = 0, miss = 0;
       if (lane < nb) {
         uint32_t w = S.pbw[L1][lane];
         uint32_t bp = w & 0xFFu;
         if (bp == L2) {
           is_par = 1;
+          // keep the parent slot's cached dirty bit (the slot belongs to
+          // tasks that start at the last position)
+          my_dirty = VERSIONED ? ((S.pdirty[L1] >> lane) & 1u) : 0u;
         } else {
-          uint32_t ry;
-          uint2 sg = seg_of(a.g, S.m[bp], w >> 8, &ry);
-          my_lo = sg.x;
-          my_hi = sg.y;
-          my_size = sg.y - sg.x;
-          my_dirty = VERSIONED ? (ry >> 31) : 0u;
-          S.plo[L1][lane] = my_lo;
-          S.phi[L1][lane] = my_hi;
+          const uint32_t vx = S.m[bp];
+          if (vx == S.pvx[L1][lane]) {
+            my_lo = S.plo[L1][lane];
+            my_hi = S.phi[L1][lane];
+            my_dirty = VERSIONED ? ((S.pdirty[L1] >> lane) & 1u) : 0u;
+          } else {
+            miss = 1;
+            uint32_t ry;
+            uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
+            my_lo = sg.x;
+            my_hi = sg.y;
+            my_dirty = VERSIONED ? (ry >> 31) : 0u;
+            S.plo[L1][lane] = my_lo;
+            S.phi[L1][lane] = my_hi;
+            S.pvx[L1][lane] = vx;
+          }
+          my_size = my_hi - my_lo;
         }
       }
+      const bool reuse = S.fvalid != 0 && ballot(miss != 0) == 0;
       uint64_t pm = ballot(is_par != 0);
       uint32_t jp = pm ? (uint32_t)__builtin_ctzll(pm) : NONE;
       uint64_t mn = wave_min_u64(((uint64_t)my_size << 32) | lane);
       uint32_t jf = (uint32_t)(mn >> 32) == 0xFFFFFFFFu ? NONE : uniform((uint32_t)mn);
       uint32_t fsize = uniform((uint32_t)(mn >> 32));
-      S.ljp = jp;
-      S.ljf = jf;
-      S.lfsize = fsize;
-      S.pdirty[L1] = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
+      const uint32_t dmask = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
       n_bytes += 4ull * wave_sum((unsigned long long)(my_size == 0xFFFFFFFFu ? 0u : my_size));
+      wave_sync();
+      if (lane == 0) {
+        S.ljp = jp;
+        S.ljf = jf;
+        S.lfsize = fsize;
+        S.pdirty[L1] = dmask;
+        if (!reuse) S.fvalid = 0;
+      }
       wave_sync();
       if (jf != NONE && fsize == 0) return false;
       PROF_ADD(10, jp == NONE ? 1u : 0u);
       PROF_ADD(11, jf == NONE ? 0u : fsize);
-      S.lF = NONE;
+      const uint32_t mk3 = S.m[L2 - 1];
+      // C without the vertex at position k-3 (it is in F at most once)
+      auto c_eff = [&](uint32_t C) -> uint32_t {
+        if (a.homo || C == 0) return C;
+        const uint32_t i = lds_lower_bound(S.F, C, mk3);
+        return (i < C && S.F[i] == mk3) ? C - 1u : C;
+      };
+      if (reuse) {
+        const uint32_t Ce = c_eff(S.lC);
+        if (lane == 0) S.lCeff = Ce;
+        wave_sync();
+        PROF_END(pl, 2);
+        return Ce != 0;
+      }
+      if (lane == 0) S.lF = NONE;
+      wave_sync();
       if (jf != NONE && (fsize <= FCAP || (jp == NONE && !HASH))) {
         // walk the smallest fixed segment once: F (when it fits in LDS) and
-        // C = |F|, the number of valid leaves shared by every parent
+        // C, the number of valid leaves shared by every parent
         const uint32_t lo = uniform(__shfl(my_lo, jf)), hi = uniform(__shfl(my_hi, jf));
-        const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
+        const bool fd = VERSIONED && ((dmask >> jf) & 1u);
         const bool keep = fsize <= FCAP;
+        const uint32_t ninj = keep ? L2 - 1 : L2;
         uint32_t C = 0;
         for (uint32_t b = lo; b < hi; b += WAVE) {
           uint32_t e = b + lane;
@@ -1064,17 +1113,25 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
             w = adj[e] & vmask;
             if (fd) ok = ts_visible(ats[e], pos);
           }
-          ok = ok && last_fixed_ok(w, jf);
+          ok = ok && last_fixed_ok(w, jf, ninj);
           const uint64_t m = ballot(ok);
           if (keep && ok) S.F[C + popc64(m & lt_mask)] = w;
           C += popc64(m);
           n_cands += (hi - b) < WAVE ? (hi - b) : WAVE;
         }
-        S.lC = C;
-        if (keep) S.lF = C;
         work += (hi - lo) / WAVE;
         wave_sync();
-        if (C == 0) return false;   // no leaf can complete under this prefix
+        const uint32_t Ce = keep ? c_eff(C) : C;
+        if (lane == 0) {
+          S.lC = C;
+          S.lCeff = Ce;
+          if (keep) {
+            S.lF = C;
+            S.fvalid = 1;
+          }
+        }
+        wave_sync();
+        if (Ce == 0) return false;   // no leaf can complete under this prefix
       }
       PROF_END(pl, 2);
       return true;
@@ -1087,14 +1144,21 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       if (lane < nb) {
         uint32_t w = S.pbw[lv][lane];
         uint32_t vx = S.m[w & 0xFFu];
-        uint32_t ry;
-        uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
-        my_lo = sg.x;
-        my_hi = sg.y;
-        my_size = sg.y - sg.x;
-        my_dirty = VERSIONED ? (ry >> 31) : 0u;
-        S.plo[lv][lane] = my_lo;
-        S.phi[lv][lane] = my_hi;
+        if (vx == S.pvx[lv][lane]) {
+          my_lo = S.plo[lv][lane];
+          my_hi = S.phi[lv][lane];
+          my_dirty = VERSIONED ? ((S.pdirty[lv] >> lane) & 1u) : 0u;
+        } else {
+          uint32_t ry;
+          uint2 sg = seg_of(a.g, vx, w >> 8, &ry);
+          my_lo = sg.x;
+          my_hi = sg.y;
+          my_dirty = VERSIONED ? (ry >> 31) : 0u;
+          S.plo[lv][lane] = my_lo;
+          S.phi[lv][lane] = my_hi;
+          S.pvx[lv][lane] = vx;
+        }
+        my_size = my_hi - my_lo;
       }
       uint64_t mk = ((uint64_t)my_size << 32) | lane;
       uint64_t mn = (forced_j >= 0) ? (((uint64_t)__shfl(my_size, forced_j) << 32) | (uint32_t)forced_j)
@@ -1104,6 +1168,9 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       uint32_t dmask = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
       unsigned long long seg_total = wave_sum((unsigned long long)(lane < nb ? my_size : 0u));
       n_bytes += 4ull * seg_total;
+      // the cached slots' dirty bits and F's validity, before any early return
+      S.pdirty[lv] = dmask;
+      if (lv < L2) S.fvalid = 0;   // the prefix that F excludes has changed
       wave_sync();
       if (msz == 0) return false;
       uint32_t lo = uniform(__shfl(my_lo, jm)), hi = uniform(__shfl(my_hi, jm));
@@ -1114,7 +1181,6 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       S.bn[lv] = 0;
       S.bi[lv] = 0;
       S.jmin[lv] = jm;
-      S.pdirty[lv] = dmask;
       S.udirty[lv] = (dmask >> jm) & 1u;
       if (lo >= hi) return false;
       if (lv == L2) return prepare_last();
@@ -1182,6 +1248,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       const uint32_t jp = S.ljp, jf = S.ljf;
       const uint32_t nF = S.lF;          // F in LDS (NONE: not built)
       const bool useF = nF != NONE;
+      const uint32_t mk3 = S.m[L2 - 1];
       uint32_t v = lane < bn ? S.cand[L2][lane] : 0u;
       if (jp == NONE && !HASH) {
         // C - [parent is itself a valid leaf]
@@ -1195,10 +1262,10 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
             uint32_t bw = S.pbw[L1][jf];
             uint32_t x = ((bw >> 8) << vbits) | v;
             self = contains(adj, ats, S.plo[L1][jf], S.phi[L1][jf], x, fd, pos) &&
-                   last_fixed_ok(v, jf);
+                   last_fixed_ok(v, jf, L2);
           }
         }
-        cnt += (unsigned long long)bn * S.lC - popc64(ballot(self));
+        cnt += (unsigned long long)bn * S.lCeff - popc64(ballot(self));
         n_bytes += 4ull * bn * S.lfsize;
         PROF_END(fp, 5);
         return;
@@ -1293,10 +1360,14 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
           const uint32_t vp = S.cand[L2][p];
           if (ok && !a.homo) ok = w != vp;
           if (!useF) {
-            ok = ok && last_fixed_ok(w, (sp & 1u) ? NONE : jf);
-          } else if (ok && (sp & 1u)) {
-            const uint32_t i = lds_lower_bound(S.F, nF, w);
-            ok = i < nF && S.F[i] == w;
+            ok = ok && last_fixed_ok(w, (sp & 1u) ? NONE : jf, L2);
+          } else {
+            // F leaves the test against position k-3 to the leaves
+            if (ok && !a.homo) ok = w != mk3;
+            if (ok && (sp & 1u)) {
+              const uint32_t i = lds_lower_bound(S.F, nF, w);
+              ok = i < nF && S.F[i] == w;
+            }
           }
           if (ok && jp != NONE && !(sp & 1u)) {
             uint32_t x = ((S.pbw[L1][jp] >> 8) << vbits) | w;

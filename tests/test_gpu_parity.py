@@ -409,3 +409,24 @@ def test_vertex_record_in_a_device_batch_is_an_error():
         d_u.upload(u.view(np.uint8))
         with pytest.raises(CsmError):
             m.apply_batch_device(d_u.ptr, len(u), d_c.ptr)
+
+
+@pytest.mark.parametrize("chunk", [1, 3])
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_fine_chunks_exact(oracle_lib, chunk, algo):
+    """Tasks cut into chunks of 1 or 3 candidates: each wave interleaves many
+    tasks and plans, so state a wave keeps between chunks (cached segment
+    bounds and dirty bits, the leaf filter, plan fields) is exercised in
+    many orders; counts and hashes stay bit-exact."""
+    from paracosm_amd.engine import make_updates
+    for seed in range(6):
+        vl, init, ups, q = _case(1200 + seed, nv=60, p=0.3, n_vl=2, n_upd=150,
+                                 k=5 + seed % 4, extra=seed % 3)
+        ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+        with _engine(algo, q, vl, init, want_hash=True, split_chunk=chunk) as m:
+            pos, neg, _, hsh = m.apply_batch(make_updates(*ups))
+        print("fine", algo, chunk, seed, "k", q.k, "total", int(ref.pos.sum() + ref.neg.sum()),
+              flush=True)
+        np.testing.assert_array_equal(pos[:, 0], ref.pos)
+        np.testing.assert_array_equal(neg[:, 0], ref.neg)
+        np.testing.assert_array_equal(hsh[:, 0], ref.hash)
