@@ -162,9 +162,11 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
  * batch needed. */
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
 
-/* Diagnostic read-back of the SymBi index of one query (parallel_symbi
- * engines), the state behind Parrllel_SymBi's counters
- * (Parallel_SymBi/parallel.h:67-76; symbi.cpp:246-377): what = 0 DAG root
+/* Diagnostic read-back of the candidate index of one query (parallel_symbi
+ * and parallel_turboflux engines), the state behind Parrllel_SymBi's counters
+ * (Parallel_SymBi/parallel.h:67-76; symbi.cpp:246-377) and Parallel_TurboFlux's
+ * DCG over its spanning tree (parallel_turboflux.h:34-55,
+ * parallel_turboflux.cpp:326-469): what = 0 DAG/tree root
  * (out[0]); 1 d1, 2 d2 (0/1 per query vertex and data vertex, [k][nv]);
  * 3 np1, 4 nc2 ([k][nv]); 5 n1 (per query vertex, per parent, [nv] each);
  * 6 n2 (per query vertex, per neighbour in id order, [nv] each). */

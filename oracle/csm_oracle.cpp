@@ -455,7 +455,11 @@ struct SymBi {
   bool pe_insert = false;
   std::vector<std::pair<uint32_t, uint32_t>> pe_done;
 
-  SymBi(const Graph& q_, Graph& d_, Opts o_) : q(q_), d(d_), o(o_), k(q_.nv()) {
+  // tree = true: TurboFlux's spanning tree in place of SymBi's DAG (the index
+  // is the same construction over it).  dag_src: the graph whose label
+  // frequencies choose the root and the tree (default: d).
+  SymBi(const Graph& q_, Graph& d_, Opts o_, bool tree = false, const Graph* dag_src = nullptr)
+      : q(q_), d(d_), o(o_), k(q_.nv()) {
     eidx.assign(k, std::vector<uint32_t>(k, NOT_EXIST));
     uint32_t pos = 0;
     for (uint32_t i = 0; i < k; ++i)
@@ -464,7 +468,7 @@ struct SymBi {
     tn.assign(k, Node());
     ser.assign(k, 0);
     visited.assign(d.nv(), 0);
-    build_dag();
+    build_dag(tree, dag_src ? *dag_src : d);
     alloc();
     build_dcs(d1, d2, n1, n2, np1, nc2);
   }
@@ -479,7 +483,8 @@ struct SymBi {
     n2.assign(nde, std::vector<uint32_t>(V, 0));
   }
 
-  void build_dag() {  // symbi.cpp:113-244
+  // symbi.cpp:113-244; with `tree`, parallel_turboflux.cpp:123-320
+  void build_dag(bool tree, const Graph& g) {
     std::vector<std::vector<uint64_t>> freq(k, std::vector<uint64_t>(k, 0));
     uint64_t min_freq = UINT64_MAX;
     uint32_t mv0 = 0, mv1 = 0;
@@ -487,25 +492,26 @@ struct SymBi {
       for (size_t i = 0; i < q.nb[u].size(); ++i) {
         uint32_t uo = q.nb[u][i], uol = q.el[u][i];
         if (u > uo) { freq[u][uo] = freq[uo][u]; continue; }
-        for (uint32_t v = 0; v < d.nv(); ++v) {
-          if (d.label(v) == NOT_EXIST) continue;
-          if (d.label(v) != q.label(u)) continue;
-          for (size_t j = 0; j < d.nb[v].size(); ++j)
-            if (d.label(d.nb[v][j]) == q.label(uo) && d.el[v][j] == uol) ++freq[u][uo];
+        for (uint32_t v = 0; v < g.nv(); ++v) {
+          if (g.label(v) == NOT_EXIST) continue;
+          if (g.label(v) != q.label(u)) continue;
+          for (size_t j = 0; j < g.nb[v].size(); ++j)
+            if (g.label(g.nb[v][j]) == q.label(uo) && g.el[v][j] == uol) ++freq[u][uo];
         }
         if (freq[u][uo] < min_freq) { min_freq = freq[u][uo]; mv0 = u; mv1 = uo; }
       }
     }
     uint64_t f0 = 0, f1 = 0;
-    for (uint32_t v = 0; v < d.nv(); ++v) {
-      if (d.label(v) == NOT_EXIST) continue;
-      if (d.label(v) == q.label(mv0)) ++f0;
-      if (d.label(v) == q.label(mv1)) ++f1;
+    for (uint32_t v = 0; v < g.nv(); ++v) {
+      if (g.label(v) == NOT_EXIST) continue;
+      if (g.label(v) == q.label(mv0)) ++f0;
+      if (g.label(v) == q.label(mv1)) ++f1;
     }
     if (f0 > f1) root = mv1;
     else if (f0 < f1) root = mv0;
     else if (q.deg(mv0) > q.deg(mv1)) root = mv0;
     else root = mv1;
+    if (tree) { build_tree(freq); return; }
     std::vector<uint8_t> vis(k, 0), on(k, 0);
     std::deque<uint32_t> bfs{root};
     on[root] = 1;
@@ -522,6 +528,51 @@ struct SymBi {
         else { tn[u].bw.push_back(uo); tn[u].bwl.push_back(uol); }
         tn[u].nbrs.push_back(uo);
       }
+    }
+  }
+
+  // TurboFlux's spanning tree (parallel_turboflux.cpp:239-320): level by
+  // level from the root, repeatedly take the least frequent edge from the
+  // current level to an unvisited vertex (the first one on ties), drop every
+  // queued edge into that vertex, and queue its edges for the next level.
+  // The serialized tree is the BFS order over tree children.
+  void build_tree(const std::vector<std::vector<uint64_t>>& freq) {
+    struct E { uint32_t a, b; uint64_t f; };
+    std::vector<uint8_t> vis(k, 0);
+    vis[root] = 1;
+    std::vector<E> level;
+    for (uint32_t w : q.nb[root]) level.push_back({root, w, freq[root][w]});
+    while (!level.empty()) {
+      std::vector<E> next;
+      while (!level.empty()) {
+        size_t best = 0;
+        for (size_t i = 1; i < level.size(); ++i)
+          if (level[i].f < level[best].f) best = i;
+        const E sel = level[best];
+        std::vector<E> keep;
+        for (const E& x : level)
+          if (x.b != sel.b) keep.push_back(x);
+        level.swap(keep);
+        if (vis[sel.b]) continue;
+        vis[sel.b] = 1;
+        const uint32_t l = q.elabel(sel.a, sel.b);
+        tn[sel.a].fw.push_back(sel.b);
+        tn[sel.a].fwl.push_back(l);
+        tn[sel.b].bw.push_back(sel.a);
+        tn[sel.b].bwl.push_back(l);
+        for (uint32_t w : q.nb[sel.b])
+          if (!vis[w]) next.push_back({sel.b, w, freq[sel.b][w]});
+      }
+      level.swap(next);
+    }
+    for (uint32_t u = 0; u < k; ++u) tn[u].nbrs = q.nb[u];
+    std::deque<uint32_t> fifo{root};
+    uint32_t p = 0;
+    while (!fifo.empty()) {
+      uint32_t u = fifo.front();
+      fifo.pop_front();
+      ser[p++] = u;
+      for (uint32_t c : tn[u].fw) fifo.push_back(c);
     }
   }
 
@@ -897,12 +948,14 @@ struct Brute {
   bool homo, want_hash;
   std::vector<uint32_t> order, m;
   uint64_t cnt = 0, h = 0;
+  const std::vector<std::vector<uint8_t>>* d2 = nullptr;  // optional candidate filter
   void rec(uint32_t i) {
     uint32_t k = q.nv();
     if (i == k) { ++cnt; if (want_hash) h += emb_hash(m.data(), k); return; }
     uint32_t u = order[i];
     for (uint32_t v = 0; v < d.nv(); ++v) {
       if (d.label(v) != q.label(u)) continue;
+      if (d2 && !(*d2)[u][v]) continue;
       bool ok = true;
       if (!homo)
         for (uint32_t j = 0; j < i && ok; ++j) ok = m[order[j]] != v;
@@ -945,8 +998,26 @@ int orc_classify(const orc_graph* q, uint32_t l1, uint32_t l2, uint32_t el) {
   return 1;
 }
 
+uint64_t orc_brute_count_filtered(const orc_graph* d, const orc_graph* q, int homo,
+                                  uint64_t* hash, const std::vector<std::vector<uint8_t>>* d2);
+
 uint64_t orc_brute_count(const orc_graph* d, const orc_graph* q, int homo, uint64_t* hash) {
+  return orc_brute_count_filtered(d, q, homo, hash, nullptr);
+}
+
+// Embeddings whose every vertex image passes the d2 flag of the index with
+// the given shape (tree != 0: TurboFlux's tree, else SymBi's DAG) built on d.
+// d2 is a sound filter iff this equals orc_brute_count.
+uint64_t orc_d2_filtered_count(const orc_graph* d, const orc_graph* q, int tree) {
+  Graph g = d->g;
+  SymBi sb(q->g, g, Opts(), tree != 0);
+  return orc_brute_count_filtered(d, q, 0, nullptr, &sb.d2);
+}
+
+uint64_t orc_brute_count_filtered(const orc_graph* d, const orc_graph* q, int homo,
+                                  uint64_t* hash, const std::vector<std::vector<uint8_t>>* d2) {
   Brute b{q->g, d->g, homo != 0, hash != nullptr, {}, {}};
+  b.d2 = d2;
   uint32_t k = q->g.nv();
   b.m.assign(k, NOT_EXIST);
   // BFS-ish order: each next vertex adjacent to an earlier one when possible
@@ -1025,6 +1096,28 @@ int orc_run(const char* algo, const orc_graph* data, const orc_graph* query, siz
     }
     return finish(0);
   }
+  if (a == "symbi_index" || a == "turboflux_index") {
+    // index maintenance only (no search; counts stay 0), checked against a
+    // from-scratch build after every update when check_index is set
+    SymBi sb(q, d, o, a == "turboflux_index");
+    t0 = std::chrono::steady_clock::now();
+    for (size_t i = 0; i < n; ++i) {
+      if (add[i]) {
+        d.add_edge(v1[i], v2[i], lab[i]);
+        sb.index_update(v1[i], v2[i], lab[i], true);
+      } else {
+        uint32_t l = d.elabel(v1[i], v2[i]);
+        if (l == NOT_EXIST) { if (fail_at) *fail_at = i; return finish(-2); }
+        sb.index_update(v1[i], v2[i], l, false);
+        d.remove_edge(v1[i], v2[i]);
+      }
+      if (check_index && !sb.index_consistent()) { if (fail_at) *fail_at = i; return finish(-3); }
+      if (pos) pos[i] = 0;
+      if (neg) neg[i] = 0;
+      if (hash) hash[i] = 0;
+    }
+    return finish(0);
+  }
   if (a == "symbi") {
     SymBi sb(q, d, o);
     if (initial) { uint64_t h = 0; *initial = sb.initial(h); }
@@ -1047,10 +1140,21 @@ int orc_run(const char* algo, const orc_graph* data, const orc_graph* query, siz
 // layout of csm_index_dump: what = 0 root, 1 d1, 2 d2, 3 np1, 4 nc2 ([k][nv]),
 // 5 n1 (per vertex u, per DAG parent in BFS order), 6 n2 (per u, per
 // neighbour in id order).  Returns the number of words written, or -1.
+long orc_index_dump(const orc_graph* dag_src, const orc_graph* data, const orc_graph* query,
+                    int tree, uint32_t what, uint32_t* out, size_t n);
+
 long orc_symbi_dump(const orc_graph* data, const orc_graph* query, uint32_t what, uint32_t* out,
                     size_t n) {
+  return orc_index_dump(nullptr, data, query, 0, what, out, n);
+}
+
+// The same for either index shape: tree != 0 builds TurboFlux's spanning
+// tree (parallel_turboflux.cpp:123-469) instead of SymBi's DAG; the root and
+// the tree/DAG come from dag_src's label frequencies (nullptr: data's).
+long orc_index_dump(const orc_graph* dag_src, const orc_graph* data, const orc_graph* query,
+                    int tree, uint32_t what, uint32_t* out, size_t n) {
   Graph d = data->g;
-  SymBi sb(query->g, d, Opts());
+  SymBi sb(query->g, d, Opts(), tree != 0, dag_src ? &dag_src->g : nullptr);
   const uint32_t k = query->g.nv(), V = d.nv();
   size_t w = 0;
   auto put_row = [&](const auto& row) {
@@ -1073,6 +1177,18 @@ long orc_symbi_dump(const orc_graph* data, const orc_graph* query, uint32_t what
     if (!ok) return -1;
   }
   return (long)w;
+}
+
+// Parents of every query vertex in the index's DAG (tree != 0: TurboFlux's
+// spanning tree) built on `data`: out[u*k + j] = j-th parent of u, NOT_EXIST
+// past the last.  Returns the root.
+uint32_t orc_dag_parents(const orc_graph* data, const orc_graph* query, int tree, uint32_t* out) {
+  Graph d = data->g;
+  SymBi sb(query->g, d, Opts(), tree != 0);
+  const uint32_t k = query->g.nv();
+  for (uint32_t u = 0; u < k; ++u)
+    for (uint32_t j = 0; j < k; ++j) out[u * k + j] = j < sb.tn[u].bw.size() ? sb.tn[u].bw[j] : NOT_EXIST;
+  return sb.root;
 }
 
 // Matching orders of the GraphFlow plugin (graphflow.cpp:36-160), for

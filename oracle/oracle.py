@@ -48,6 +48,13 @@ def lib():
         L.orc_symbi_dump.restype = ctypes.c_long
         L.orc_symbi_dump.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, _u32p,
                                      ctypes.c_size_t]
+        L.orc_index_dump.restype = ctypes.c_long
+        L.orc_index_dump.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int, ctypes.c_uint32, _u32p, ctypes.c_size_t]
+        L.orc_d2_filtered_count.restype = ctypes.c_uint64
+        L.orc_d2_filtered_count.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.orc_dag_parents.restype = ctypes.c_uint32
+        L.orc_dag_parents.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _u32p]
         L.orc_graphflow_orders.restype = ctypes.c_int
         L.orc_graphflow_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p]
         _lib = L
@@ -135,3 +142,30 @@ def symbi_dump(data: OGraph, query: OGraph, what: int, nv: int, rows: int) -> np
     if n < 0:
         raise ValueError("orc_symbi_dump failed")
     return out if what == 0 else out.reshape(rows, nv)
+
+
+def index_dump(data: OGraph, query: OGraph, what: int, nv: int, rows: int, tree: bool = False,
+               dag_src: OGraph = None) -> np.ndarray:
+    """From-scratch index of `data` in the layout of csm_index_dump: SymBi's
+    DAG, or TurboFlux's spanning tree (tree=True); the root and DAG/tree are
+    chosen from dag_src's label frequencies (default: data's)."""
+    out = np.zeros(max(1, rows * nv), dtype=np.uint32)
+    n = lib().orc_index_dump(dag_src.h if dag_src is not None else None, data.h, query.h,
+                             1 if tree else 0, what, out, out.size)
+    if n < 0:
+        raise ValueError("orc_index_dump failed")
+    return out if what == 0 else out.reshape(rows, nv)
+
+
+def d2_filtered_count(data: OGraph, query: OGraph, tree: bool) -> int:
+    """Embeddings whose every vertex passes the d2 flag of a from-scratch
+    index (SymBi DAG or TurboFlux tree): equals brute_count iff d2 is sound."""
+    return int(lib().orc_d2_filtered_count(data.h, query.h, 1 if tree else 0))
+
+
+def dag_parents(data: OGraph, query: OGraph, k: int, tree: bool):
+    """(root, parents) of the index DAG or TurboFlux tree built on `data`."""
+    out = np.zeros(k * k, dtype=np.uint32)
+    root = lib().orc_dag_parents(data.h, query.h, 1 if tree else 0, out)
+    par = [[int(x) for x in out[u * k:(u + 1) * k] if x != 0xFFFFFFFF] for u in range(k)]
+    return int(root), par

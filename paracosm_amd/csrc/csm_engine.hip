@@ -223,6 +223,12 @@ inline uint32_t dense_of(const std::vector<uint32_t>& map, uint32_t x) {
   return (it != map.end() && *it == x) ? (uint32_t)(it - map.begin()) : NO_LABEL;
 }
 
+// SymBi and TurboFlux keep the candidate index (d1/d2 and its counters on
+// SymBi's DAG or TurboFlux's spanning tree) and filter the search by d2
+inline bool uses_index(const csm_engine* e) {
+  return e->algo == ALGO_SYMBI || e->algo == ALGO_TURBOFLUX;
+}
+
 GraphView view(const csm_engine* e) {
   GraphView g;
   g.vrec = e->d_vrec;
@@ -282,7 +288,7 @@ int plan_query(csm_engine* e, QueryInfo& qi, uint32_t qid) {
     std::memset(&P, 0, sizeof P);
     P.k = g.k;
     P.q = qid;
-    P.symbi = e->algo == ALGO_SYMBI ? 1u : 0u;
+    P.symbi = uses_index(e) ? 1u : 0u;
     std::vector<uint32_t> posof(g.k, 0);
     for (uint32_t p = 0; p < g.k; ++p) posof[o[p]] = p;
     bool matchable = true;
@@ -632,7 +638,7 @@ void launch_search(csm_engine* e, const SearchArgs& sa) {
 }
 
 void launch_round(csm_engine* e, SearchArgs& sa, bool versioned, bool h) {
-  const bool s = e->algo == ALGO_SYMBI && e->d_d2 != nullptr;
+  const bool s = uses_index(e) && e->d_d2 != nullptr;
   if (e->ev_search[0]) (void)hipEventRecord(e->ev_search[0], e->stream);
   (void)versioned;  // every list carries version words; clean lists skip them
   if (h) { if (s) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa); }
@@ -742,7 +748,7 @@ void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long 
   a.tab_off = e->d_tab_off;
   a.tab_ent = e->d_tab_ent;
   a.plans = e->d_plans;
-  a.d2 = e->algo == ALGO_SYMBI ? e->d_d2 : nullptr;
+  a.d2 = uses_index(e) ? e->d_d2 : nullptr;
   a.d2_words = e->d2_words;
   a.nlab = e->nlab;
   a.nq = (uint32_t)e->queries.size();
@@ -842,7 +848,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
                                                          e->d_dirty[1 - e->dcur]);
   k_dedup<<<grid_for(n, 256), 256, 0, e->stream>>>(b, g, e->d_ats[e->cur]);
   // 4. SymBi index: insertions (before the search) ...
-  if (e->algo == ALGO_SYMBI && e->d_d2)
+  if (uses_index(e) && e->d_d2)
     if ((rc = symbi_batch_insert(e, n))) return rc;
   // 5. seeds + search
   if (search_matches) {
@@ -850,7 +856,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
     if ((rc = search(e, true, counts, hashes))) return rc;
   }
   // 6. ... and deletions (after it)
-  if (e->algo == ALGO_SYMBI && e->d_d2)
+  if (uses_index(e) && e->d_d2)
     if ((rc = symbi_batch_delete(e, n))) return rc;
   // next batch compacts what this one dirtied
   k_batch_end<<<1, 64, 0, e->stream>>>(e->d_ctrl);
@@ -977,7 +983,7 @@ int symbi_prepare_query(csm_engine* e, QueryInfo& q) {
     return e->h_triple_freq[(dl[a] * e->nlab + dl[b]) * e->nel + l];
   };
   auto vfreq = [&](uint32_t a) -> uint64_t { return dl[a] == NO_LABEL ? 0 : e->h_vfreq[dl[a]]; };
-  q.dag = symbi_dag(g, efreq, vfreq);
+  q.dag = e->algo == ALGO_TURBOFLUX ? turboflux_tree(g, efreq, vfreq) : symbi_dag(g, efreq, vfreq);
   // rows, n1 rows (one per DAG edge, at the child), n2 rows (one per directed edge)
   const uint32_t base = e->d2_rows;
   std::vector<std::vector<uint32_t>> n2id(k, std::vector<uint32_t>(k, NONE));
@@ -1149,7 +1155,7 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
     k_initial_seeds<<<grid_for(v1 - v0, 256 / WAVE, 8192), 256, 0, e->stream>>>(
         g, e->d_plans, q.plan_base, la, key01, v0, v1, e->opts.split_chunk, e->d_seeds,
         e->seed_cap, e->d_chunk_owner, e->chunk_cap, e->d_ctrl, e->d_d2, e->d2_words,
-        (e->algo == ALGO_SYMBI && e->d_d2) ? 1u : 0u, e->d_counts);
+        (uses_index(e) && e->d_d2) ? 1u : 0u, e->d_counts);
     int rc = read_ctrl(e);
     if (rc) return rc;
     if (e->h_ctrl->err & (ERR_SEED_OVERFLOW | ERR_CHUNK_OVERFLOW)) {
@@ -1209,7 +1215,7 @@ int remove_vertex(csm_engine* e, uint32_t id) {
   e->num_edges -= dels.size();
   e->h_vlab[id] = NO_LABEL;
   HIP_TRY(e, hipMemcpyAsync(e->d_vlab + id, &e->h_vlab[id], 4, hipMemcpyHostToDevice, e->stream));
-  if (e->algo == ALGO_SYMBI && e->d_sxrows)
+  if (uses_index(e) && e->d_sxrows)
     k_sx_remove_vertex<<<1, 256, 0, e->stream>>>(sx_view(e), e->d2_rows, e->sx_n1rows,
                                                  e->sx_n2rows, id);
   HIP_TRY(e, hipStreamSynchronize(e->stream));
@@ -1339,7 +1345,7 @@ int csm_register_query(csm_engine* e, const csm_graph* query, uint32_t* qid) {
   if ((rc = ensure_results(e))) return rc;
   if (e->preprocessed) {
     QueryInfo& q = *e->queries.back();
-    if (e->algo == ALGO_SYMBI) {
+    if (uses_index(e)) {
       q.d2_base = e->d2_rows;
       if ((rc = symbi_prepare_query(e, q))) return rc;
     }
@@ -1360,7 +1366,7 @@ int csm_preprocess(csm_engine* e) {
   e->h_plans.clear();
   for (size_t i = 0; i < e->queries.size(); ++i) {
     QueryInfo& q = *e->queries[i];
-    if (e->algo == ALGO_SYMBI) {
+    if (uses_index(e)) {
       q.d2_base = e->d2_rows;
       if ((rc = symbi_prepare_query(e, q))) return rc;
     }
@@ -1460,7 +1466,7 @@ int csm_add_vertex(csm_engine* e, uint32_t id, uint32_t label) {
   if (id >= e->nv) e->nv = id + 1;
   HIP_TRY(e, hipMemcpyAsync(e->d_vlab + id, &e->h_vlab[id], 4, hipMemcpyHostToDevice, e->stream));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
-  if (e->algo == ALGO_SYMBI && e->d_d2) return symbi_add_vertex(e, id);
+  if (uses_index(e) && e->d_d2) return symbi_add_vertex(e, id);
   return CSM_OK;
 }
 
@@ -1602,8 +1608,8 @@ int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
 
 int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, size_t n) {
   if (!e || !out) return CSM_E_ARG;
-  if (e->algo != ALGO_SYMBI || qid >= e->queries.size() || !e->preprocessed)
-    return fail(e, CSM_E_ARG, "no SymBi index for query %u", qid);
+  if (!uses_index(e) || qid >= e->queries.size() || !e->preprocessed)
+    return fail(e, CSM_E_ARG, "no candidate index for query %u", qid);
   HIP_TRY(e, hipSetDevice(e->dev));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
   QueryInfo& q = *e->queries[qid];
@@ -1652,7 +1658,10 @@ int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, si
   if (n < rows.size() * (size_t)nv) return fail(e, CSM_E_ARG, "dump buffer too small");
   for (size_t i = 0; i < rows.size(); ++i) {
     uint32_t* dst = out + i * nv;
-    if (bits) {
+    if (rows[i] == NONE) {
+      // a query edge outside TurboFlux's tree keeps no counters (zero rows)
+      std::fill(dst, dst + nv, 0u);
+    } else if (bits) {
       std::vector<uint32_t> w(e->d2_words);
       HIP_TRY(e, hipMemcpy(w.data(), arr + (size_t)rows[i] * e->d2_words, e->d2_words * 4,
                            hipMemcpyDeviceToHost));

@@ -124,22 +124,30 @@ struct Dag {
   std::vector<std::vector<uint32_t>> parents, children;
 };
 
+// The index root shared by SymBi and TurboFlux: the endpoint of the rarest
+// query edge (first in (u < w) order on ties) with the rarer vertex label,
+// ties to the higher query degree.
 template <class EdgeFreq, class VertexFreq>
-Dag symbi_dag(const QueryGraph& q, EdgeFreq edge_freq, VertexFreq vertex_freq) {
-  Dag d;
-  const uint32_t k = q.k;
+uint32_t index_root(const QueryGraph& q, EdgeFreq edge_freq, VertexFreq vertex_freq) {
   uint64_t min_f = UINT64_MAX;
   uint32_t a0 = 0, a1 = 0;
-  for (uint32_t u = 0; u < k; ++u)
+  for (uint32_t u = 0; u < q.k; ++u)
     for (uint32_t w : q.nb[u]) {
       if (u > w) continue;
       uint64_t f = edge_freq(u, w);
       if (f < min_f) { min_f = f; a0 = u; a1 = w; }
     }
   uint64_t f0 = vertex_freq(a0), f1 = vertex_freq(a1);
-  if (f0 > f1) d.root = a1;
-  else if (f0 < f1) d.root = a0;
-  else d.root = q.deg(a0) > q.deg(a1) ? a0 : a1;
+  if (f0 > f1) return a1;
+  if (f0 < f1) return a0;
+  return q.deg(a0) > q.deg(a1) ? a0 : a1;
+}
+
+template <class EdgeFreq, class VertexFreq>
+Dag symbi_dag(const QueryGraph& q, EdgeFreq edge_freq, VertexFreq vertex_freq) {
+  Dag d;
+  const uint32_t k = q.k;
+  d.root = index_root(q, edge_freq, vertex_freq);
   d.parents.assign(k, {});
   d.children.assign(k, {});
   std::vector<uint8_t> done(k, 0), queued(k, 0);
@@ -155,6 +163,56 @@ Dag symbi_dag(const QueryGraph& q, EdgeFreq edge_freq, VertexFreq vertex_freq) {
       if (!done[w]) d.children[u].push_back(w);
       else d.parents[u].push_back(w);
     }
+  }
+  return d;
+}
+
+// TurboFlux's spanning tree (ParaCOSM/CSM/matching/Parallel_TurboFlux/
+// parallel_turboflux.cpp:123-320): from the same root, level by level, take
+// the rarest queued edge into an unvisited vertex (the earliest queued on
+// ties), drop the other queued edges into that vertex, and queue the new
+// vertex's edges to unvisited vertices for the next level.  Edge frequency is
+// that of the (lower id, higher id) orientation.  `bfs` is the BFS order over
+// tree children.
+template <class EdgeFreq, class VertexFreq>
+Dag turboflux_tree(const QueryGraph& q, EdgeFreq edge_freq, VertexFreq vertex_freq) {
+  Dag d;
+  const uint32_t k = q.k;
+  d.root = index_root(q, edge_freq, vertex_freq);
+  d.parents.assign(k, {});
+  d.children.assign(k, {});
+  auto freq = [&](uint32_t a, uint32_t b) { return a < b ? edge_freq(a, b) : edge_freq(b, a); };
+  struct Cand { uint32_t from, to; uint64_t f; };
+  std::vector<uint8_t> seen(k, 0);
+  seen[d.root] = 1;
+  std::vector<Cand> level, next;
+  for (uint32_t w : q.nb[d.root]) level.push_back({d.root, w, freq(d.root, w)});
+  while (!level.empty()) {
+    next.clear();
+    while (!level.empty()) {
+      size_t best = 0;
+      for (size_t i = 1; i < level.size(); ++i)
+        if (level[i].f < level[best].f) best = i;
+      const Cand c = level[best];
+      size_t o = 0;
+      for (size_t i = 0; i < level.size(); ++i)
+        if (level[i].to != c.to) level[o++] = level[i];
+      level.resize(o);
+      if (seen[c.to]) continue;
+      seen[c.to] = 1;
+      d.children[c.from].push_back(c.to);
+      d.parents[c.to].push_back(c.from);
+      for (uint32_t w : q.nb[c.to])
+        if (!seen[w]) next.push_back({c.to, w, freq(c.to, w)});
+    }
+    level.swap(next);
+  }
+  std::deque<uint32_t> fifo{d.root};
+  while (!fifo.empty()) {
+    uint32_t u = fifo.front();
+    fifo.pop_front();
+    d.bfs.push_back(u);
+    for (uint32_t c : d.children[u]) fifo.push_back(c);
   }
   return d;
 }

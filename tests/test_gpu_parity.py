@@ -225,32 +225,37 @@ def _edges_after(init, ups, upto):
             np.asarray([edges[k] for k in ks], np.uint32))
 
 
+@pytest.mark.parametrize("algo", ["parallel_symbi", "parallel_turboflux"])
 @pytest.mark.parametrize("seed", range(5))
-def test_symbi_index_is_the_fixpoint(oracle_lib, seed):
-    """After every batch the device SymBi index (d1, d2, np1, nc2, n1, n2)
-    equals a from-scratch BuildDCS of the graph at that point."""
+def test_index_is_the_fixpoint(oracle_lib, algo, seed):
+    """After every batch the device index (d1, d2, np1, nc2, n1, n2) equals a
+    from-scratch build of the graph at that point: over SymBi's DAG, or over
+    TurboFlux's spanning tree (its DCG).  The DAG/tree is chosen from the
+    initial graph's label frequencies, as the engine does at Preprocessing."""
     O = oracle_lib
+    tree = algo == "parallel_turboflux"
     vl, init, ups, q = _case(40 + seed, nv=45, p=0.25, n_vl=2, n_el=2, n_upd=150,
                              k=4 + seed % 3, extra=seed % 3)
     from paracosm_amd.engine import make_updates
     nv = len(vl)
     k = q.k
-    npar = None
-    ref0 = None
+    g0 = O.OGraph(vl, *init)
+    qg = O.OGraph.from_query(q)
     u = make_updates(*ups)
-    with _engine("parallel_symbi", q, vl, init) as m:
+    n_dag_edges = k - 1 if tree else q.num_edges
+    with _engine(algo, q, vl, init) as m:
         bounds = [0, 40, 41, 100, len(u)]
         for bi in range(len(bounds)):
             if bi > 0:
                 m.apply_batch(u[bounds[bi - 1]:bounds[bi]])
             dg = O.OGraph(vl, *_edges_after(init, ups, bounds[bi]))
-            qg = O.OGraph.from_query(q)
-            assert m.index_dump(0, 0, nv, 1)[0] == O.symbi_dump(dg, qg, 0, nv, 1)[0]
-            # rows per dump kind: k, k, k, k, #DAG edges (= |E_q|), 2|E_q|
-            for what, rows in ((1, k), (2, k), (3, k), (4, k), (5, q.num_edges),
+            assert m.index_dump(0, 0, nv, 1)[0] == O.index_dump(dg, qg, 0, nv, 1, tree=tree,
+                                                                dag_src=g0)[0]
+            # rows per dump kind: k, k, k, k, #DAG or tree edges, 2|E_q|
+            for what, rows in ((1, k), (2, k), (3, k), (4, k), (5, n_dag_edges),
                                (6, 2 * q.num_edges)):
                 dev = m.index_dump(0, what, nv, rows)
-                ref = O.symbi_dump(dg, qg, what, nv, rows)
+                ref = O.index_dump(dg, qg, what, nv, rows, tree=tree, dag_src=g0)
                 np.testing.assert_array_equal(dev, ref, err_msg=f"batch {bi} what {what}")
 
 

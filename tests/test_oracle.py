@@ -8,7 +8,10 @@ for the embedding-set hash), the telescoping identity against
 InitialMatching, agreement of three independent search formulations
 (GraphFlow, SymBi, the parallel GraphFlow shape), the SymBi index fix-point
 (incremental maintenance == from-scratch BuildDCS after every update), and the
-label-triple filter checked exhaustively.
+label-triple filter checked exhaustively.  TurboFlux's spanning tree is pinned
+by a hand-worked case of its level-wise rarest-edge rule, by being a spanning
+tree of the query, by the soundness of its d2 filter against brute force, and
+by its own incremental fix-point.
 """
 import itertools
 from math import perm
@@ -207,3 +210,86 @@ def test_graphflow_orders_cover_every_query_edge(oracle_lib):
         assert firsts == sorted(zip(q.src.tolist(), q.dst.tolist()))
         for o in orders:
             assert sorted(o.tolist()) == list(range(q.k))
+
+
+# ------------------------------------------------ TurboFlux spanning tree
+def test_turboflux_tree_prefers_rare_edges_per_level(oracle_lib):
+    """Hand-worked case of the tree rule (parallel_turboflux.cpp:239-320):
+    query R-X, R-Y, X-Y, X-Z, Y-Z with distinct labels 0,1,2,3; R-X is the
+    rarest edge and label 0 the rarest vertex label, so R is the root; both
+    X and Y hang off R; on level 2, Y-Z (2 data edges) is rarer than X-Z (6),
+    so Z's parent is Y -- where a plain BFS (SymBi's DAG) makes Z a child of
+    both X and Y."""
+    O = oracle_lib
+    from paracosm_amd.gen import Query
+    q = Query(vlabels=np.asarray([0, 1, 2, 3], np.uint32),
+              src=np.asarray([0, 0, 1, 1, 2], np.uint32),
+              dst=np.asarray([1, 2, 2, 3, 3], np.uint32),
+              elabels=np.zeros(5, np.uint32))
+    # data: one vertex of label 0, four of each other label; edge counts per
+    # label pair: (0,1) 1, (0,2) 3, (1,2) 5, (1,3) 6, (2,3) 2
+    vl = np.asarray([0] + [1] * 4 + [2] * 4 + [3] * 4, np.uint32)
+    L1, L2, L3 = [1, 2, 3, 4], [5, 6, 7, 8], [9, 10, 11, 12]
+    edges = [(0, 1)] + [(0, v) for v in L2[:3]]
+    edges += [(a, b) for a, b in itertools.product(L1, L2)][:5]
+    edges += [(a, b) for a, b in itertools.product(L1, L3)][:6]
+    edges += [(a, b) for a, b in itertools.product(L2, L3)][:2]
+    s = np.asarray([e[0] for e in edges], np.uint32)
+    d = np.asarray([e[1] for e in edges], np.uint32)
+    dg = O.OGraph(vl, s, d, np.zeros(len(edges), np.uint32))
+    root, par = O.dag_parents(dg, O.OGraph.from_query(q), 4, tree=True)
+    assert root == 0
+    assert par == [[], [0], [0], [2]]
+    root_d, par_d = O.dag_parents(dg, O.OGraph.from_query(q), 4, tree=False)
+    assert root_d == 0 and sorted(par_d[3]) == [1, 2]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_turboflux_tree_is_a_spanning_tree(oracle_lib, seed):
+    O = oracle_lib
+    rng = np.random.default_rng(500 + seed)
+    vl, s, d, el = random_graph(30, 0.3, 3, 2, rng)
+    q = random_connected_query(3 + seed % 5, seed % 3, 3, 2, rng)
+    root, par = O.dag_parents(O.OGraph(vl, s, d, el), O.OGraph.from_query(q), q.k, tree=True)
+    qe = {(min(a, b), max(a, b)) for a, b in zip(q.src.tolist(), q.dst.tolist())}
+    assert par[root] == []
+    for u in range(q.k):
+        if u != root:
+            assert len(par[u]) == 1 and (min(u, par[u][0]), max(u, par[u][0])) in qe
+    # every vertex reaches the root
+    for u in range(q.k):
+        seen, x = set(), u
+        while x != root:
+            assert x not in seen
+            seen.add(x)
+            x = par[x][0]
+
+
+@pytest.mark.parametrize("tree", [False, True])
+@pytest.mark.parametrize("seed", range(6))
+def test_d2_is_a_sound_filter(oracle_lib, tree, seed):
+    """Every embedding maps each query vertex to a vertex whose d2 flag is set
+    (SymBi DAG and TurboFlux tree), and the filter is not trivial."""
+    O = oracle_lib
+    rng = np.random.default_rng(600 + seed)
+    vl, s, d, el = random_graph(14, 0.35, 2, 2, rng)
+    q = random_connected_query(3 + seed % 3, seed % 2, 2, 2, rng)
+    dg, qg = O.OGraph(vl, s, d, el), O.OGraph.from_query(q)
+    full = O.brute_count(dg, qg)
+    assert O.d2_filtered_count(dg, qg, tree) == full
+    d2 = O.index_dump(dg, qg, 2, len(vl), q.k, tree=tree)
+    label_ok = np.asarray([[vl[v] == q.vlabels[u] for v in range(len(vl))] for u in range(q.k)])
+    assert d2.sum() < label_ok.sum()
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_turboflux_index_is_the_fixpoint(oracle_lib, seed):
+    """Incremental maintenance over the tree (the SymBi rules restricted to
+    tree edges, parallel_turboflux.cpp:1823-2000) equals a from-scratch build
+    after every update."""
+    O = oracle_lib
+    rng = np.random.default_rng(650 + seed)
+    vl, s, d, el = random_graph(30, 0.25, 2, 2, rng)
+    init, ups = mixed_stream_small(s, d, el, 30, 120, 0.6, 2, rng)
+    q = random_connected_query(4 + seed % 3, 1 + seed % 2, 2, 2, rng)
+    O.run("turboflux_index", O.OGraph(vl, *init), O.OGraph.from_query(q), *ups, check_index=True)
