@@ -77,6 +77,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--split-chunk", type=int, default=0)
+    ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -202,6 +203,7 @@ def main() -> int:
                            "matches_per_s": round(st.matches / st.elapsed, 1)})
             log(f"served at {load:.0f}/s offered: {served[-1]}")
         srv.close()
+    ingest = None if args.no_ingest else ingest_rate(m, w)
     m.close()
 
     # roofline of the search kernel: algorithmic bytes per launch over its
@@ -254,10 +256,46 @@ def main() -> int:
         "matches": matches,
         "exact_counts_plain_vs_batched": exact,
         "roofline": roof,
+        "ingest": ingest,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def ingest_rate(m, w, reps=5):
+    """Device parsing of the workload's whole update stream as text
+    (`e a b l` / `-e a b l` lines, the Graph::LoadUpdateStream grammar),
+    resident in HBM: text bytes and records per second of csm_parse_text_device
+    (scratch allocation included), best of `reps`."""
+    from paracosm_amd.engine import PARSE_STREAM
+    tok = np.where(w.upd_add == 1, "e", "-e")
+    lines = np.char.add(np.char.add(np.char.add(np.char.add(tok, " "), w.upd_v1.astype(str)),
+                                    np.char.add(" ", w.upd_v2.astype(str))),
+                        np.char.add(" ", w.upd_label.astype(str)))
+    text = ("\n".join(lines.tolist()) + "\n").encode()
+    n = len(lines)
+    d_text = m.device_buffer(len(text))
+    d_text.upload(np.frombuffer(text, np.uint8))
+    d_rec = m.device_buffer(16 * n)
+    best = None
+    for _ in range(reps):
+        m.sync()
+        t = time.perf_counter()
+        got = m.parse_text_device(d_text.ptr, len(text), PARSE_STREAM, d_rec.ptr, n)
+        m.sync()
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    assert got == n
+    d_text.close()
+    d_rec.close()
+    out = {"text_bytes": len(text), "records": n, "seconds": round(best, 5),
+           "text_GB_per_s": round(len(text) / best / 1e9, 2),
+           "records_per_s": round(n / best, 1),
+           "note": "csm_parse_text_device on the whole update stream in HBM; host iostream "
+                   "parsing (Graph::LoadUpdateStream) is what it replaces"}
+    log(f"ingest: {out}")
+    return out
 
 
 def cpu_baseline(w, queries, seconds):

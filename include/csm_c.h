@@ -188,6 +188,35 @@ int csm_copy_device(csm_engine* e, void* dst, const void* src, size_t bytes);
 /* The HIP stream the engine launches on (hipStream_t). */
 void* csm_stream(csm_engine* e);
 
+/* Text ingestion on the device.  Parses the reference's input formats, one
+ * record per line, into csm_update records:
+ *   CSM_PARSE_GRAPH  - query / data graph files (Graph::LoadFromFile,
+ *                      graph.cpp:217-249): "t ..." headers skipped,
+ *                      "v id label" -> type 'v', "e a b label" -> type 'e'
+ *                      (is_add = 1);
+ *   CSM_PARSE_STREAM - update streams (Graph::LoadUpdateStream,
+ *                      graph.cpp:251-285): "v"/"-v id label",
+ *                      "e"/"-e a b label" (is_add = 0 for '-').
+ * Blank lines are skipped; ids and labels are decimal uint32.  Returns
+ * CSM_E_ARG for a malformed line (its 1-based number in *err_line) and
+ * CSM_E_CAPACITY when more than `cap` records result (*n holds the count,
+ * so cap = 0 sizes the output).  Replaces the host iostream parsing of
+ * Graph::LoadFromFile / LoadUpdateStream; no reference counterpart as an
+ * entry point. */
+#define CSM_PARSE_GRAPH 0
+#define CSM_PARSE_STREAM 1
+
+/* Host text in, host records out, on device `device` (no engine needed, so
+ * a data graph can be parsed before csm_create). */
+int csm_parse_text(int device, int mode, const char* text, size_t len, csm_update* out,
+                   size_t cap, size_t* n, uint64_t* err_line);
+
+/* Text already in HBM (16-byte aligned) to device records, on the engine's
+ * stream; scratch counts towards csm_device_bytes.  Stream records with edge
+ * updates only can go straight to csm_apply_batch_device. */
+int csm_parse_text_device(csm_engine* e, int mode, const char* d_text, size_t len,
+                          csm_update* d_out, size_t cap, size_t* n, uint64_t* err_line);
+
 const char* csm_last_error(const csm_engine* e);
 void csm_destroy(csm_engine* e);
 

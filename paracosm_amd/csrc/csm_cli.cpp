@@ -40,59 +40,64 @@ bool file_exists(const std::string& p) {
   return f.good();
 }
 
-// Graph::LoadFromFile grammar (graph.cpp:217-249): "t # n" headers skipped,
-// "v id label", anything else read as an edge "a b label".
-bool load_graph(const std::string& path, TextGraph& g) {
-  if (!file_exists(path)) {
+bool read_file(const std::string& path, std::vector<char>& buf) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  buf.clear();
+  char tmp[1 << 16];
+  size_t got;
+  while ((got = fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+  fclose(f);
+  return true;
+}
+
+// The whole file parsed on the device (csm_parse_text) in the grammar of
+// Graph::LoadFromFile (graph.cpp:217-249) or Graph::LoadUpdateStream
+// (graph.cpp:251-285).
+bool parse_file(const std::string& path, int mode, int device, std::vector<csm_update>& recs) {
+  std::vector<char> text;
+  if (!file_exists(path) || !read_file(path, text)) {
     std::cout << "Failed to open: " << path << std::endl;
     return false;
   }
-  std::ifstream ifs(path);
-  char type;
-  while (ifs >> type) {
-    if (type == 't') {
-      char c;
-      uint32_t n;
-      ifs >> c >> n;
-    } else if (type == 'v') {
-      uint32_t id, label;
-      ifs >> id >> label;
-      if (id >= g.vl.size()) g.vl.resize(id + 1, ABSENT);
-      if (g.vl[id] == ABSENT) g.vl[id] = label;
+  size_t lines = 1;
+  for (char c : text) lines += c == '\n';
+  recs.resize(lines);
+  size_t n = 0;
+  uint64_t bad = 0;
+  int rc = csm_parse_text(device, mode, text.data(), text.size(), recs.data(), recs.size(), &n, &bad);
+  if (rc == CSM_E_ARG && bad) {
+    std::cout << "malformed line " << bad << " in " << path << std::endl;
+    return false;
+  }
+  if (rc) {
+    std::cout << "parsing " << path << " on the device failed (" << rc << ")" << std::endl;
+    return false;
+  }
+  recs.resize(n);
+  return true;
+}
+
+// Graph records as Graph::AddVertex / AddEdge keep them (graph.cpp:23-93): a
+// vertex keeps its first label; edges in file order.
+bool load_graph(const std::string& path, int device, TextGraph& g) {
+  std::vector<csm_update> recs;
+  if (!parse_file(path, CSM_PARSE_GRAPH, device, recs)) return false;
+  for (const csm_update& u : recs) {
+    if (u.type == 'v') {
+      if (u.id1 >= g.vl.size()) g.vl.resize(u.id1 + 1, ABSENT);
+      if (g.vl[u.id1] == ABSENT) g.vl[u.id1] = u.label;
     } else {
-      uint32_t a, b, l;
-      ifs >> a >> b >> l;
-      g.src.push_back(a);
-      g.dst.push_back(b);
-      g.el.push_back(l);
+      g.src.push_back(u.id1);
+      g.dst.push_back(u.id2);
+      g.el.push_back(u.label);
     }
   }
   return true;
 }
 
-// Graph::LoadUpdateStream grammar (graph.cpp:251-285).
-bool load_stream(const std::string& path, std::vector<csm_update>& ups) {
-  if (!file_exists(path)) {
-    std::cout << "Failed to open: " << path << std::endl;
-    return false;
-  }
-  std::ifstream ifs(path);
-  std::string type;
-  while (ifs >> type) {
-    csm_update u;
-    memset(&u, 0, sizeof u);
-    if (type == "v" || type == "-v") {
-      u.type = 'v';
-      u.is_add = type == "v";
-      ifs >> u.id1 >> u.label;
-    } else {
-      u.type = 'e';
-      u.is_add = type == "e";
-      ifs >> u.id1 >> u.id2 >> u.label;
-    }
-    ups.push_back(u);
-  }
-  return true;
+bool load_stream(const std::string& path, int device, std::vector<csm_update>& ups) {
+  return parse_file(path, CSM_PARSE_STREAM, device, ups);
 }
 
 size_t count_unique_edges(const TextGraph& g) {
@@ -187,9 +192,9 @@ int main(int argc, char** argv) {
   auto t0 = std::chrono::steady_clock::now();
   std::cout << "----------- Loading graphs ------------" << std::endl;
   TextGraph q, d;
-  if (!load_graph(query_path, q)) return -1;
+  if (!load_graph(query_path, (int)device, q)) return -1;
   std::cout << "# vertices = " << q.vl.size() << "\n# edges = " << count_unique_edges(q) << std::endl;
-  if (!load_graph(data_path, d)) return -1;
+  if (!load_graph(data_path, (int)device, d)) return -1;
   std::cout << "# vertices = " << d.vl.size() << "\n# edges = " << count_unique_edges(d) << std::endl;
   std::cout << "Load Graphs: " << ms_since(t0) << "ms" << std::endl;
 
@@ -235,7 +240,7 @@ int main(int argc, char** argv) {
     std::cout << "use " << threads << " threads for incresemental matching " << std::endl;
   }
   std::vector<csm_update> ups;
-  if (!load_stream(stream_path, ups)) {
+  if (!load_stream(stream_path, (int)device, ups)) {
     csm_destroy(e);
     return -1;
   }

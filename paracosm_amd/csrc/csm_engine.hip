@@ -7,6 +7,7 @@
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -23,6 +24,7 @@
 #include "csm_kernels.hpp"
 #include "csm_plan.hpp"
 #include "csm_symbi.hpp"
+#include "csm_parse.hpp"
 
 using namespace csm;
 
@@ -1222,6 +1224,100 @@ int remove_vertex(csm_engine* e, uint32_t id) {
   return CSM_OK;
 }
 
+// Device parse of `len` bytes of text at d_text (see csm_parse.hpp) on
+// stream st, scratch through alloc/release; deliver(d_records, count) copies
+// the records out (it returns a CSM_* code).  Returns CSM_OK, CSM_E_ARG
+// (malformed line: *err_line, 1-based), CSM_E_CAPACITY or CSM_E_DEVICE; msg
+// gets the text.  *n = records found.
+template <class Alloc, class Release, class Deliver>
+int parse_text_on(hipStream_t st, int mode, const uint8_t* d_text, uint64_t len, size_t* n,
+                  uint64_t* err_line, Alloc alloc, Release release, Deliver deliver,
+                  std::string& msg) {
+  *n = 0;
+  if (err_line) *err_line = 0;
+  if (mode != PARSE_GRAPH && mode != PARSE_STREAM) { msg = "unknown parse mode"; return CSM_E_ARG; }
+  if (len == 0) return CSM_OK;
+  if ((reinterpret_cast<uintptr_t>(d_text) & 15u) != 0) {
+    msg = "text must be 16-byte aligned in device memory";
+    return CSM_E_ARG;
+  }
+  const uint64_t ntiles = (len + PARSE_TILE - 1) / PARSE_TILE;
+  if (ntiles > 0x7FFFFFFFull) { msg = "text too large"; return CSM_E_CAPACITY; }
+  std::vector<void*> held;
+  auto take = [&](size_t bytes) -> void* {
+    void* p = alloc(std::max<size_t>(bytes, 256));
+    if (p) held.push_back(p);
+    return p;
+  };
+  auto done = [&](int rc) {
+    (void)hipStreamSynchronize(st);
+    for (void* p : held) release(p);
+    return rc;
+  };
+#define PARSE_CHECK(x)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) { msg = hipGetErrorString(e_); return done(CSM_E_DEVICE); } \
+  } while (0)
+  auto* tcount = static_cast<uint32_t*>(take(ntiles * 4));
+  auto* toff = static_cast<uint32_t*>(take(ntiles * 4));
+  if (!tcount || !toff) { msg = "device allocation failed"; return done(CSM_E_DEVICE); }
+  k_parse_count<<<(uint32_t)ntiles, PARSE_BS, 0, st>>>(d_text, len, tcount);
+  size_t need = 0;
+  PARSE_CHECK(rocprim::exclusive_scan(nullptr, need, tcount, toff, 0u, (size_t)ntiles,
+                                      rocprim::plus<uint32_t>(), st));
+  void* tmp = take(need);
+  if (!tmp) { msg = "device allocation failed"; return done(CSM_E_DEVICE); }
+  PARSE_CHECK(rocprim::exclusive_scan(tmp, need, tcount, toff, 0u, (size_t)ntiles,
+                                      rocprim::plus<uint32_t>(), st));
+  uint32_t last_off = 0, last_cnt = 0;
+  uint8_t last_byte = 0;
+  PARSE_CHECK(hipMemcpyAsync(&last_off, toff + ntiles - 1, 4, hipMemcpyDeviceToHost, st));
+  PARSE_CHECK(hipMemcpyAsync(&last_cnt, tcount + ntiles - 1, 4, hipMemcpyDeviceToHost, st));
+  PARSE_CHECK(hipMemcpyAsync(&last_byte, d_text + len - 1, 1, hipMemcpyDeviceToHost, st));
+  PARSE_CHECK(hipStreamSynchronize(st));
+  const uint64_t n_nl = (uint64_t)last_off + last_cnt;
+  const uint64_t n_lines = n_nl + (last_byte != '\n' ? 1 : 0);
+  auto* lend = static_cast<uint64_t*>(take(std::max<uint64_t>(n_nl, 1) * 8));
+  auto* rec = static_cast<UpdateRec*>(take(n_lines * sizeof(UpdateRec)));
+  auto* rec2 = static_cast<UpdateRec*>(take(n_lines * sizeof(UpdateRec)));
+  auto* keep = static_cast<uint8_t*>(take(n_lines));
+  auto* flags = static_cast<uint64_t*>(take(16));   // [0] error bit, [1] first bad line
+  auto* count = static_cast<uint64_t*>(take(8));
+  if (!lend || !rec || !rec2 || !keep || !flags || !count) {
+    msg = "device allocation failed";
+    return done(CSM_E_DEVICE);
+  }
+  const uint64_t init[2] = {0, ~0ull};
+  PARSE_CHECK(hipMemcpyAsync(flags, init, 16, hipMemcpyHostToDevice, st));
+  k_parse_lines<<<(uint32_t)ntiles, PARSE_BS, 0, st>>>(d_text, len, toff, lend);
+  k_parse_records<<<grid_for(n_lines, 256, 65536), 256, 0, st>>>(
+      d_text, len, lend, n_nl, n_lines, mode, rec, keep, reinterpret_cast<uint32_t*>(flags),
+      reinterpret_cast<unsigned long long*>(flags + 1));
+  size_t need2 = 0;
+  PARSE_CHECK(rocprim::select(nullptr, need2, rec, keep, rec2, count, (size_t)n_lines, st));
+  void* tmp2 = take(need2);
+  if (!tmp2) { msg = "device allocation failed"; return done(CSM_E_DEVICE); }
+  PARSE_CHECK(rocprim::select(tmp2, need2, rec, keep, rec2, count, (size_t)n_lines, st));
+  uint64_t h_flags[2] = {0, 0}, h_count = 0;
+  PARSE_CHECK(hipMemcpyAsync(h_flags, flags, 16, hipMemcpyDeviceToHost, st));
+  PARSE_CHECK(hipMemcpyAsync(&h_count, count, 8, hipMemcpyDeviceToHost, st));
+  PARSE_CHECK(hipStreamSynchronize(st));
+  if (h_flags[0] & 1u) {
+    if (err_line) *err_line = h_flags[1] + 1;
+    char b[96];
+    snprintf(b, sizeof b, "malformed line %llu", (unsigned long long)(h_flags[1] + 1));
+    msg = b;
+    return done(CSM_E_ARG);
+  }
+  *n = h_count;
+  const int rc = deliver(rec2, (size_t)h_count);
+  if (rc == CSM_E_CAPACITY) msg = "output capacity too small";
+  else if (rc) msg = "copying the records out failed";
+#undef PARSE_CHECK
+  return done(rc);
+}
+
 }  // namespace
 
 // ================================================================== C-ABI
@@ -1720,6 +1816,59 @@ int csm_copy_to_host(csm_engine* e, void* dst, const void* src, size_t bytes) {
   HIP_TRY(e, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
   return CSM_OK;
+}
+
+int csm_parse_text_device(csm_engine* e, int mode, const char* d_text, size_t len,
+                          csm_update* d_out, size_t cap, size_t* n, uint64_t* err_line) {
+  if (!e || !n || (len && !d_text)) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  std::string msg;
+  int rc = parse_text_on(
+      e->stream, mode, reinterpret_cast<const uint8_t*>(d_text), len, n, err_line,
+      [&](size_t b) -> void* { uint8_t* p = nullptr; return dalloc(e, &p, b) ? nullptr : p; },
+      [&](void* p) { uint8_t* q = static_cast<uint8_t*>(p); dfree(e, q); },
+      [&](const UpdateRec* recs, size_t cnt) -> int {
+        if (cnt > cap) return CSM_E_CAPACITY;
+        if (cnt && hipMemcpyAsync(d_out, recs, cnt * sizeof(UpdateRec), hipMemcpyDeviceToDevice,
+                                  e->stream) != hipSuccess)
+          return CSM_E_DEVICE;
+        return CSM_OK;
+      },
+      msg);
+  if (rc) return fail(e, rc, "%s", msg.c_str());
+  return CSM_OK;
+}
+
+int csm_parse_text(int device, int mode, const char* text, size_t len, csm_update* out,
+                   size_t cap, size_t* n, uint64_t* err_line) {
+  if (!n || (len && !text)) return CSM_E_ARG;
+  *n = 0;
+  if (hipSetDevice(device) != hipSuccess) return CSM_E_DEVICE;
+  hipStream_t st = nullptr;
+  if (hipStreamCreate(&st) != hipSuccess) return CSM_E_DEVICE;
+  auto alloc = [](size_t b) -> void* { void* p = nullptr; return hipMalloc(&p, b) == hipSuccess ? p : nullptr; };
+  auto release = [](void* p) { (void)hipFree(p); };
+  int rc = CSM_OK;
+  uint8_t* d_text = static_cast<uint8_t*>(len ? alloc(len) : nullptr);
+  if (len && !d_text) rc = CSM_E_DEVICE;
+  if (!rc && len && hipMemcpyAsync(d_text, text, len, hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = CSM_E_DEVICE;
+  std::string msg;
+  if (!rc)
+    rc = parse_text_on(st, mode, d_text, len, n, err_line, alloc, release,
+                       [&](const UpdateRec* recs, size_t cnt) -> int {
+                         if (cnt > cap) return CSM_E_CAPACITY;
+                         if (cnt && out &&
+                             (hipMemcpyAsync(out, recs, cnt * sizeof(UpdateRec),
+                                             hipMemcpyDeviceToHost, st) != hipSuccess ||
+                              hipStreamSynchronize(st) != hipSuccess))
+                           return CSM_E_DEVICE;
+                         return CSM_OK;
+                       },
+                       msg);
+  if (d_text) release(d_text);
+  (void)hipStreamDestroy(st);
+  return rc;
 }
 
 int csm_copy_device(csm_engine* e, void* dst, const void* src, size_t bytes) {

@@ -100,6 +100,10 @@ def lib():
             "csm_copy_to_host": (ctypes.c_int, [vp, vp, vp, sz]),
             "csm_copy_device": (ctypes.c_int, [vp, vp, vp, sz]),
             "csm_stream": (vp, [vp]),
+            "csm_parse_text": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, vp, sz, vp, sz, P(sz),
+                                              P(u64)]),
+            "csm_parse_text_device": (ctypes.c_int, [vp, ctypes.c_int, vp, sz, vp, sz, P(sz),
+                                                     P(u64)]),
             "csm_last_error": (ctypes.c_char_p, [vp]),
             "csm_destroy": (None, [vp]),
         }
@@ -118,8 +122,8 @@ def exported_symbols() -> Sequence[str]:
             "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_totals", "csm_memory",
             "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
             "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
-            "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_last_error",
-            "csm_destroy")
+            "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_parse_text",
+            "csm_parse_text_device", "csm_last_error", "csm_destroy")
 
 
 def _u32(a) -> np.ndarray:
@@ -315,6 +319,18 @@ class Matching:
     def sync(self) -> None:
         self._check(lib().csm_sync(self.h))
 
+    def parse_text_device(self, d_text_ptr: int, nbytes: int, mode: int, d_out_ptr: int,
+                          cap: int) -> int:
+        """Text in HBM to device records (see csm_parse_text_device); returns
+        the record count."""
+        n, bad = ctypes.c_size_t(0), ctypes.c_uint64(0)
+        rc = lib().csm_parse_text_device(self.h, mode, d_text_ptr, nbytes, d_out_ptr, cap,
+                                         ctypes.byref(n), ctypes.byref(bad))
+        if rc == -1 and bad.value:
+            raise ParseError(rc, int(bad.value))
+        self._check(rc)
+        return int(n.value)
+
 
 class DeviceBuffer:
     """HBM buffer allocated through the engine (freed with the engine or close())."""
@@ -365,6 +381,49 @@ class HostBuffer:
         if self.ptr and self.m.h:
             lib().csm_host_free(self.m.h, self.ptr)
         self.ptr = 0
+
+
+PARSE_GRAPH, PARSE_STREAM = 0, 1
+
+
+class ParseError(CsmError):
+    """A malformed line in a graph or stream text (1-based `line`)."""
+
+    def __init__(self, code: int, line: int):
+        super().__init__(code, f"malformed line {line}")
+        self.line = line
+
+
+def parse_text(text: bytes, mode: int = PARSE_STREAM, device: int = 0) -> np.ndarray:
+    """Parse graph (PARSE_GRAPH) or stream (PARSE_STREAM) text on the device
+    into csm_update records (see csm_parse_text)."""
+    data = bytes(text)
+    cap = data.count(b"\n") + 1
+    out = np.zeros(cap, dtype=UPDATE_DTYPE)
+    n, bad = ctypes.c_size_t(0), ctypes.c_uint64(0)
+    rc = lib().csm_parse_text(device, mode, data if data else None, len(data), out.ctypes.data,
+                              cap, ctypes.byref(n), ctypes.byref(bad))
+    if rc == -1 and bad.value:
+        raise ParseError(rc, int(bad.value))
+    if rc:
+        raise CsmError(rc, "csm_parse_text failed")
+    return out[: n.value]
+
+
+def records_to_graph(rec: np.ndarray):
+    """Graph-file records to (vlabels, src, dst, elabels) as Graph::AddVertex /
+    AddEdge would keep them (graph.cpp:23-93): a vertex keeps its first label,
+    vertices never declared carry 0xFFFFFFFF."""
+    v = rec[rec["type"] == ord("v")]
+    e = rec[rec["type"] == ord("e")]
+    nv = 0
+    if len(v):
+        nv = max(nv, int(v["id1"].max()) + 1)
+    if len(e):
+        nv = max(nv, int(max(e["id1"].max(), e["id2"].max())) + 1)
+    vl = np.full(nv, 0xFFFFFFFF, np.uint32)
+    vl[v["id1"][::-1]] = v["label"][::-1]   # the first occurrence is written last
+    return vl, e["id1"].copy(), e["id2"].copy(), e["label"].copy()
 
 
 def run_stream(m: Matching, add, v1, v2, label, batch: Optional[int] = None):
