@@ -890,6 +890,9 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t 
 #define PROF_ADD(i, x) (void)0
 #endif
 
+#ifndef CSM_CHUNK_GRAB
+#define CSM_CHUNK_GRAB 4
+#endif
 #ifndef CSM_SEARCH_WAVES_PER_EU
 #define CSM_SEARCH_WAVES_PER_EU 5
 #endif
@@ -917,13 +920,24 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
   const uint64_t prof_kt0 = __builtin_amdgcn_s_memrealtime();
 #endif
 
+  // a wave takes up to CSM_CHUNK_GRAB consecutive chunk ids per atomic: they
+  // mostly belong to one task, so its cached segments and L1 lines are
+  // reused; rounds with few chunks per wave take fewer, to keep the tail short
+  uint32_t c_next = 0, c_end = 0;
+  const uint32_t nwaves = gridDim.x * (uint32_t)search_waves<MK>();
   while (true) {
     PROF_BEGIN(fetch);
-    uint32_t c = 0;
-    if (lane == 0) c = atomicAdd(&a.ctrl->chunk_head, 1u);
-    c = uniform(__shfl(c, 0));
     uint32_t nch = uniform(__atomic_load_n(&a.ctrl->nchunks, __ATOMIC_RELAXED));
     nch = nch < a.chunk_cap ? nch : a.chunk_cap;
+    if (c_next >= c_end) {
+      uint32_t grab = CSM_CHUNK_GRAB;
+      while (grab > 1 && (uint64_t)nch < 8ull * grab * nwaves) grab >>= 1;
+      uint32_t c0 = 0;
+      if (lane == 0) c0 = atomicAdd(&a.ctrl->chunk_head, grab);
+      c_next = uniform(__shfl(c0, 0));
+      c_end = c_next + grab;
+    }
+    const uint32_t c = c_next++;
     if (c >= nch) break;
     if (__atomic_load_n(&a.ctrl->abort, __ATOMIC_RELAXED)) break;
     const uint32_t sidx = uniform(a.chunk_owner[c]);
