@@ -75,7 +75,12 @@ typedef struct csm_opts {
   uint32_t spill_budget;    /* DFS loop iterations before a search task
                                hands the rest of its subtree to the next
                                round (0 = default)                         */
-  uint32_t pad0;
+  uint32_t reference_orders; /* 1: search in the GraphFlow plugin's matching
+                               orders (graphflow.cpp:36-160); 0 (default):
+                               the same seeds with positions 2..k-1 chosen by
+                               estimated candidate counts from the data
+                               graph's label statistics.  Counts are the same
+                               either way; only the search effort differs.  */
 } csm_opts;
 
 /* Creates an engine and uploads the data graph.  `algo` is one of the

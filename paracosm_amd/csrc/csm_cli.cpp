@@ -6,7 +6,7 @@
 //       [--max-results N] [--time-limit S] [--print-prep 0|1] [--print-enum 0|1]
 //       [--homo 0|1] [--report-initial 0|1] [--initial-time-limit S]
 //       [-t N] [--auto-tuning 0|1]
-//       [--compat 0|1] [--batch N] [--device N]
+//       [--compat 0|1] [--batch N] [--device N] [--reference-orders 0|1]
 //
 // Default mode applies every update of the stream, in order, batched on the
 // device (--batch, default 65534), insertions and deletions alike, and
@@ -157,7 +157,7 @@ int main(int argc, char** argv) {
   uint64_t max_results = 0;
   double time_limit = 0.0, initial_time_limit = 0.0;
   int print_prep = 1, print_enum = 0, homo = 0, report_initial = 1, compat = 0;
-  uint32_t threads = 6, auto_tuning = 1, batch = 0, device = 0;
+  uint32_t threads = 6, auto_tuning = 1, batch = 0, device = 0, reference_orders = 0;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto next = [&]() -> std::string {
@@ -176,6 +176,7 @@ int main(int argc, char** argv) {
     else if (a == "--report-initial") report_initial = std::stoi(next());
     else if (a == "--initial-time-limit") initial_time_limit = std::stod(next());
     else if (a == "--orders") next();  // accepted; orders are planned on the host
+    else if (a == "--reference-orders") reference_orders = (uint32_t)std::stoul(next());
     else if (a == "-t" || a == "--thread-num") threads = (uint32_t)std::stoul(next());
     else if (a == "--auto-tuning") auto_tuning = (uint32_t)std::stoul(next());
     else if (a == "--compat") compat = std::stoi(next());
@@ -211,6 +212,7 @@ int main(int argc, char** argv) {
   o.max_batch = batch;
   o.time_limit_s = time_limit;
   o.device = device;
+  o.reference_orders = reference_orders;
   csm_graph cq = as_graph(q), cd = as_graph(d);
   int err = 0;
   csm_engine* e = csm_create(algorithm.c_str(), &cq, &cd, &o, &err);

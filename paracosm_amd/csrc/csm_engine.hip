@@ -282,7 +282,24 @@ int plan_query(csm_engine* e, QueryInfo& qi, uint32_t qid) {
     uint32_t d = dense_of(e->vl_map, g.vl[u]);
     qi.dlab[u] = d;
   }
-  qi.orders = graphflow_orders(g);
+  if (!e->opts.reference_orders) {
+    auto el_of = [&](uint32_t a, uint32_t b) { return dense_of(e->el_map, g.elabel(a, b)); };
+    auto triple = [&](uint32_t a, uint32_t b) -> double {
+      uint32_t l = el_of(a, b);
+      if (qi.dlab[a] == NO_LABEL || qi.dlab[b] == NO_LABEL || l == NO_LABEL) return 0.0;
+      return (double)e->h_triple_freq[(qi.dlab[a] * e->nlab + qi.dlab[b]) * e->nel + l];
+    };
+    auto vf = [&](uint32_t a) -> double {
+      return qi.dlab[a] == NO_LABEL ? 0.0 : (double)e->h_vfreq[qi.dlab[a]];
+    };
+    // fanout(w, u): directed (L(w) -> L(u), l) entries per vertex of label L(w)
+    auto fanout = [&](uint32_t w, uint32_t u) { double n = vf(w); return n > 0 ? triple(w, u) / n : 0.0; };
+    // hit(w, u): chance that a candidate of u is adjacent to w's match
+    auto hit = [&](uint32_t w, uint32_t u) { double n = vf(u); return n > 0 ? fanout(w, u) / n : 0.0; };
+    qi.orders = costed_orders(g, fanout, hit);
+  } else {
+    qi.orders = graphflow_orders(g);
+  }
   qi.plan_base = (uint32_t)e->h_plans.size();
   for (size_t oi = 0; oi < qi.orders.size(); ++oi) {
     const auto& o = qi.orders[oi];

@@ -115,6 +115,54 @@ inline std::vector<std::vector<uint32_t>> graphflow_orders(const QueryGraph& q) 
   return orders;
 }
 
+// The same orders with positions 2..k-1 re-chosen by estimated cost (the
+// first two positions, which make every query edge the seed of exactly one
+// order, are kept).  `fanout(w, u)` estimates how many candidates for u one
+// data vertex matched to w yields (neighbours with u's label over the query
+// edge's label); `hit(w, u)` the probability that a given candidate of u is
+// adjacent to w's match.  The next vertex minimizes fanout from its best
+// placed neighbour times the hit rates of its other placed neighbours; ties
+// go to more placed neighbours, then the higher degree, then the lower id.
+// Any order enumerates the same embeddings: only the search effort changes.
+template <class Fanout, class Hit>
+std::vector<std::vector<uint32_t>> costed_orders(const QueryGraph& q, Fanout fanout, Hit hit) {
+  std::vector<std::vector<uint32_t>> orders = graphflow_orders(q);
+  const uint32_t k = q.k;
+  for (auto& o : orders) {
+    if (k <= 3) continue;
+    std::vector<uint8_t> placed(k, 0);
+    placed[o[0]] = placed[o[1]] = 1;
+    for (uint32_t p = 2; p < k; ++p) {
+      uint32_t best = UINT32_MAX, best_adj = 0;
+      double best_cost = 0.0;
+      for (uint32_t u = 0; u < k; ++u) {
+        if (placed[u]) continue;
+        uint32_t adj = 0, wbest = UINT32_MAX;
+        double f = 1e300;
+        for (uint32_t w : q.nb[u]) {
+          if (!placed[w]) continue;
+          ++adj;
+          const double fw = fanout(w, u);
+          if (fw < f) { f = fw; wbest = w; }
+        }
+        if (adj == 0) continue;
+        // selectivity of the placed neighbours other than the cheapest one
+        double h = 1.0;
+        for (uint32_t w : q.nb[u])
+          if (placed[w] && w != wbest) h *= hit(w, u);
+        const double cost = f * h;
+        bool better = best == UINT32_MAX || cost < best_cost ||
+                      (cost == best_cost && (adj > best_adj ||
+                                             (adj == best_adj && q.deg(u) > q.deg(best))));
+        if (better) { best = u; best_adj = adj; best_cost = cost; }
+      }
+      o[p] = best;
+      placed[best] = 1;
+    }
+  }
+  return orders;
+}
+
 // SymBi's DAG: BFS from the root chosen by edge frequency (the rarest query
 // edge's less frequent endpoint; ParaCOSM/CSM/matching/SymBi/symbi.cpp:113-244).
 // `edge_freq(u, u2)` counts directed data edges whose labels match (u -> u2).

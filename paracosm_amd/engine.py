@@ -54,7 +54,7 @@ class _Opts(ctypes.Structure):
                 ("time_limit_s", ctypes.c_double), ("device_budget", ctypes.c_uint64),
                 ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("match_budget", ctypes.c_uint64), ("spill_budget", ctypes.c_uint32),
-                ("pad0", ctypes.c_uint32)]
+                ("reference_orders", ctypes.c_uint32)]
 
 
 UPDATE_DTYPE = np.dtype([("type", np.uint8), ("is_add", np.uint8), ("pad", np.uint16),
@@ -161,13 +161,13 @@ class Matching:
                  homo: bool = False, threads: int = 0, auto_tuning: int = 1,
                  max_batch: int = 0, want_hash: bool = False, device_budget: int = 0,
                  split_chunk: int = 0, device: int = 0, match_budget: int = 0,
-                 spill_budget: int = 0):
+                 spill_budget: int = 0, reference_orders: bool = False):
         L = lib()
         self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         d = _GraphArgs(vlabels, src, dst, elabels)
         o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
                   auto_tuning, max_batch, int(want_hash), 0.0, device_budget, split_chunk,
-                  device, match_budget, spill_budget, 0)
+                  device, match_budget, spill_budget, int(reference_orders))
         err = ctypes.c_int(0)
         self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
                               ctypes.byref(o), ctypes.byref(err))

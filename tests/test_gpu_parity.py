@@ -435,3 +435,21 @@ def test_fine_chunks_exact(oracle_lib, chunk, algo):
         np.testing.assert_array_equal(pos[:, 0], ref.pos)
         np.testing.assert_array_equal(neg[:, 0], ref.neg)
         np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi", "parallel_turboflux"])
+def test_reference_orders_agree(oracle_lib, algo):
+    """The GraphFlow plugin's own matching orders and the default
+    estimated-cost orders enumerate the same embeddings (counts and hashes)."""
+    from paracosm_amd.engine import make_updates
+    for seed in (2, 5, 9):
+        vl, init, ups, q = _case(1400 + seed, nv=50, p=0.3, n_vl=3, n_upd=120, k=5 + seed % 3,
+                                 extra=seed % 2)
+        ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+        for reference_orders in (False, True):
+            with _engine(algo, q, vl, init, want_hash=True,
+                         reference_orders=reference_orders) as m:
+                pos, neg, _, hsh = m.apply_batch(make_updates(*ups))
+            np.testing.assert_array_equal(pos[:, 0], ref.pos)
+            np.testing.assert_array_equal(neg[:, 0], ref.neg)
+            np.testing.assert_array_equal(hsh[:, 0], ref.hash)
