@@ -65,7 +65,10 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
+    # warm-up batches also let the engine's autotuner score its candidate
+    # search settings (a settling batch, then one batch searched once per
+    # candidate)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=62500)
     ap.add_argument("--algo", default="parallel_graphflow")
     ap.add_argument("--config", default="c2")
@@ -77,6 +80,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--split-chunk", type=int, default=0)
+    ap.add_argument("--spill-budget", type=int, default=0)
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (HBM counter calibration)")
@@ -129,7 +133,8 @@ def main() -> int:
                        w.upd_label[:n_total])
     nq = len(queries)
     m = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
-                 max_batch=args.batch, split_chunk=args.split_chunk)
+                 max_batch=args.batch, split_chunk=args.split_chunk,
+                 spill_budget=args.spill_budget)
     for q in queries[1:]:
         m.register_query(q)
     m.Preprocessing()

@@ -33,6 +33,7 @@ namespace {
 enum Algo { ALGO_GRAPHFLOW = 0, ALGO_SYMBI = 1, ALGO_TURBOFLUX = 2, ALGO_NONE = 3 };
 
 constexpr uint32_t DEFAULT_MAX_BATCH = 65534;
+constexpr size_t TUNE_MIN_BATCH = 4096;   // smallest batch the autotuner scores
 constexpr int BUILD_BS = 256;
 constexpr int APPLY_BS = 256;
 
@@ -112,6 +113,13 @@ struct csm_engine {
   uint32_t* d_spill_owner[2] = {nullptr, nullptr};
   uint32_t spill_cap = 0, spill_chunk_cap = 0;
   uint32_t budget_iters = 1024;
+  // online autotuner of (split_chunk, spill budget) (the reference's
+  // --auto-tuning): after one settling batch, the next large batch runs its
+  // search once per candidate; the least search time is kept
+  bool tune_active = false;
+  uint32_t tune_idx = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> tune_cands;
+  std::vector<double> tune_cost;
   uint32_t max_rounds = 0;                        // stats: deepest round count seen
   unsigned long long* d_counts = nullptr;
   unsigned long long* d_hashes = nullptr;
@@ -815,6 +823,7 @@ int symbi_add_vertex(csm_engine* e, uint32_t id);
 // d_type/d_add/d_v1/d_v2/d_lab.  With search == false the graph and the SymBi
 // index are updated but no match is searched (vertex deletion).
 int dir_reserve(csm_engine* e, uint32_t n);
+void tune_finish(csm_engine* e);
 
 int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
               uint8_t* safe, bool search_matches = true) {
@@ -871,8 +880,33 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
     if ((rc = symbi_batch_insert(e, n))) return rc;
   // 5. seeds + search
   if (search_matches) {
-    seeds(e, n, counts, hashes, safe, true);
-    if ((rc = search(e, true, counts, hashes))) return rc;
+    if (e->tune_active && n >= TUNE_MIN_BATCH && e->tune_idx > 0) {
+      // autotuning: the search alone runs once per candidate setting on this
+      // same batch (identical work and identical counts each time)
+      const size_t nq = e->queries.size();
+      for (size_t i = 0; i < e->tune_cands.size(); ++i) {
+        if (i) {
+          k_search_reset<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+          HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
+          if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
+          if (e->opts.match_budget) {
+            HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
+            HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
+          }
+        }
+        e->opts.split_chunk = e->tune_cands[i].first;
+        e->budget_iters = e->tune_cands[i].second;
+        const uint64_t ns0 = e->stats[5];
+        seeds(e, n, counts, hashes, safe, true);
+        if ((rc = search(e, true, counts, hashes))) return rc;
+        e->tune_cost.push_back((double)(e->stats[5] - ns0));
+      }
+      tune_finish(e);
+    } else {
+      seeds(e, n, counts, hashes, safe, true);
+      if ((rc = search(e, true, counts, hashes))) return rc;
+      if (e->tune_active && n >= TUNE_MIN_BATCH) ++e->tune_idx;   // the settling batch
+    }
   }
   // 6. ... and deletions (after it)
   if (uses_index(e) && e->d_d2)
@@ -1335,6 +1369,24 @@ int parse_text_on(hipStream_t st, int mode, const uint8_t* d_text, uint64_t len,
   return done(rc);
 }
 
+// After the tuning batch: keep the candidate with the least search time for
+// the engine's lifetime.
+void tune_finish(csm_engine* e) {
+  size_t best = 0;
+  for (size_t i = 1; i < e->tune_cost.size(); ++i)
+    if (e->tune_cost[i] < e->tune_cost[best]) best = i;
+  e->opts.split_chunk = e->tune_cands[best].first;
+  e->budget_iters = e->tune_cands[best].second;
+  e->tune_active = false;
+  if (getenv("CSM_TRACE_ROUNDS")) {
+    fprintf(stderr, "[csm] autotune:");
+    for (size_t i = 0; i < e->tune_cost.size(); ++i)
+      fprintf(stderr, " (%u,%u)=%.2fms", e->tune_cands[i].first, e->tune_cands[i].second,
+              e->tune_cost[i] * 1e-6);
+    fprintf(stderr, " -> split %u, budget %u\n", e->opts.split_chunk, e->budget_iters);
+  }
+}
+
 }  // namespace
 
 // ================================================================== C-ABI
@@ -1351,6 +1403,9 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   e->algo_name = algo;
   if (opts) e->opts = *opts;
   if (e->opts.max_batch == 0 || e->opts.max_batch > DEFAULT_MAX_BATCH) e->opts.max_batch = DEFAULT_MAX_BATCH;
+  e->tune_active = e->opts.auto_tuning != 0 && e->opts.split_chunk == 0 && e->opts.spill_budget == 0;
+  if (e->tune_active)
+    e->tune_cands = {{64, 1024}, {32, 1024}, {64, 512}, {64, 2048}, {128, 1024}};
   if (e->opts.split_chunk == 0) e->opts.split_chunk = 64;
   e->max_batch = e->opts.max_batch;
   if (e->opts.spill_budget) e->budget_iters = e->opts.spill_budget;
@@ -1716,6 +1771,9 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes) {
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
   if (!e || !out) return CSM_E_ARG;
   for (size_t i = 0; i < n_out && i < 8; ++i) out[i] = e->stats[i];
+  if (n_out > 8) out[8] = e->opts.split_chunk;
+  if (n_out > 9) out[9] = e->budget_iters;
+  if (n_out > 10) out[10] = e->tune_active ? 1 : 0;
   return CSM_OK;
 }
 

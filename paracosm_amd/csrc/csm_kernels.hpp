@@ -1574,6 +1574,18 @@ __global__ void k_batch_begin(Ctrl* c) {
   }
 }
 
+// forget the seeds and spills of a search so that it can run again on the
+// same batch (autotuning); the graph-update state of the batch is kept
+__global__ void k_search_reset(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    c->nseeds = 0;
+    c->nchunks = 0;
+    c->chunk_head = 0;
+    c->nspill = 0;
+    c->nspill_chunks = 0;
+  }
+}
+
 // the spilled tasks of the last round become the input of the next
 __global__ void k_next_round(Ctrl* c) {
 #ifdef CSM_PHASE_PROF

@@ -267,11 +267,12 @@ class Matching:
         return int(a.value), int(b.value), int(c.value)
 
     def stats(self):
-        out = (ctypes.c_uint64 * 8)()
-        self._check(lib().csm_stats(self.h, out, 8))
+        out = (ctypes.c_uint64 * 11)()
+        self._check(lib().csm_stats(self.h, out, 11))
         return {"tasks": out[0], "candidates": out[1], "alg_bytes": out[2], "batches": out[3],
                 "repacks": out[4], "search_ns": out[5], "search_launches": out[6],
-                "max_rounds": out[7]}
+                "max_rounds": out[7], "split_chunk": out[8], "spill_budget": out[9],
+                "tuning": out[10]}
 
     def device_bytes(self) -> int:
         b = ctypes.c_uint64(0)
