@@ -82,6 +82,8 @@ def main() -> int:
     ap.add_argument("--split-chunk", type=int, default=0)
     ap.add_argument("--spill-budget", type=int, default=0)
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--also", default="parallel_symbi,parallel_turboflux",
+                    help="other algorithms timed on the same batches (comma list, '' for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -211,6 +213,15 @@ def main() -> int:
     ingest = None if args.no_ingest else ingest_rate(m, w)
     m.close()
 
+    # the other plugins on the same batches: rate, and per-update, per-query
+    # counts equal to this run's
+    others = {}
+    for algo in [a for a in args.also.split(",") if a and a != args.algo]:
+        v, c = batched_only(algo, w, queries, ups, args)
+        same = bool(np.array_equal(c, counts))
+        others[algo] = {"updates_per_s": round(v, 1), "counts_equal": same}
+        log(f"{algo}: {v:.0f} updates/s, counts equal to {args.algo}: {same}")
+
     # roofline of the search kernel: algorithmic bytes per launch over its
     # average launch duration (HIP events on the engine stream)
     launches = st1["search_launches"] - st0["search_launches"]
@@ -261,11 +272,48 @@ def main() -> int:
         "matches": matches,
         "exact_counts_plain_vs_batched": exact,
         "roofline": roof,
+        "other_algorithms": others,
         "ingest": ingest,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def batched_only(algo, w, queries, ups, args):
+    """Batched rate of another plugin on the same warm-up and timed batches;
+    returns (updates/s, counts of every batch)."""
+    from paracosm_amd.engine import Matching
+    nq = len(queries)
+    n_total = len(ups)
+    m = Matching(algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                 max_batch=args.batch, split_chunk=args.split_chunk,
+                 spill_budget=args.spill_budget)
+    for q in queries[1:]:
+        m.register_query(q)
+    m.Preprocessing()
+    d_ups = m.device_buffer(n_total * 16)
+    d_ups.upload(ups.view(np.uint8))
+    d_counts = m.device_buffer(n_total * nq * 8)
+
+    def step(i):
+        s = i * args.batch
+        n = min(args.batch, n_total - s)
+        m.apply_batch_device(d_ups.ptr + s * 16, n, d_counts.ptr + s * nq * 8)
+        return n
+
+    for i in range(args.warmup):
+        step(i)
+    m.sync()
+    t = time.perf_counter()
+    done = 0
+    for i in range(args.warmup, args.warmup + args.steps):
+        done += step(i)
+    m.sync()
+    elapsed = time.perf_counter() - t
+    counts = d_counts.download(np.zeros(n_total * nq, np.uint64)).reshape(n_total, nq)
+    m.close()
+    return done / elapsed, counts
 
 
 def ingest_rate(m, w, reps=5):
