@@ -28,8 +28,9 @@ _RMAT_CHUNK = 1 << 22  # draws per chunk; part of the generator's definition
 
 
 def rmat_edges(scale: int, edge_factor: int, a: float = 0.57, b: float = 0.19,
-               c: float = 0.19, seed: int = 0):
-    """Unique undirected R-MAT edges as (lo, hi) uint32 arrays, sorted by (lo, hi)."""
+               c: float = 0.19, seed: int = 0, progress=None):
+    """Unique undirected R-MAT edges as (lo, hi) uint32 arrays, sorted by (lo, hi).
+    progress(msg), if given, is called after each chunk of draws."""
     n_draws = edge_factor << scale
     rng = np.random.default_rng(np.random.SeedSequence([seed, 0x524D4154]))
     keys = []
@@ -49,6 +50,8 @@ def rmat_edges(scale: int, edge_factor: int, a: float = 0.57, b: float = 0.19,
         hi = np.maximum(u, v)
         keep = lo != hi
         keys.append((lo[keep] << np.uint64(32)) | hi[keep])
+        if progress:
+            progress(f"R-MAT draws {start + n}/{n_draws}")
     key = np.unique(np.concatenate(keys))
     lo = (key >> np.uint64(32)).astype(np.uint32)
     hi = (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
@@ -248,8 +251,10 @@ def make_workload(scale: int, edge_factor: int, n_vlabels: int, vlabel_dist: str
                   n_updates: int = 0, p_insert: float = 1.0,
                   query_sizes: Sequence[int] = (6,), queries_per_size: int = 5,
                   densities: Sequence[Optional[str]] = (None,), seed: int = 0,
-                  zipf_s: float = 1.0, accept=None, name: str = "") -> Workload:
-    lo, hi = rmat_edges(scale, edge_factor, seed=seed)
+                  zipf_s: float = 1.0, accept=None, name: str = "", progress=None) -> Workload:
+    lo, hi = rmat_edges(scale, edge_factor, seed=seed, progress=progress)
+    if progress:
+        progress(f"{lo.shape[0]} unique edges")
     nv = 1 << scale
     vl = vertex_labels(nv, n_vlabels, vlabel_dist, zipf_s, seed)
     el = edge_labels(lo.shape[0], n_elabels, seed)
@@ -265,9 +270,12 @@ def make_workload(scale: int, edge_factor: int, n_vlabels: int, vlabel_dist: str
     flip = rng.random(ssrc.shape[0]) < 0.5
     v1 = np.where(flip, sdst, ssrc).astype(np.uint32)
     v2 = np.where(flip, ssrc, sdst).astype(np.uint32)
-    off, nbr, ecsr = build_csr(nv, isrc, idst, iel)
+    if progress:
+        progress(f"stream of {ssrc.shape[0]} updates")
     queries: List[Query] = []
-    for k in query_sizes:
+    if queries_per_size and query_sizes:
+        off, nbr, ecsr = build_csr(nv, isrc, idst, iel)
+    for k in query_sizes if queries_per_size else ():
         for di, dens in enumerate(densities):
             queries += sample_queries(off, nbr, ecsr, vl, k, queries_per_size,
                                       seed * 1000003 + di, dens, accept)

@@ -69,7 +69,7 @@ class Data:
 
 
 def build(name: str, seed: int = 1, scale: Optional[int] = None,
-          n_updates: Optional[int] = None) -> Data:
+          n_updates: Optional[int] = None, progress=None) -> Data:
     cfg = CONFIGS[name]
     if scale is not None:
         cfg = dataclasses.replace(cfg, scale=scale)
@@ -78,8 +78,10 @@ def build(name: str, seed: int = 1, scale: Optional[int] = None,
     w = gen.make_workload(cfg.scale, cfg.edge_factor, cfg.n_vlabels, cfg.vlabel_dist,
                           cfg.n_elabels, stream=cfg.stream, frac_stream=cfg.frac_stream,
                           n_updates=cfg.n_updates, p_insert=cfg.p_insert, query_sizes=(),
-                          queries_per_size=0, seed=seed, name=name)
+                          queries_per_size=0, seed=seed, name=name, progress=progress)
     off, nbr, ecsr = gen.build_csr(w.nv, w.init_src, w.init_dst, w.init_elab)
+    if progress:
+        progress("initial CSR built")
     return Data(cfg, seed, w, off, nbr, ecsr)
 
 
