@@ -140,10 +140,25 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// first index i in [lo, hi) with a[i] >= x (hi if none)
+// first index i in [lo, hi) with a[i] >= x (hi if none).  The halvings of
+// the binary search are unchanged, but while at least 3 entries remain the
+// 3 pivots the next two halvings can visit are loaded together and the two
+// comparisons made afterwards: one memory round trip per two levels (the
+// loads are independent, so they overlap).
 __device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ a, uint32_t lo,
                                                 uint32_t hi, uint32_t x) {
   uint32_t n = hi - lo;
+  while (n >= 3) {
+    // [lo, lo + n1) | m | [m + 1, m + 1 + n2), both halves non-empty
+    const uint32_t n1 = n >> 1, n2 = n - n1 - 1, m = lo + n1;
+    const uint32_t vm = a[m], vA = a[lo + (n1 >> 1)], vB = a[m + 1 + (n2 >> 1)];
+    const bool c1 = vm < x;
+    const uint32_t l2 = c1 ? m + 1 : lo, k2 = c1 ? n2 : n1;
+    const bool c2 = (c1 ? vB : vA) < x;
+    const uint32_t h2 = k2 >> 1;
+    lo = c2 ? l2 + h2 + 1 : l2;
+    n = c2 ? k2 - h2 - 1 : h2;
+  }
   while (n > 0) {
     uint32_t half = n >> 1;
     uint32_t mid = lo + half;
@@ -890,6 +905,10 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t 
 #define PROF_ADD(i, x) (void)0
 #endif
 
+// the search's lambdas are always inlined: an outlined one is called with its
+// captures spilled to scratch memory
+#define CSM_AINL __attribute__((always_inline))
+
 #ifndef CSM_CHUNK_GRAB
 #define CSM_CHUNK_GRAB 4
 #endif
@@ -979,7 +998,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     const uint32_t L1 = k - 1, L2 = k - 2;   // last and second-to-last positions
 
     // publish the running count; stop if the query's budget is exhausted
-    auto flush = [&](bool final_) {
+    auto flush = [&](bool final_) CSM_AINL {
       unsigned long long d = cnt - flushed;
       if (lane == 0 && d) {
         atomicAdd(&a.counts[(size_t)t * a.nq + q], d);
@@ -995,7 +1014,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
 
     // test a candidate entry e of position lv's u_min segment (joins with the
     // other backward neighbours of lv)
-    auto test = [&](uint32_t lv, uint32_t e, uint32_t end, uint32_t& vout) -> bool {
+    auto test = [&](uint32_t lv, uint32_t e, uint32_t end, uint32_t& vout) CSM_AINL -> bool {
       bool ok = e < end;
       uint32_t v = 0;
       if (ok) {
@@ -1022,7 +1041,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
 
     // is w a valid last-position vertex w.r.t. the fixed neighbours (all but
     // `skip`), injectivity against m[0..ninj-1] and the d2 bit
-    auto last_fixed_ok = [&](uint32_t w, uint32_t skip, uint32_t ninj) -> bool {
+    auto last_fixed_ok = [&](uint32_t w, uint32_t skip, uint32_t ninj) CSM_AINL -> bool {
       bool ok = true;
       if (!a.homo)
         for (uint32_t i = 0; i < ninj; ++i) ok = ok && (S.m[i] != w);
@@ -1043,7 +1062,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     // F is built without the injectivity test against position k-3, which the
     // leaves make themselves, so it is reused across the entries of position
     // k-2 that share the fixed vertices and the prefix m[0..k-4].
-    auto prepare_last = [&]() -> bool {
+    auto prepare_last = [&]() CSM_AINL -> bool {
       PROF_BEGIN(pl);
       const uint32_t nb = uniform(S.pnb[L1]);
       uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu, is_par = 0, miss = 0;
@@ -1097,7 +1116,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       PROF_ADD(11, jf == NONE ? 0u : fsize);
       const uint32_t mk3 = S.m[L2 - 1];
       // C without the vertex at position k-3 (it is in F at most once)
-      auto c_eff = [&](uint32_t C) -> uint32_t {
+      auto c_eff = [&](uint32_t C) CSM_AINL -> uint32_t {
         if (a.homo || C == 0) return C;
         const uint32_t i = lds_lower_bound(S.F, C, mk3);
         return (i < C && S.F[i] == mk3) ? C - 1u : C;
@@ -1152,7 +1171,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     };
 
     // enter position lv: partner segments, u_min, candidate range
-    auto enter = [&](uint32_t lv, uint32_t rlo, uint32_t rhi, int forced_j) -> bool {
+    auto enter = [&](uint32_t lv, uint32_t rlo, uint32_t rhi, int forced_j) CSM_AINL -> bool {
       const uint32_t nb = uniform(S.pnb[lv]);
       uint32_t my_lo = 0, my_hi = 0, my_dirty = 0, my_size = 0xFFFFFFFFu;
       if (lane < nb) {
@@ -1204,7 +1223,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     // reserve n_t spill tasks and n_c chunk ids with one atomic each; when
     // they do not fit, the chunk ids below the cap are marked ownerless and
     // the caller keeps the work
-    auto reserve_spill = [&](uint32_t n_t, uint32_t n_c, uint32_t& tb, uint32_t& cb) -> bool {
+    auto reserve_spill = [&](uint32_t n_t, uint32_t n_c, uint32_t& tb, uint32_t& cb) CSM_AINL -> bool {
       tb = 0;
       cb = 0;
       if (lane == 0) {
@@ -1222,7 +1241,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     // write spill task ti: positions 0..depth-2 from m[], position depth-1 =
     // extra, candidate range [lo, hi) of backward neighbour jm in chunks from ci
     auto emit_task = [&](uint32_t& ti, uint32_t& ci, uint32_t depth, uint32_t jm, uint32_t lo,
-                         uint32_t hi, uint2 ch, uint32_t extra) {
+                         uint32_t hi, uint2 ch, uint32_t extra) CSM_AINL {
       const uint32_t nch = ch.x;
       Task* o = a.spill + ti;
       uint32_t word = 0;
@@ -1248,7 +1267,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
 
     // the leaves of parent v (candidate range [lo, hi) of the last position's
     // backward neighbour jm) as a task of the next round
-    auto spill_leaves = [&](uint32_t v, uint32_t jm, uint32_t lo, uint32_t hi) -> bool {
+    auto spill_leaves = [&](uint32_t v, uint32_t jm, uint32_t lo, uint32_t hi) CSM_AINL -> bool {
       const uint2 ch = chunking(hi - lo, a.chunk);
       uint32_t tb, cb;
       if (!reserve_spill(1u, ch.x, tb, cb)) return false;
@@ -1257,7 +1276,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     };
 
     // all leaves under the bn parents now in cand[k-2]
-    auto finish_parents = [&](uint32_t bn) {
+    auto finish_parents = [&](uint32_t bn) CSM_AINL {
       PROF_BEGIN(fp);
       const uint32_t jp = S.ljp, jf = S.ljf;
       const uint32_t nF = S.lF;          // F in LDS (NONE: not built)
@@ -1337,11 +1356,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       PROF_ADD(0, total);
       PROF_ADD(1, bn);
       PROF_ADD(2, popc64(ballot(lane < bn && src == 1)));
-      PROF_ADD(3, popc64(ballot(lane < bn && src == 0 && size > 0)));
-      PROF_ADD(4, jf == NONE ? 1u : 0u);
-      PROF_ADD(5, uniform(S.pnb[L1]));
+      PROF_ADD(useF ? 5 : 3, total);
       PROF_ADD(6, uniform(popc64(ballot(lane < bn)) ? (uint32_t)wave_sum((unsigned long long)(lane < bn ? (ohi - olo) : 0u)) : 0u));
-      PROF_ADD(7, S.lfsize);
       S.pref[lane] = incl - size;
       if (lane == 0) S.pref[WAVE] = total;
       wave_sync();
@@ -1404,12 +1420,15 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
         }
       }
       if (jp == NONE) n_bytes += 4ull * bn * S.lfsize;
+#ifdef CSM_PHASE_PROF
+      PROF_ADD(useF ? 7 : 4, __builtin_readcyclecounter() - prof_t0_fp);
+#endif
       PROF_END(fp, 6);
     };
 
     // spill the rest of this task's DFS as new tasks (one per remaining range
     // and one per buffered candidate); false if the spill buffers are full
-    auto spill = [&](uint32_t lv_now) -> bool {
+    auto spill = [&](uint32_t lv_now) CSM_AINL -> bool {
       uint32_t n_t = 0, n_c = 0;
       for (uint32_t L = d0; L <= lv_now; ++L) {
         uint32_t cu = S.cur[L], en = S.end[L];
