@@ -1469,14 +1469,19 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     PROF_BEGIN(e0);
     bool live = enter(d0, rlo, rhi, tj == NONE ? -1 : (int)tj);
     PROF_END(e0, 1);
+    // a task takes one DFS step before it may spill: entering position k-2
+    // can charge more than the whole budget (the walk of the fixed segment),
+    // and a task spilled before any step would be handed on unchanged
+    bool moved = false;
     while (live && !stop) {
-      if (may_spill && ++work > a.budget_iters && (uint32_t)lv < L1) {
+      if (may_spill && ++work > a.budget_iters && (uint32_t)lv < L1 && moved) {
         PROF_BEGIN(sp);
         bool ok_sp = spill((uint32_t)lv);
         PROF_END(sp, 7);
         if (ok_sp) break;
         may_spill = false;
       }
+      moved = true;
       const uint32_t L = (uint32_t)lv;
       if (L == L1) {
         // k == 3: the task's range is the last position itself

@@ -303,6 +303,53 @@ def test_spilled_subtrees_are_exact(oracle_lib, budget, algo):
         assert budget > 5 or rounds > 1
 
 
+def _hub_case(seed, nv=300, p=0.03, n_upd=40):
+    """One hub adjacent to every vertex plus a sparse random graph, one label:
+    the hub's segment is far longer than a 64-entry window."""
+    from tests.helpers import mixed_stream_small
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for b in range(1, nv):
+        src.append(0)
+        dst.append(b)
+    for a in range(1, nv):
+        for b in range(a + 1, nv):
+            if rng.random() < p:
+                src.append(a)
+                dst.append(b)
+    src, dst = np.array(src, np.uint32), np.array(dst, np.uint32)
+    el = np.zeros(len(src), np.uint32)
+    vl = np.zeros(nv, np.uint32)
+    init, ups = mixed_stream_small(src, dst, el, nv, n_upd, 0.65, 1, rng)
+    return vl, init, ups
+
+
+@pytest.mark.parametrize("budget", [1, 2])
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_tiny_budget_with_long_fixed_segments(oracle_lib, budget, algo):
+    """Entering position k-2 charges the walk of the last position's fixed
+    segment to the task's budget, which a hub's segment exceeds at once; a
+    task still takes one step before it spills, so the rounds end and the
+    counts stay exact (k = 4: seeds start at position k-2)."""
+    from paracosm_amd.engine import make_updates
+    from paracosm_amd.gen import Query
+    vl, init, ups = _hub_case(11)
+    for edges in ([(0, 1), (1, 2), (2, 3), (0, 3)], [(0, 1), (1, 2), (0, 2), (2, 3)]):
+        q = Query(vlabels=np.zeros(4, np.uint32), src=np.asarray([e[0] for e in edges], np.uint32),
+                  dst=np.asarray([e[1] for e in edges], np.uint32),
+                  elabels=np.zeros(len(edges), np.uint32))
+        ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+        with _engine(algo, q, vl, init, want_hash=True, spill_budget=budget, split_chunk=4) as m:
+            pos, neg, _, hsh = m.apply_batch(make_updates(*ups))
+            rounds = m.stats()["max_rounds"]
+        print("hub", algo, budget, edges, "rounds", rounds, "total",
+              int(ref.pos.sum() + ref.neg.sum()), flush=True)
+        assert ref.pos.sum() > 0
+        np.testing.assert_array_equal(pos[:, 0], ref.pos)
+        np.testing.assert_array_equal(neg[:, 0], ref.neg)
+        np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+
+
 @pytest.mark.parametrize("k", [12, 17, 24])
 @pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
 def test_large_queries(oracle_lib, k, algo):
