@@ -56,6 +56,9 @@ constexpr uint32_t ERR_POOL_OVERFLOW = 8u;
 constexpr uint32_t ERR_BAD_UPDATE = 16u;
 constexpr uint32_t ERR_VERTEX_UPDATE = 64u;   // a vertex record inside a device batch
 
+constexpr uint32_t NXCD = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr uint32_t XHEAD_STRIDE = 32;  // u32 words between two XCD chunk heads
+
 struct Ctrl {
   uint32_t pool_top;      // bump pointer of the current pool (entries)
   uint32_t nseeds;        // seeds emitted this batch
@@ -76,6 +79,8 @@ struct Ctrl {
   unsigned long long stat_cands;
   unsigned long long stat_bytes;
   unsigned long long prof[32];    // per-phase wave cycles, calls and sizes (CSM_PHASE_PROF builds)
+  // next chunk of each XCD's share of the search round, one 128 B line each
+  uint32_t xhead[NXCD * XHEAD_STRIDE];
 };
 
 // One search plan per (query, matching order): positions 0..k-1.
@@ -909,6 +914,9 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t 
 // captures spilled to scratch memory
 #define CSM_AINL __attribute__((always_inline))
 
+#ifndef CSM_XCD_SPLIT
+#define CSM_XCD_SPLIT 1
+#endif
 #ifndef CSM_CHUNK_GRAB
 #define CSM_CHUNK_GRAB 4
 #endif
@@ -942,7 +950,14 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
   // a wave takes up to CSM_CHUNK_GRAB consecutive chunk ids per atomic: they
   // mostly belong to one task, so its cached segments and L1 lines are
   // reused; rounds with few chunks per wave take fewer, to keep the tail short
+  //
+  // Each XCD (workgroups are dispatched to the 8 XCDs round-robin) first
+  // takes chunks from its own eighth of the round's chunk ids, so that the
+  // consecutive chunks of a task run under one L2; a wave whose share is
+  // used up takes from the next XCD's share.
   uint32_t c_next = 0, c_end = 0;
+  uint32_t xdone = 0;                       // XCD shares seen used up
+  const uint32_t xcd = blockIdx.x % NXCD;
   const uint32_t nwaves = gridDim.x * (uint32_t)search_waves<MK>();
   while (true) {
     PROF_BEGIN(fetch);
@@ -951,10 +966,31 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     if (c_next >= c_end) {
       uint32_t grab = CSM_CHUNK_GRAB;
       while (grab > 1 && (uint64_t)nch < 8ull * grab * nwaves) grab >>= 1;
+#if CSM_XCD_SPLIT
+      bool got = false;
+      for (uint32_t xi = 0; xi < NXCD && !got; ++xi) {
+        const uint32_t y = (xcd + xi) % NXCD;
+        if ((xdone >> y) & 1u) continue;
+        const uint32_t lo = (uint32_t)((uint64_t)nch * y / NXCD);
+        const uint32_t hi = (uint32_t)((uint64_t)nch * (y + 1) / NXCD);
+        uint32_t g = 0;
+        if (lane == 0 && lo < hi) g = atomicAdd(&a.ctrl->xhead[y * XHEAD_STRIDE], grab);
+        g = uniform(__shfl(g, 0));
+        if (lo < hi && g < hi - lo) {
+          c_next = lo + g;
+          c_end = hi - c_next < grab ? hi : c_next + grab;
+          got = true;
+        } else {
+          xdone |= 1u << y;
+        }
+      }
+      if (!got) break;
+#else
       uint32_t c0 = 0;
       if (lane == 0) c0 = atomicAdd(&a.ctrl->chunk_head, grab);
       c_next = uniform(__shfl(c0, 0));
       c_end = c_next + grab;
+#endif
     }
     const uint32_t c = c_next++;
     if (c >= nch) break;
@@ -1589,6 +1625,7 @@ __global__ void k_batch_begin(Ctrl* c) {
     c->nseeds = 0;
     c->nchunks = 0;
     c->chunk_head = 0;
+    for (uint32_t x = 0; x < NXCD; ++x) c->xhead[x * XHEAD_STRIDE] = 0;
     c->nspill = 0;
     c->nspill_chunks = 0;
     c->nruns = 0;
@@ -1605,6 +1642,7 @@ __global__ void k_search_reset(Ctrl* c) {
     c->nseeds = 0;
     c->nchunks = 0;
     c->chunk_head = 0;
+    for (uint32_t x = 0; x < NXCD; ++x) c->xhead[x * XHEAD_STRIDE] = 0;
     c->nspill = 0;
     c->nspill_chunks = 0;
   }
@@ -1618,6 +1656,7 @@ __global__ void k_next_round(Ctrl* c) {
   if (threadIdx.x == 0) {
     c->nchunks = c->nspill_chunks;
     c->chunk_head = 0;
+    for (uint32_t x = 0; x < NXCD; ++x) c->xhead[x * XHEAD_STRIDE] = 0;
     c->nspill = 0;
     c->nspill_chunks = 0;
   }
