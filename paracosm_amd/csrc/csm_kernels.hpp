@@ -918,7 +918,7 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t 
 #define CSM_XCD_SPLIT 1
 #endif
 #ifndef CSM_CHUNK_GRAB
-#define CSM_CHUNK_GRAB 4
+#define CSM_CHUNK_GRAB 1
 #endif
 #ifndef CSM_SEARCH_WAVES_PER_EU
 #define CSM_SEARCH_WAVES_PER_EU 5
@@ -947,14 +947,12 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
   const uint64_t prof_kt0 = __builtin_amdgcn_s_memrealtime();
 #endif
 
-  // a wave takes up to CSM_CHUNK_GRAB consecutive chunk ids per atomic: they
-  // mostly belong to one task, so its cached segments and L1 lines are
-  // reused; rounds with few chunks per wave take fewer, to keep the tail short
-  //
   // Each XCD (workgroups are dispatched to the 8 XCDs round-robin) first
   // takes chunks from its own eighth of the round's chunk ids, so that the
   // consecutive chunks of a task run under one L2; a wave whose share is
-  // used up takes from the next XCD's share.
+  // used up takes from the next XCD's share.  A wave claims CSM_CHUNK_GRAB
+  // consecutive ids per atomic (1 measures best once the shares give the L2
+  // locality; fewer per claim when a round has few chunks per wave).
   uint32_t c_next = 0, c_end = 0;
   uint32_t xdone = 0;                       // XCD shares seen used up
   const uint32_t xcd = blockIdx.x % NXCD;
