@@ -711,6 +711,7 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
   for (uint32_t round = 0;; ++round) {
     sa.spill = e->d_spill[out];
     sa.spill_owner = e->d_spill_owner[out];
+    sa.xregions = round > 0 ? 1u : 0u;
     uint32_t chunks_in = trace ? e->h_ctrl->nspill_chunks : 0;
     launch_round(e, sa, versioned, h);
     int rc = read_ctrl(e);
@@ -746,7 +747,7 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
       }
       return CSM_OK;
     }
-    k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+    k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl, e->spill_chunk_cap);
     sa.tasks = e->d_spill[out];
     sa.chunk_owner = e->d_spill_owner[out];
     sa.chunk_cap = e->spill_chunk_cap;

@@ -81,6 +81,11 @@ struct Ctrl {
   unsigned long long prof[32];    // per-phase wave cycles, calls and sizes (CSM_PHASE_PROF builds)
   // next chunk of each XCD's share of the search round, one 128 B line each
   uint32_t xhead[NXCD * XHEAD_STRIDE];
+  // tasks and chunk ids each XCD has spilled into its region of the spill
+  // buffers this round, and the chunk count of each region for this round
+  uint32_t xspill[NXCD * XHEAD_STRIDE];
+  uint32_t xspill_chunks[NXCD * XHEAD_STRIDE];
+  uint32_t xn[NXCD];
 };
 
 // One search plan per (query, matching order): positions 0..k-1.
@@ -822,6 +827,7 @@ struct SearchArgs {
   uint32_t* spill_owner;      // next round's chunk owners
   uint32_t spill_cap, spill_chunk_cap;
   uint32_t budget_iters;      // loop iterations before a task spills
+  uint32_t xregions;          // this round's chunks sit in per-XCD regions (spilled rounds)
   Ctrl* ctrl;
   unsigned long long* counts;
   unsigned long long* hashes;
@@ -917,6 +923,11 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t 
 #ifndef CSM_XCD_SPLIT
 #define CSM_XCD_SPLIT 1
 #endif
+#ifndef CSM_XCD_REGIONS
+#define CSM_XCD_REGIONS 1
+#endif
+// spilled tasks stay on the XCD that spilled them (needs the XCD shares)
+constexpr bool XCD_REGIONS = CSM_XCD_REGIONS && CSM_XCD_SPLIT;
 #ifndef CSM_CHUNK_GRAB
 #define CSM_CHUNK_GRAB 1
 #endif
@@ -956,6 +967,9 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
   uint32_t c_next = 0, c_end = 0;
   uint32_t xdone = 0;                       // XCD shares seen used up
   const uint32_t xcd = blockIdx.x % NXCD;
+  // spilled rounds: XCD y's share is region y of the spill buffers, where
+  // XCD y's waves wrote the tasks of the previous round
+  const uint32_t xn_lane = (XCD_REGIONS && a.xregions && lane < NXCD) ? a.ctrl->xn[lane] : 0u;
   const uint32_t nwaves = gridDim.x * (uint32_t)search_waves<MK>();
   while (true) {
     PROF_BEGIN(fetch);
@@ -969,8 +983,14 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       for (uint32_t xi = 0; xi < NXCD && !got; ++xi) {
         const uint32_t y = (xcd + xi) % NXCD;
         if ((xdone >> y) & 1u) continue;
-        const uint32_t lo = (uint32_t)((uint64_t)nch * y / NXCD);
-        const uint32_t hi = (uint32_t)((uint64_t)nch * (y + 1) / NXCD);
+        uint32_t lo, hi;
+        if (XCD_REGIONS && a.xregions) {
+          lo = y * (a.chunk_cap / NXCD);
+          hi = lo + uniform(__shfl(xn_lane, y));
+        } else {
+          lo = (uint32_t)((uint64_t)nch * y / NXCD);
+          hi = (uint32_t)((uint64_t)nch * (y + 1) / NXCD);
+        }
         uint32_t g = 0;
         if (lane == 0 && lo < hi) g = atomicAdd(&a.ctrl->xhead[y * XHEAD_STRIDE], grab);
         g = uniform(__shfl(g, 0));
@@ -991,7 +1011,9 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
 #endif
     }
     const uint32_t c = c_next++;
+#if !CSM_XCD_SPLIT
     if (c >= nch) break;
+#endif
     if (__atomic_load_n(&a.ctrl->abort, __ATOMIC_RELAXED)) break;
     const uint32_t sidx = uniform(a.chunk_owner[c]);
     if (sidx == NONE) continue;
@@ -1260,6 +1282,26 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
     auto reserve_spill = [&](uint32_t n_t, uint32_t n_c, uint32_t& tb, uint32_t& cb) CSM_AINL -> bool {
       tb = 0;
       cb = 0;
+      if (XCD_REGIONS) {
+        // this XCD's region: an eighth of each spill buffer
+        const uint32_t tcap = a.spill_cap / NXCD, ccap = a.spill_chunk_cap / NXCD;
+        if (lane == 0) {
+          tb = atomicAdd(&a.ctrl->xspill[xcd * XHEAD_STRIDE], n_t);
+          cb = atomicAdd(&a.ctrl->xspill_chunks[xcd * XHEAD_STRIDE], n_c);
+          atomicAdd(&a.ctrl->nspill, n_t);
+        }
+        tb = uniform(__shfl(tb, 0));
+        cb = uniform(__shfl(cb, 0));
+        const uint32_t tb0 = xcd * tcap, cb0 = xcd * ccap;
+        if ((uint64_t)tb + n_t <= tcap && (uint64_t)cb + n_c <= ccap) {
+          tb += tb0;
+          cb += cb0;
+          return true;
+        }
+        for (uint64_t id = (uint64_t)cb + lane; id < (uint64_t)cb + n_c && id < ccap; id += WAVE)
+          a.spill_owner[cb0 + id] = NONE;
+        return false;
+      }
       if (lane == 0) {
         tb = atomicAdd(&a.ctrl->nspill, n_t);
         cb = atomicAdd(&a.ctrl->nspill_chunks, n_c);
@@ -1623,7 +1665,12 @@ __global__ void k_batch_begin(Ctrl* c) {
     c->nseeds = 0;
     c->nchunks = 0;
     c->chunk_head = 0;
-    for (uint32_t x = 0; x < NXCD; ++x) c->xhead[x * XHEAD_STRIDE] = 0;
+    for (uint32_t x = 0; x < NXCD; ++x) {
+      c->xhead[x * XHEAD_STRIDE] = 0;
+      c->xspill[x * XHEAD_STRIDE] = 0;
+      c->xspill_chunks[x * XHEAD_STRIDE] = 0;
+      c->xn[x] = 0;
+    }
     c->nspill = 0;
     c->nspill_chunks = 0;
     c->nruns = 0;
@@ -1640,19 +1687,39 @@ __global__ void k_search_reset(Ctrl* c) {
     c->nseeds = 0;
     c->nchunks = 0;
     c->chunk_head = 0;
-    for (uint32_t x = 0; x < NXCD; ++x) c->xhead[x * XHEAD_STRIDE] = 0;
+    for (uint32_t x = 0; x < NXCD; ++x) {
+      c->xhead[x * XHEAD_STRIDE] = 0;
+      c->xspill[x * XHEAD_STRIDE] = 0;
+      c->xspill_chunks[x * XHEAD_STRIDE] = 0;
+      c->xn[x] = 0;
+    }
     c->nspill = 0;
     c->nspill_chunks = 0;
   }
 }
 
 // the spilled tasks of the last round become the input of the next
-__global__ void k_next_round(Ctrl* c) {
+__global__ void k_next_round(Ctrl* c, uint32_t spill_chunk_cap) {
 #ifdef CSM_PHASE_PROF
   if (threadIdx.x == 0) { c->prof[30] = ~0ull; c->prof[31] = 0; }
 #endif
   if (threadIdx.x == 0) {
-    c->nchunks = c->nspill_chunks;
+    if (XCD_REGIONS) {
+      // region x holds XCD x's spills (ids past the region's end were never
+      // handed out)
+      const uint32_t ccap = spill_chunk_cap / NXCD;
+      uint32_t total = 0;
+      for (uint32_t x = 0; x < NXCD; ++x) {
+        const uint32_t n = c->xspill_chunks[x * XHEAD_STRIDE];
+        c->xn[x] = n < ccap ? n : ccap;
+        total += c->xn[x];
+        c->xspill[x * XHEAD_STRIDE] = 0;
+        c->xspill_chunks[x * XHEAD_STRIDE] = 0;
+      }
+      c->nchunks = total;
+    } else {
+      c->nchunks = c->nspill_chunks;
+    }
     c->chunk_head = 0;
     for (uint32_t x = 0; x < NXCD; ++x) c->xhead[x * XHEAD_STRIDE] = 0;
     c->nspill = 0;
