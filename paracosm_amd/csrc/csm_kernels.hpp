@@ -871,16 +871,17 @@ struct WaveState {
   uint32_t pbw[MK][MK - 1];
 };
 
-// first index in the ascending LDS array a[0, n) holding a value >= x
-__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t n, uint32_t x) {
-  uint32_t lo = 0;
-  while (n > 0) {
-    uint32_t half = n >> 1;
-    bool less = a[lo + half] < x;
-    lo = less ? lo + half + 1 : lo;
-    n = less ? n - half - 1 : half;
+// is x in the ascending LDS array a[0, n): the halvings depend on n only, so
+// every lane runs the same number of steps (no divergent loop exits)
+__device__ __forceinline__ bool lds_contains(const uint32_t* a, uint32_t n, uint32_t x) {
+  if (n == 0) return false;
+  uint32_t b = 0;
+  for (uint32_t m = n; m > 1;) {
+    const uint32_t half = m >> 1;
+    b = a[b + half] <= x ? b + half : b;
+    m -= half;
   }
-  return lo;
+  return a[b] == x;
 }
 
 // One wavefront runs one depth-2 chunk of a seed at a time: an iterative DFS
@@ -1174,8 +1175,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       // C without the vertex at position k-3 (it is in F at most once)
       auto c_eff = [&](uint32_t C) CSM_AINL -> uint32_t {
         if (a.homo || C == 0) return C;
-        const uint32_t i = lds_lower_bound(S.F, C, mk3);
-        return (i < C && S.F[i] == mk3) ? C - 1u : C;
+        return lds_contains(S.F, C, mk3) ? C - 1u : C;
       };
       if (reuse) {
         const uint32_t Ce = c_eff(S.lC);
@@ -1364,8 +1364,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
         bool self = false;
         if (lane < bn && !a.homo) {
           if (useF) {
-            const uint32_t i = lds_lower_bound(S.F, nF, v);
-            self = i < nF && S.F[i] == v;
+            self = lds_contains(S.F, nF, v);
           } else {
             const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
             uint32_t bw = S.pbw[L1][jf];
@@ -1444,13 +1443,13 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
         bool ok = g < total;
         uint32_t w = 0, p = 0;
         if (ok) {
-          // parent p: last index with pref[p] <= g (pref is non-decreasing)
-          uint32_t lo = 0, hi = bn;
-          while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (S.pref[mid] <= g) lo = mid; else hi = mid;
+          // parent p: last index with pref[p] <= g (pref is non-decreasing;
+          // the halvings depend on bn only)
+          for (uint32_t m = bn; m > 1;) {
+            const uint32_t half = m >> 1;
+            p = S.pref[p + half] <= g ? p + half : p;
+            m -= half;
           }
-          p = lo;
           const uint32_t sp = S.src[p];
           const uint32_t off = g - S.pref[p];
           if ((sp & 1u) || !useF) {
@@ -1470,10 +1469,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
           } else {
             // F leaves the test against position k-3 to the leaves
             if (ok && !a.homo) ok = w != mk3;
-            if (ok && (sp & 1u)) {
-              const uint32_t i = lds_lower_bound(S.F, nF, w);
-              ok = i < nF && S.F[i] == w;
-            }
+            if (ok && (sp & 1u)) ok = lds_contains(S.F, nF, w);
           }
           if (ok && jp != NONE && !(sp & 1u)) {
             uint32_t x = ((S.pbw[L1][jp] >> 8) << vbits) | w;
