@@ -221,6 +221,33 @@ __device__ __forceinline__ bool contains(const uint32_t* __restrict__ adj,
   return false;
 }
 
+// contains() for a segment [lo, lo + n) that is the same for every lane of
+// the wave (only x differs): the halvings depend on n only, so all lanes
+// step together, two halvings per round trip (3 pivots loaded together); the
+// first pivots are one address for the whole wave.  Finds the last entry
+// <= x, then (versioned) scans back over equal entries for a visible one.
+__device__ __forceinline__ bool contains_u(const uint32_t* __restrict__ adj,
+                                           const uint32_t* __restrict__ ats, uint32_t lo,
+                                           uint32_t n, uint32_t x, bool versioned, uint32_t pos) {
+  if (n == 0) return false;
+  uint32_t b = lo, m = n;
+  while (m > 2) {
+    const uint32_t h1 = m >> 1, m1 = m - h1;
+    const uint32_t h2 = m1 >> 1;
+    const uint32_t v1 = adj[b + h1], vA = adj[b + h2], vB = adj[b + h1 + h2];
+    const bool c1 = v1 <= x;
+    const bool c2 = (c1 ? vB : vA) <= x;
+    b = c1 ? b + h1 : b;
+    b = c2 ? b + h2 : b;
+    m = m1 - h2;
+  }
+  if (m == 2) b = adj[b + 1] <= x ? b + 1 : b;
+  if (!versioned) return adj[b] == x;
+  for (uint32_t i = b + 1; i-- > lo && adj[i] == x;)
+    if (ts_visible(ats[i], pos)) return true;
+  return false;
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1090,7 +1117,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
         uint32_t w = S.pbw[lv][j];
         uint32_t x = ((w >> 8) << vbits) | v;
         bool vd = VERSIONED && ((S.pdirty[lv] >> j) & 1u);
-        ok = contains(adj, ats, S.plo[lv][j], S.phi[lv][j], x, vd, pos);
+        ok = contains_u(adj, ats, S.plo[lv][j], S.phi[lv][j] - S.plo[lv][j], x, vd, pos);
       }
       vout = v;
       return ok;
@@ -1109,7 +1136,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
         uint32_t bw = S.pbw[L1][j];
         uint32_t x = ((bw >> 8) << vbits) | w;
         bool vd = VERSIONED && ((S.pdirty[L1] >> j) & 1u);
-        ok = contains(adj, ats, S.plo[L1][j], S.phi[L1][j], x, vd, pos);
+        ok = contains_u(adj, ats, S.plo[L1][j], S.phi[L1][j] - S.plo[L1][j], x, vd, pos);
       }
       return ok;
     };
@@ -1369,7 +1396,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
             const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
             uint32_t bw = S.pbw[L1][jf];
             uint32_t x = ((bw >> 8) << vbits) | v;
-            self = contains(adj, ats, S.plo[L1][jf], S.phi[L1][jf], x, fd, pos) &&
+            self = contains_u(adj, ats, S.plo[L1][jf], S.phi[L1][jf] - S.plo[L1][jf], x, fd, pos) &&
                    last_fixed_ok(v, jf, L2);
           }
         }
