@@ -1199,13 +1199,21 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       PROF_ADD(10, jp == NONE ? 1u : 0u);
       PROF_ADD(11, jf == NONE ? 0u : fsize);
       const uint32_t mk3 = S.m[L2 - 1];
-      // C without the vertex at position k-3 (it is in F at most once)
-      auto c_eff = [&](uint32_t C) CSM_AINL -> uint32_t {
+      // C without the vertex at position k-3 (one of the C leaves at most
+      // once): looked up in F, or checked in the fixed segments when F is
+      // not in LDS
+      auto c_eff = [&](uint32_t C, bool kept) CSM_AINL -> uint32_t {
         if (a.homo || C == 0) return C;
-        return lds_contains(S.F, C, mk3) ? C - 1u : C;
+        if (kept) return lds_contains(S.F, C, mk3) ? C - 1u : C;
+        const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
+        const uint32_t x = ((S.pbw[L1][jf] >> 8) << vbits) | mk3;
+        const bool in = contains_u(adj, ats, S.plo[L1][jf], S.phi[L1][jf] - S.plo[L1][jf], x, fd,
+                                   pos) &&
+                        last_fixed_ok(mk3, jf, L2 - 1);
+        return in ? C - 1u : C;
       };
       if (reuse) {
-        const uint32_t Ce = c_eff(S.lC);
+        const uint32_t Ce = c_eff(S.lC, S.lF != NONE);
         if (lane == 0) S.lCeff = Ce;
         wave_sync();
         PROF_END(pl, 2);
@@ -1215,11 +1223,13 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       wave_sync();
       if (jf != NONE && (fsize <= FCAP || (jp == NONE && !HASH))) {
         // walk the smallest fixed segment once: F (when it fits in LDS) and
-        // C, the number of valid leaves shared by every parent
+        // C, the number of valid leaves shared by every parent, both without
+        // the injectivity test against position k-3, so that they hold for
+        // every entry of position k-2 under the same fixed vertices
         const uint32_t lo = uniform(__shfl(my_lo, jf)), hi = uniform(__shfl(my_hi, jf));
         const bool fd = VERSIONED && ((dmask >> jf) & 1u);
         const bool keep = fsize <= FCAP;
-        const uint32_t ninj = keep ? L2 - 1 : L2;
+        const uint32_t ninj = L2 - 1;
         uint32_t C = 0;
         for (uint32_t b = lo; b < hi; b += WAVE) {
           uint32_t e = b + lane;
@@ -1237,14 +1247,12 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
         }
         work += (hi - lo) / WAVE;
         wave_sync();
-        const uint32_t Ce = keep ? c_eff(C) : C;
+        const uint32_t Ce = c_eff(C, keep);
         if (lane == 0) {
           S.lC = C;
           S.lCeff = Ce;
-          if (keep) {
-            S.lF = C;
-            S.fvalid = 1;
-          }
+          if (keep) S.lF = C;
+          S.fvalid = 1;
         }
         wave_sync();
         if (Ce == 0) return false;   // no leaf can complete under this prefix

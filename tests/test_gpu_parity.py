@@ -350,6 +350,30 @@ def test_tiny_budget_with_long_fixed_segments(oracle_lib, budget, algo):
         np.testing.assert_array_equal(hsh[:, 0], ref.hash)
 
 
+@pytest.mark.parametrize("budget", [0, 2])
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_fixed_segment_beyond_the_leaf_filter(oracle_lib, budget, algo):
+    """A hub of ~1,300 neighbours under one label: the last position's fixed
+    segment exceeds the 512-entry leaf filter, so the shared leaf count is
+    walked in global memory and reused across the entries of position k-2
+    (no hashes: the count-only path)."""
+    from paracosm_amd.engine import make_updates
+    from paracosm_amd.gen import Query
+    vl, init, ups = _hub_case(21, nv=2600, p=0.001, n_upd=40)
+    edges = [(0, 1), (1, 2), (0, 2), (0, 3)]   # triangle with a pendant on vertex 0
+    q = Query(vlabels=np.zeros(4, np.uint32), src=np.asarray([e[0] for e in edges], np.uint32),
+              dst=np.asarray([e[1] for e in edges], np.uint32),
+              elabels=np.zeros(len(edges), np.uint32))
+    ref = _oracle(oracle_lib, q, vl, init, ups)
+    kw = {"spill_budget": budget, "split_chunk": 4} if budget else {}
+    with _engine(algo, q, vl, init, **kw) as m:
+        pos, neg, _, _ = m.apply_batch(make_updates(*ups))
+    print("wide hub", algo, budget, "total", int(ref.pos.sum() + ref.neg.sum()), flush=True)
+    assert ref.pos.sum() > 0 and ref.neg.sum() > 0
+    np.testing.assert_array_equal(pos[:, 0], ref.pos)
+    np.testing.assert_array_equal(neg[:, 0], ref.neg)
+
+
 @pytest.mark.parametrize("k", [12, 17, 24])
 @pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
 def test_large_queries(oracle_lib, k, algo):
