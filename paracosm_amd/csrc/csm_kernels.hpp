@@ -929,7 +929,17 @@ __device__ __forceinline__ bool lds_contains(const uint32_t* a, uint32_t n, uint
 //     own and the fixed segment is the candidate source, and the wave walks
 //     the concatenation of all parents' sources 64 entries at a time.
 // Optional per-phase cycle accounting of the search (built with
-// -DCSM_PHASE_PROF): wave cycles spent in each phase, summed over waves.
+// -DCSM_PHASE_PROF): wave cycles spent in each phase, summed over waves,
+// and counters (printed as "counter i" with CSM_TRACE_ROUNDS set):
+//   0 leaf entries walked            1 parents handed to the leaf loop
+//   2 parents walking their own segment
+//   3 / 5 leaf entries walked without / with F in LDS
+//   4 / 7 wave cycles of leaf loops without / with F in LDS
+//   6 sum of the parents' own segment lengths
+//   8 backward neighbours and 9 candidates at refills of position k-2
+//  10 entries of position k-2 without a parent neighbour
+//  11 fixed segment entries at those entries
+//  12 wave lifetime (100 MHz ticks)    13 waves
 #ifdef CSM_PHASE_PROF
 #define PROF_BEGIN(id) const uint64_t prof_t0_##id = __builtin_readcyclecounter()
 #define PROF_END(id, i) \
@@ -1471,7 +1481,6 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
       S.pref[lane] = incl - size;
       if (lane == 0) S.pref[WAVE] = total;
       wave_sync();
-      const uint32_t nfb = uniform(S.pnb[L1]);
       const uint32_t fsum_lo = jf != NONE ? S.plo[L1][jf] : 0u;
       for (uint32_t b = 0; b < total; b += WAVE) {
         const uint32_t g = b + lane;
@@ -1511,7 +1520,6 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 16 
             ok = contains(adj, ats, S.olo[p], S.ohi[p], x, VERSIONED && ((sp >> 1) & 1u), pos);
           }
         }
-        (void)nfb;
         cnt += popc64(ballot(ok));
         n_cands += (total - b) < WAVE ? (total - b) : WAVE;
         if (HASH) {
