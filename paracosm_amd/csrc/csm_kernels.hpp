@@ -938,7 +938,7 @@ __device__ __forceinline__ bool lds_contains(const uint32_t* a, uint32_t n, uint
 //   6 sum of the parents' own segment lengths
 //   8 backward neighbours and 9 candidates at refills of position k-2
 //  10 entries of position k-2 without a parent neighbour
-//  11 fixed segment entries at those entries
+//  11 smallest fixed segment lengths at entries of position k-2
 //  12 wave lifetime (100 MHz ticks)    13 waves
 #ifdef CSM_PHASE_PROF
 #define PROF_BEGIN(id) const uint64_t prof_t0_##id = __builtin_readcyclecounter()
