@@ -325,7 +325,7 @@ def _hub_case(seed, nv=300, p=0.03, n_upd=40):
 
 
 @pytest.mark.parametrize("budget", [1, 2])
-@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi", "parallel_turboflux"])
 def test_tiny_budget_with_long_fixed_segments(oracle_lib, budget, algo):
     """Entering position k-2 charges the walk of the last position's fixed
     segment to the task's budget, which a hub's segment exceeds at once; a
@@ -351,7 +351,7 @@ def test_tiny_budget_with_long_fixed_segments(oracle_lib, budget, algo):
 
 
 @pytest.mark.parametrize("budget", [0, 2])
-@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi", "parallel_turboflux"])
 def test_fixed_segment_beyond_the_leaf_filter(oracle_lib, budget, algo):
     """A hub of ~1,300 neighbours under one label: the last position's fixed
     segment exceeds the 512-entry leaf filter, so the shared leaf count is
