@@ -108,6 +108,10 @@ struct csm_engine {
   Task* d_seeds = nullptr;
   uint32_t seed_cap = 0;
   uint32_t* d_chunk_owner = nullptr;
+  // repack scratch, kept between repacks: list lengths, new offsets, scan space
+  uint32_t *d_rp_lens = nullptr, *d_rp_offs = nullptr;
+  uint8_t* d_rp_tmp = nullptr;
+  size_t rp_tmp_bytes = 0;
   uint32_t chunk_cap = 0;
   Task* d_spill[2] = {nullptr, nullptr};          // ping-pong task lists of later rounds
   uint32_t* d_spill_owner[2] = {nullptr, nullptr};
@@ -537,18 +541,20 @@ int repack(csm_engine* e) {
         (rc = dalloc(e, &e->d_ats[other], e->pool_cap)))
       return rc;
   }
-  uint32_t *lens = nullptr, *offs = nullptr;
   int rc;
-  if ((rc = dalloc(e, &lens, e->nv_cap)) || (rc = dalloc(e, &offs, e->nv_cap))) return rc;
+  if (!e->d_rp_lens) {
+    if ((rc = dalloc(e, &e->d_rp_lens, e->nv_cap)) || (rc = dalloc(e, &e->d_rp_offs, e->nv_cap)))
+      return rc;
+    HIP_TRY(e, rocprim::exclusive_scan(nullptr, e->rp_tmp_bytes, e->d_rp_lens, e->d_rp_offs, 0u,
+                                       e->nv_cap, rocprim::plus<uint32_t>(), e->stream));
+    if ((rc = dalloc(e, &e->d_rp_tmp, e->rp_tmp_bytes))) return rc;
+  }
+  uint32_t *lens = e->d_rp_lens, *offs = e->d_rp_offs;
   k_repack_lens<<<grid_for(e->nv_cap, 256), 256, 0, e->stream>>>(e->d_vrec, lens, e->nv_cap);
-  size_t need = 0;
-  HIP_TRY(e, rocprim::exclusive_scan(nullptr, need, lens, offs, 0u, e->nv_cap,
+  size_t need = e->rp_tmp_bytes;
+  HIP_TRY(e, rocprim::exclusive_scan(e->d_rp_tmp, need, lens, offs, 0u, e->nv_cap,
                                      rocprim::plus<uint32_t>(), e->stream));
-  uint8_t* tmp = nullptr;
-  if ((rc = dalloc(e, &tmp, need))) return rc;
-  HIP_TRY(e, rocprim::exclusive_scan(tmp, need, lens, offs, 0u, e->nv_cap,
-                                     rocprim::plus<uint32_t>(), e->stream));
-  k_repack_copy<256><<<grid_for(e->nv_cap, 1, 8192), 256, 0, e->stream>>>(
+  k_repack_copy<256><<<grid_for(e->nv_cap, 16, 65536), 256, 0, e->stream>>>(
       e->d_vrec, offs, e->d_adj[e->cur], e->d_ats[e->cur], e->d_adj[other], e->d_ats[other],
       e->nv_cap);
   HIP_TRY(e, hipStreamSynchronize(e->stream));
@@ -556,7 +562,6 @@ int repack(csm_engine* e) {
   e->h_ctrl->pool_top = (uint32_t)e->h_ctrl->live;
   HIP_TRY(e, hipMemcpy(e->d_ctrl, e->h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice));
   e->cur = other;
-  dfree(e, lens); dfree(e, offs); dfree(e, tmp);
   ++e->stats[4];
   return CSM_OK;
 }

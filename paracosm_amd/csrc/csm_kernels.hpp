@@ -1837,6 +1837,7 @@ __global__ void k_repack_lens(const uint2* __restrict__ vrec, uint32_t* __restri
     lens[v] = vrec[v].y & LEN_MASK;
 }
 
+// one wavefront per list: entries and versions to their new offset
 template <int BS>
 __global__ __launch_bounds__(BS) void k_repack_copy(uint2* __restrict__ vrec,
                                                     const uint32_t* __restrict__ offs,
@@ -1844,15 +1845,16 @@ __global__ __launch_bounds__(BS) void k_repack_copy(uint2* __restrict__ vrec,
                                                     const uint32_t* __restrict__ ats_in,
                                                     uint32_t* __restrict__ adj_out,
                                                     uint32_t* __restrict__ ats_out, uint32_t nv) {
-  for (uint32_t v = blockIdx.x; v < nv; v += gridDim.x) {
-    uint2 r = vrec[v];
-    uint32_t len = r.y & LEN_MASK, no = offs[v];
-    for (uint32_t i = threadIdx.x; i < len; i += BS) {
+  const uint32_t lane = lane_id();
+  const uint32_t nw = gridDim.x * (BS / WAVE);
+  for (uint32_t v = blockIdx.x * (BS / WAVE) + threadIdx.x / WAVE; v < nv; v += nw) {
+    const uint2 r = vrec[v];
+    const uint32_t len = r.y & LEN_MASK, no = offs[v];
+    for (uint32_t i = lane; i < len; i += WAVE) {
       adj_out[no + i] = adj_in[r.x + i];
       ats_out[no + i] = ats_in[r.x + i];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) vrec[v].x = no;
+    if (lane == 0) vrec[v].x = no;
   }
 }
 
