@@ -974,7 +974,7 @@ constexpr bool XCD_REGIONS = CSM_XCD_REGIONS && CSM_XCD_SPLIT;
 #endif
 template <int MK, bool VERSIONED, bool HASH, bool SYMBI>
 __global__ __launch_bounds__(search_waves<MK>() * WAVE)
-__attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 12 ? 4 : MK <= 16 ? 3 : 1, 8))) void k_search(SearchArgs a) {
+__attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 ? 5 : MK <= 12 ? 4 : MK <= 16 ? 3 : 1, 8))) void k_search(SearchArgs a) {
   __shared__ WaveState<MK> S_all[search_waves<MK>()];
   const uint32_t lane = lane_id();
   const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -1328,23 +1328,30 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 12 
       tb = 0;
       cb = 0;
       if (XCD_REGIONS) {
-        // this XCD's region: an eighth of each spill buffer
+        // this XCD's region (an eighth of each spill buffer), or the next
+        // one with room; ids a failed try took are written ownerless
         const uint32_t tcap = a.spill_cap / NXCD, ccap = a.spill_chunk_cap / NXCD;
         if (lane == 0) {
-          tb = atomicAdd(&a.ctrl->xspill[xcd * XHEAD_STRIDE], n_t);
-          cb = atomicAdd(&a.ctrl->xspill_chunks[xcd * XHEAD_STRIDE], n_c);
           atomicAdd(&a.ctrl->nspill, n_t);
+          atomicAdd(&a.ctrl->nspill_chunks, n_c);
         }
-        tb = uniform(__shfl(tb, 0));
-        cb = uniform(__shfl(cb, 0));
-        const uint32_t tb0 = xcd * tcap, cb0 = xcd * ccap;
-        if ((uint64_t)tb + n_t <= tcap && (uint64_t)cb + n_c <= ccap) {
-          tb += tb0;
-          cb += cb0;
-          return true;
+        for (uint32_t xi = 0; xi < NXCD; ++xi) {
+          const uint32_t y = (xcd + xi) % NXCD;
+          uint32_t t0 = 0, c0 = 0;
+          if (lane == 0) {
+            t0 = atomicAdd(&a.ctrl->xspill[y * XHEAD_STRIDE], n_t);
+            c0 = atomicAdd(&a.ctrl->xspill_chunks[y * XHEAD_STRIDE], n_c);
+          }
+          t0 = uniform(__shfl(t0, 0));
+          c0 = uniform(__shfl(c0, 0));
+          if ((uint64_t)t0 + n_t <= tcap && (uint64_t)c0 + n_c <= ccap) {
+            tb = y * tcap + t0;
+            cb = y * ccap + c0;
+            return true;
+          }
+          for (uint64_t id = (uint64_t)c0 + lane; id < (uint64_t)c0 + n_c && id < ccap; id += WAVE)
+            a.spill_owner[y * ccap + id] = NONE;
         }
-        for (uint64_t id = (uint64_t)cb + lane; id < (uint64_t)cb + n_c && id < ccap; id += WAVE)
-          a.spill_owner[cb0 + id] = NONE;
         return false;
       }
       if (lane == 0) {
