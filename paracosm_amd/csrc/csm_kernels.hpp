@@ -1328,30 +1328,23 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       tb = 0;
       cb = 0;
       if (XCD_REGIONS) {
-        // this XCD's region (an eighth of each spill buffer), or the next
-        // one with room; ids a failed try took are written ownerless
+        // this XCD's region: an eighth of each spill buffer
         const uint32_t tcap = a.spill_cap / NXCD, ccap = a.spill_chunk_cap / NXCD;
         if (lane == 0) {
+          tb = atomicAdd(&a.ctrl->xspill[xcd * XHEAD_STRIDE], n_t);
+          cb = atomicAdd(&a.ctrl->xspill_chunks[xcd * XHEAD_STRIDE], n_c);
           atomicAdd(&a.ctrl->nspill, n_t);
-          atomicAdd(&a.ctrl->nspill_chunks, n_c);
         }
-        for (uint32_t xi = 0; xi < NXCD; ++xi) {
-          const uint32_t y = (xcd + xi) % NXCD;
-          uint32_t t0 = 0, c0 = 0;
-          if (lane == 0) {
-            t0 = atomicAdd(&a.ctrl->xspill[y * XHEAD_STRIDE], n_t);
-            c0 = atomicAdd(&a.ctrl->xspill_chunks[y * XHEAD_STRIDE], n_c);
-          }
-          t0 = uniform(__shfl(t0, 0));
-          c0 = uniform(__shfl(c0, 0));
-          if ((uint64_t)t0 + n_t <= tcap && (uint64_t)c0 + n_c <= ccap) {
-            tb = y * tcap + t0;
-            cb = y * ccap + c0;
-            return true;
-          }
-          for (uint64_t id = (uint64_t)c0 + lane; id < (uint64_t)c0 + n_c && id < ccap; id += WAVE)
-            a.spill_owner[y * ccap + id] = NONE;
+        tb = uniform(__shfl(tb, 0));
+        cb = uniform(__shfl(cb, 0));
+        const uint32_t tb0 = xcd * tcap, cb0 = xcd * ccap;
+        if ((uint64_t)tb + n_t <= tcap && (uint64_t)cb + n_c <= ccap) {
+          tb += tb0;
+          cb += cb0;
+          return true;
         }
+        for (uint64_t id = (uint64_t)cb + lane; id < (uint64_t)cb + n_c && id < ccap; id += WAVE)
+          a.spill_owner[cb0 + id] = NONE;
         return false;
       }
       if (lane == 0) {
