@@ -580,8 +580,10 @@ int alloc_batch(csm_engine* e) {
     return rc;
   e->seed_cap = std::max<uint32_t>(1u << 16, std::min<uint64_t>(16ull * B, 1u << 22));
   e->chunk_cap = 1u << 26;
-  e->spill_cap = 1u << 21;
-  e->spill_chunk_cap = 1u << 24;
+  // spill buffers: each XCD writes into its own eighth, and a heavy subtree
+  // keeps spilling into one region (2^22 tasks: 512k per region)
+  e->spill_cap = 1u << 22;
+  e->spill_chunk_cap = 1u << 25;
   if ((rc = dalloc(e, &e->d_seeds, e->seed_cap)) ||
       (rc = dalloc(e, &e->d_chunk_owner, e->chunk_cap)) ||
       (rc = dalloc(e, &e->d_spill[0], e->spill_cap)) ||
