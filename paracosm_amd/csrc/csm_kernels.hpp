@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "csm_search.hpp"
+
 namespace csm {
 
 constexpr uint32_t DIRTY = 0x80000000u;
@@ -140,43 +142,10 @@ struct GraphView {
   uint32_t nv;
 };
 
-__device__ __forceinline__ bool ts_visible(uint32_t ts, uint32_t pos) {
-  return (ts >> 16) <= pos && pos <= (ts & 0xFFFFu);
-}
-
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
-}
-
-// first index i in [lo, hi) with a[i] >= x (hi if none).  The halvings of
-// the binary search are unchanged, but while at least 3 entries remain the
-// 3 pivots the next two halvings can visit are loaded together and the two
-// comparisons made afterwards: one memory round trip per two levels (the
-// loads are independent, so they overlap).
-__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ a, uint32_t lo,
-                                                uint32_t hi, uint32_t x) {
-  uint32_t n = hi - lo;
-  while (n >= 3) {
-    // [lo, lo + n1) | m | [m + 1, m + 1 + n2), both halves non-empty
-    const uint32_t n1 = n >> 1, n2 = n - n1 - 1, m = lo + n1;
-    const uint32_t vm = a[m], vA = a[lo + (n1 >> 1)], vB = a[m + 1 + (n2 >> 1)];
-    const bool c1 = vm < x;
-    const uint32_t l2 = c1 ? m + 1 : lo, k2 = c1 ? n2 : n1;
-    const bool c2 = (c1 ? vB : vA) < x;
-    const uint32_t h2 = k2 >> 1;
-    lo = c2 ? l2 + h2 + 1 : l2;
-    n = c2 ? k2 - h2 - 1 : h2;
-  }
-  while (n > 0) {
-    uint32_t half = n >> 1;
-    uint32_t mid = lo + half;
-    bool less = a[mid] < x;
-    lo = less ? mid + 1 : lo;
-    n = less ? n - half - 1 : half;
-  }
-  return lo;
 }
 
 // segment of key `key` inside the list [off, off+len)
@@ -208,44 +177,6 @@ __device__ __forceinline__ uint2 seg_of(const GraphView& g, uint32_t v, uint32_t
   const uint2 r = g.vrec[v];
   *rec_y = r.y;
   return seg_at(g, v, r.x, r.y & LEN_MASK, key);
-}
-
-// is packed value x visible in [lo, hi) at update position pos
-__device__ __forceinline__ bool contains(const uint32_t* __restrict__ adj,
-                                         const uint32_t* __restrict__ ats, uint32_t lo,
-                                         uint32_t hi, uint32_t x, bool versioned, uint32_t pos) {
-  uint32_t i = lower_bound(adj, lo, hi, x);
-  if (!versioned) return i < hi && adj[i] == x;
-  for (; i < hi && adj[i] == x; ++i)
-    if (ts_visible(ats[i], pos)) return true;
-  return false;
-}
-
-// contains() for a segment [lo, lo + n) that is the same for every lane of
-// the wave (only x differs): the halvings depend on n only, so all lanes
-// step together, two halvings per round trip (3 pivots loaded together); the
-// first pivots are one address for the whole wave.  Finds the last entry
-// <= x, then (versioned) scans back over equal entries for a visible one.
-__device__ __forceinline__ bool contains_u(const uint32_t* __restrict__ adj,
-                                           const uint32_t* __restrict__ ats, uint32_t lo,
-                                           uint32_t n, uint32_t x, bool versioned, uint32_t pos) {
-  if (n == 0) return false;
-  uint32_t b = lo, m = n;
-  while (m > 2) {
-    const uint32_t h1 = m >> 1, m1 = m - h1;
-    const uint32_t h2 = m1 >> 1;
-    const uint32_t v1 = adj[b + h1], vA = adj[b + h2], vB = adj[b + h1 + h2];
-    const bool c1 = v1 <= x;
-    const bool c2 = (c1 ? vB : vA) <= x;
-    b = c1 ? b + h1 : b;
-    b = c2 ? b + h2 : b;
-    m = m1 - h2;
-  }
-  if (m == 2) b = adj[b + 1] <= x ? b + 1 : b;
-  if (!versioned) return adj[b] == x;
-  for (uint32_t i = b + 1; i-- > lo && adj[i] == x;)
-    if (ts_visible(ats[i], pos)) return true;
-  return false;
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -897,19 +828,6 @@ struct WaveState {
   uint32_t pnb[MK], pd2[MK], pqv[MK];
   uint32_t pbw[MK][MK - 1];
 };
-
-// is x in the ascending LDS array a[0, n): the halvings depend on n only, so
-// every lane runs the same number of steps (no divergent loop exits)
-__device__ __forceinline__ bool lds_contains(const uint32_t* a, uint32_t n, uint32_t x) {
-  if (n == 0) return false;
-  uint32_t b = 0;
-  for (uint32_t m = n; m > 1;) {
-    const uint32_t half = m >> 1;
-    b = a[b + half] <= x ? b + half : b;
-    m -= half;
-  }
-  return a[b] == x;
-}
 
 // One wavefront runs one depth-2 chunk of a seed at a time: an iterative DFS
 // over positions 2..k-2 whose per-position candidate buffers (one wave-width
