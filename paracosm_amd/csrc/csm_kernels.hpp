@@ -428,9 +428,14 @@ __global__ __launch_bounds__(BS) void k_compact(uint2* __restrict__ vrec, uint32
 }
 
 // Merge each touched vertex's sorted new entries into a fresh copy of its
-// (clean) list.  One workgroup per run.  New entries carry their update
-// position as `ins`; the list is flagged DIRTY and recorded for compaction at
-// the start of the next batch.
+// (clean) list.  Runs are taken in groups of MERGE_GROUP.  The lanes of wave 0
+// size the group's runs, and one atomic per counter reserves the pool space,
+// the live count and the dirty-list slots of the whole group: one single-lane
+// atomic per run and counter serialises on the one line of Ctrl.  Then the
+// workgroup merges the group's runs one after another.  New entries carry
+// their update position as `ins`; the list is flagged DIRTY and recorded for
+// compaction at the start of the next batch.
+constexpr uint32_t MERGE_GROUP = 16;
 template <int BS>
 __global__ __launch_bounds__(BS) void k_merge(uint2* __restrict__ vrec, uint32_t* __restrict__ adj,
                                               uint32_t* __restrict__ ats,
@@ -439,30 +444,65 @@ __global__ __launch_bounds__(BS) void k_merge(uint2* __restrict__ vrec, uint32_t
                                               const uint32_t* __restrict__ run_start,
                                               Ctrl* __restrict__ ctrl, uint32_t* __restrict__ dirty_next,
                                               uint32_t pool_cap) {
-  __shared__ uint32_t s_off;
-  __shared__ uint32_t s_b;
+  static_assert(BS % WAVE == 0 && MERGE_GROUP <= WAVE, "group sized by wave 0");
+  __shared__ uint32_t s_v[MERGE_GROUP], s_bs[MERGE_GROUP], s_b[MERGE_GROUP];
+  __shared__ uint32_t s_off[MERGE_GROUP], s_a[MERGE_GROUP], s_noff[MERGE_GROUP];
+  __shared__ uint32_t s_slot[MERGE_GROUP];
   const uint32_t nr = ctrl->nruns;
-  for (uint32_t it = blockIdx.x; it < nr; it += gridDim.x) {
-    uint32_t bs = run_start[it];
-    uint32_t v = (uint32_t)(keys[bs] >> 32);
-    if (threadIdx.x == 0) {
-      uint32_t e = bs + 1;
-      while (e < nkeys && keys[e] != ~0ull && (uint32_t)(keys[e] >> 32) == v) ++e;
-      s_b = e - bs;
+  for (uint32_t g0 = blockIdx.x * MERGE_GROUP; g0 < nr; g0 += gridDim.x * MERGE_GROUP) {
+    const uint32_t cnt = min(MERGE_GROUP, nr - g0);
+    if (threadIdx.x < WAVE) {
+      const uint32_t t = threadIdx.x;
+      uint32_t v = 0, bs = 0, b = 0, off = 0, a = 0, fresh = 0;
+      if (t < cnt) {
+        bs = run_start[g0 + t];
+        v = (uint32_t)(keys[bs] >> 32);
+        b = run_end(keys, bs, nkeys, v) - bs;
+        const uint2 r = vrec[v];
+        off = r.x;
+        a = r.y & LEN_MASK;
+        fresh = (r.y & DIRTY) ? 0u : 1u;
+      }
+      // inclusive prefix sums over lanes 0 .. MERGE_GROUP-1
+      const unsigned long long ab = (unsigned long long)a + b;
+      unsigned long long iab = ab, ib = b;
+      uint32_t ifr = fresh;
+#pragma unroll
+      for (uint32_t o = 1; o < MERGE_GROUP; o <<= 1) {
+        const unsigned long long yab = __shfl_up(iab, o), yb = __shfl_up(ib, o);
+        const uint32_t yfr = __shfl_up(ifr, o);
+        if (t >= o) { iab += yab; ib += yb; ifr += yfr; }
+      }
+      const unsigned long long tot_ab = __shfl(iab, MERGE_GROUP - 1);
+      const unsigned long long tot_b = __shfl(ib, MERGE_GROUP - 1);
+      const uint32_t tot_fr = __shfl(ifr, MERGE_GROUP - 1);
+      uint32_t o = NONE, od = 0;
+      if (t == 0) {
+        if (tot_ab <= pool_cap) {
+          o = atomicAdd(&ctrl->pool_top, (uint32_t)tot_ab);
+          if ((uint64_t)o + tot_ab > pool_cap) o = NONE;
+        }
+        if (o == NONE) atomicOr(&ctrl->err, ERR_POOL_OVERFLOW);
+        else if (tot_fr) od = atomicAdd(&ctrl->ndirty_next, tot_fr);
+        atomicAdd(&ctrl->live, tot_b);
+      }
+      o = __shfl(o, 0);
+      od = __shfl(od, 0);
+      if (t < cnt) {
+        s_v[t] = v;
+        s_bs[t] = bs;
+        s_b[t] = b;
+        s_off[t] = off;
+        s_a[t] = a;
+        s_noff[t] = o == NONE ? NONE : o + (uint32_t)(iab - ab);
+        s_slot[t] = (o != NONE && fresh) ? od + (ifr - fresh) : NONE;
+      }
     }
     __syncthreads();
-    uint32_t b = s_b;
-    uint2 r = vrec[v];
-    uint32_t off = r.x, a = r.y & LEN_MASK;
-    if (threadIdx.x == 0) {
-      uint32_t o = atomicAdd(&ctrl->pool_top, a + b);
-      if ((uint64_t)o + a + b > pool_cap) { atomicOr(&ctrl->err, ERR_POOL_OVERFLOW); o = NONE; }
-      s_off = o;
-      atomicAdd(&ctrl->live, (unsigned long long)b);
-    }
-    __syncthreads();
-    uint32_t noff = s_off;
-    if (noff != NONE) {
+    for (uint32_t q = 0; q < cnt; ++q) {
+      const uint32_t noff = s_noff[q];
+      if (noff == NONE) continue;
+      const uint32_t v = s_v[q], bs = s_bs[q], b = s_b[q], off = s_off[q], a = s_a[q];
       // old entries: out = i + #(new < old[i])
       for (uint32_t i = threadIdx.x; i < a; i += BS) {
         uint32_t x = adj[off + i];
@@ -489,10 +529,11 @@ __global__ __launch_bounds__(BS) void k_merge(uint2* __restrict__ vrec, uint32_t
         adj[noff + j + lo] = x;
         ats[noff + j + lo] = (vals[bs + j] << 16) | 0xFFFFu;
       }
+      // v's record is read only in the group pass above, so it can be
+      // rewritten while other waves still merge this run
       if (threadIdx.x == 0) {
-        uint32_t old = r.y;
         vrec[v] = make_uint2(noff, (a + b) | DIRTY);
-        if (!(old & DIRTY)) dirty_next[atomicAdd(&ctrl->ndirty_next, 1u)] = v;
+        if (s_slot[q] != NONE) dirty_next[s_slot[q]] = v;
       }
     }
     __syncthreads();

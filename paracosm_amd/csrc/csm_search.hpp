@@ -98,4 +98,30 @@ __device__ __forceinline__ bool lds_contains(const uint32_t* a, uint32_t n, uint
   return a[b] == x;
 }
 
+// Runs of the sorted insertion keys (vertex << 32 | entry, padded with ~0):
+// end of the run of vertex v that starts at bs, i.e. the first index after
+// bs whose key is not v's (a larger vertex or the padding), or n.
+// Galloping, so a run of b keys costs O(log b) loads.
+__device__ __forceinline__ uint32_t run_end(const uint64_t* __restrict__ keys, uint32_t bs,
+                                            uint32_t n, uint32_t v) {
+  const uint64_t bound = ((uint64_t)v + 1) << 32;
+  uint32_t lo = bs + 1;  // every index below lo is in the run
+  uint32_t hi = n;       // n, or an index known to be past the run
+  for (uint64_t step = 1;; step <<= 1) {
+    const uint64_t probe = lo + step - 1;
+    if (probe >= n) break;
+    if (keys[probe] >= bound) {
+      hi = (uint32_t)probe;
+      break;
+    }
+    lo = (uint32_t)probe + 1;
+  }
+  while (lo < hi) {
+    const uint32_t m = lo + ((hi - lo) >> 1);
+    if (keys[m] >= bound) hi = m;
+    else lo = m + 1;
+  }
+  return lo;
+}
+
 }  // namespace csm

@@ -43,6 +43,28 @@ int main() {
       cases += 4;
     }
   }
+  // run ends of sorted insertion keys (vertex << 32 | entry, ~0 padding)
+  // against a scan key by key from the run start
+  for (int it = 0; it < 20000; ++it) {
+    const uint32_t n = rng() % 400, pad = rng() % 4, nv = 1 + rng() % 40;
+    const uint32_t top = (it & 1) ? 0xFFFFFFFEu : nv;  // also vertices next to ~0's
+    std::vector<uint64_t> keys(n);
+    for (auto& k : keys) {
+      const uint32_t v = top - rng() % nv;
+      k = ((uint64_t)v << 32) | (rng() & 0xFFFFFFFFu);
+    }
+    std::sort(keys.begin(), keys.end());
+    for (uint32_t i = 0; i < pad; ++i) keys.push_back(~0ull);
+    const uint32_t m = (uint32_t)keys.size();
+    for (uint32_t bs = 0; bs < m; ++bs) {
+      if (keys[bs] == ~0ull || (bs > 0 && (keys[bs - 1] >> 32) == (keys[bs] >> 32))) continue;
+      const uint32_t v = (uint32_t)(keys[bs] >> 32);
+      uint32_t e = bs + 1;
+      while (e < m && keys[e] != ~0ull && (uint32_t)(keys[e] >> 32) == v) ++e;
+      bad += csm::run_end(keys.data(), bs, m, v) != e;
+      ++cases;
+    }
+  }
   std::printf("cases %ld bad %ld\n", cases, bad);
   return bad != 0;
 }
