@@ -1486,7 +1486,10 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   }
 
   int rc = CSM_OK;
-  e->pool_cap = std::max<uint64_t>(4 * 2 * data->num_edges + 8ull * e->max_batch, 1u << 20);
+  // 8x the initial directed entries: a batch may need room for twice the
+  // live entries (every list rewritten by compaction and merge), so a pool
+  // of 4x left room for about one batch of hub rewrites between repacks
+  e->pool_cap = std::max<uint64_t>(8 * 2 * data->num_edges + 8ull * e->max_batch, 1u << 20);
   if (e->pool_cap > 0xFFFFFFF0ull) e->pool_cap = 0xFFFFFFF0ull;
   if ((rc = dalloc(e.get(), &e->d_vrec, e->nv_cap)) || (rc = dalloc(e.get(), &e->d_vlab, e->nv_cap)) ||
       (rc = dalloc(e.get(), &e->d_adj[0], e->pool_cap)) ||
