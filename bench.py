@@ -7,12 +7,15 @@ insertions of held-out edges, 20 % deletions of present edges); 10 standing
 8-vertex random-walk queries (5 sparse, 5 dense) admitted by the exact-count
 rule of paracosm_amd.workloads.  One step = one batch of updates through the
 whole device pipeline (graph update, seeds, search) for all queries; the
-update records are resident in HBM before the timed region starts.
+update records are resident in HBM before the timed region starts.  The
+batch size is cut so that the warm-up and timed batches fit in the stream.
 
 Prints one JSON line: batched updates/s (value) beside the plain per-update
-GPU path (one update per launch sequence), per-update latency percentiles,
-the search kernel's roofline figures, and the CPU oracle's parallel GraphFlow
-restatement timed on a bounded sample on this host's cores.
+GPU path (one update per launch sequence), the served path (fresh engines,
+Poisson arrivals at several offered loads, p50/p99 per-update latency, counts
+checked against the batched run), the search kernel's roofline figures, and
+the CPU oracle's parallel GraphFlow restatement timed on a bounded sample on
+this host's cores.
 """
 from __future__ import annotations
 
@@ -69,7 +72,9 @@ def main() -> int:
     # search settings (a settling batch, then one batch searched once per
     # candidate)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=62500)
+    # 40,000 x 25 batches = the 10^6-update stream: the same batch size for
+    # every warm-up + steps count up to 25
+    ap.add_argument("--batch", type=int, default=40000)
     ap.add_argument("--algo", default="parallel_graphflow")
     ap.add_argument("--config", default="c2")
     ap.add_argument("--seed", type=int, default=1)
@@ -78,10 +83,16 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=6.0)
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="wall-clock cap per served offered load")
+    ap.add_argument("--serve-loads", default="1e4,1e5,1e6,sat",
+                    help="offered loads in updates/s; 'sat' = 2x the batched rate")
     ap.add_argument("--split-chunk", type=int, default=0)
     ap.add_argument("--spill-budget", type=int, default=0)
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--hbm-budget", type=int, default=0,
+                    help="served legs: device bytes the engine plus in-flight batches may "
+                         "hold (0 = 90%% of the free HBM at start)")
     ap.add_argument("--also", default="parallel_symbi,parallel_turboflux",
                     help="other algorithms timed on the same batches (comma list, '' for none)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -107,8 +118,21 @@ def main() -> int:
     log(f"admitted {len(queries)} queries in {time.time() - t0:.1f}s: "
         f"k={[q.k for q in queries]} edges={[q.num_edges for q in queries]}")
 
+    # every warm-up and timed batch must come from the stream: cut the batch
+    # so that they fit, and refuse before any GPU work if even 1 does not
+    n_batches = args.warmup + args.steps
+    batch = min(args.batch, w.num_updates // max(n_batches, 1))
+    if batch < 1 or args.steps < 1:
+        print(json.dumps({"error": f"{n_batches} batches do not fit in the "
+                                   f"{w.num_updates}-update stream"}))
+        return 2
+    if batch != args.batch:
+        log(f"batch cut from {args.batch} to {batch} so that {n_batches} batches fit in the "
+            f"{w.num_updates}-update stream")
+    args.batch = batch
+
     # plain per-update GPU path on the stream's first updates (fresh engine)
-    P = min(args.plain_sample, (args.warmup + args.steps) * args.batch, w.num_updates)
+    P = min(args.plain_sample, n_batches * args.batch, w.num_updates)
     mp = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab)
     for q in queries[1:]:
         mp.register_query(q)
@@ -130,7 +154,7 @@ def main() -> int:
     plain_ups = P / plain_elapsed
     log(f"plain per-update: {P} updates in {plain_elapsed:.3f}s = {plain_ups:.0f} updates/s")
 
-    n_total = min(w.num_updates, (args.warmup + args.steps) * args.batch)
+    n_total = n_batches * args.batch
     ups = make_updates(w.upd_add[:n_total], w.upd_v1[:n_total], w.upd_v2[:n_total],
                        w.upd_label[:n_total])
     nq = len(queries)
@@ -153,9 +177,8 @@ def main() -> int:
 
     def step(i):
         s = i * args.batch
-        n = min(args.batch, n_total - s)
-        m.apply_batch_device(d_ups.ptr + s * rec, n, d_counts.ptr + s * nq * 8)
-        return n
+        m.apply_batch_device(d_ups.ptr + s * rec, args.batch, d_counts.ptr + s * nq * 8)
+        return args.batch
 
     for i in range(args.warmup):
         tw = time.perf_counter()
@@ -185,31 +208,11 @@ def main() -> int:
     exact = bool(np.array_equal(pcounts, counts[:P].sum(axis=1)))
     log(f"plain per-update counts equal to batched: {exact}")
 
-    # served path: the same engine continues the stream, requests of 1..4096
-    # updates arriving as a Poisson process, continuous batching
+    # the search settings the autotuner chose, reused by the served engines
+    tuned = (int(st1["split_chunk"]), int(st1["spill_budget"]))
     served = []
     if not args.no_serve:
-        from paracosm_amd.serve import Server, poisson_requests
-        srv = Server(m, max_batch=args.batch)
-        cursor = n_total
-        for load in (10.0 * plain_ups, 0.8 * value):
-            n_srv = int(min(args.serve_seconds * load, w.num_updates - cursor))
-            if n_srv <= 0:
-                break
-            sl = slice(cursor, cursor + n_srv)
-            reqs = poisson_requests(make_updates(w.upd_add[sl], w.upd_v1[sl], w.upd_v2[sl],
-                                                 w.upd_label[sl]), load, seed=args.seed)
-            st = srv.serve(reqs)
-            cursor += n_srv
-            served.append({"offered_updates_per_s": round(load, 1),
-                           "updates_per_s": round(st.updates_per_s, 1),
-                           "p50_update_latency_ms": round(st.p50_ms, 3),
-                           "p99_update_latency_ms": round(st.p99_ms, 3),
-                           "requests": st.requests, "updates": st.updates,
-                           "batches": st.batches, "mean_batch": round(st.mean_batch, 1),
-                           "matches_per_s": round(st.matches / st.elapsed, 1)})
-            log(f"served at {load:.0f}/s offered: {served[-1]}")
-        srv.close()
+        served = serve_legs(w, queries, args, value, tuned, counts, n_total)
     ingest = None if args.no_ingest else ingest_rate(m, w)
     m.close()
 
@@ -238,6 +241,7 @@ def main() -> int:
             "alg_bytes_per_launch": int(per_launch_bytes), "launches": int(launches),
             "kernel_share_of_step": round(k_ns / 1e9 / elapsed, 4) if elapsed else None}
 
+    sat = next((x for x in served if x.get("saturating")), None)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, queries, args.cpu_seconds)
@@ -258,12 +262,16 @@ def main() -> int:
         "config": {"workload": d.cfg.description, "config": args.config, "seed": args.seed,
                    "algo": args.algo, "batch": args.batch, "queries": nq,
                    "query_fingerprint": workloads.query_fingerprint(queries),
+                   "admission": workloads.ADMISSION_RULE,
                    "parallelism": "single-gpu"},
         "p50_update_latency_ms": round(1e3 * pct(lat, 50), 3),
         "p99_update_latency_ms": round(1e3 * pct(lat, 99), 3),
         "latency_note": "closed loop: each update's latency is its batch's latency",
         "served": served,
-        "served_vs_plain": (round(served[0]["updates_per_s"] / plain_ups, 2) if served else None),
+        # the served rate at a saturating offered load (above the batched
+        # rate) over the plain per-update path
+        "served_vs_plain": (round(sat["updates_per_s"] / plain_ups, 2) if sat else None),
+        "served_saturated_p99_update_latency_ms": (sat["p99_update_latency_ms"] if sat else None),
         "plain_updates_per_s": round(plain_ups, 1),
         "plain_p50_update_latency_ms": round(1e3 * pct(plat, 50), 4),
         "plain_p99_update_latency_ms": round(1e3 * pct(plat, 99), 4),
@@ -278,6 +286,52 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def serve_legs(w, queries, args, batched_rate, tuned, batched_counts, n_batched):
+    """The served path at several offered loads.  Each load gets a fresh
+    engine on the initial graph and serves the stream from its first update
+    (requests of 1..4096 updates, Poisson arrivals, continuous batching) for
+    at most --serve-seconds of arrivals; every served count is compared with
+    the batched run's count of the same update and query."""
+    from paracosm_amd.engine import Matching, make_updates
+    from paracosm_amd.serve import Server, poisson_requests
+    out = []
+    for tok in [t for t in args.serve_loads.split(",") if t]:
+        saturating = tok == "sat"
+        load = 2.0 * batched_rate if saturating else float(tok)
+        n_srv = int(min(args.serve_seconds * load, w.num_updates))
+        if n_srv < 1:
+            continue
+        m = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                     max_batch=args.batch, split_chunk=tuned[0], spill_budget=tuned[1])
+        for q in queries[1:]:
+            m.register_query(q)
+        m.Preprocessing()
+        sl = slice(0, n_srv)
+        reqs = poisson_requests(make_updates(w.upd_add[sl], w.upd_v1[sl], w.upd_v2[sl],
+                                             w.upd_label[sl]), load, seed=args.seed)
+        srv = Server(m, max_batch=args.batch, hbm_budget=args.hbm_budget)
+        st = srv.serve(reqs)
+        srv.close()
+        m.close()
+        got = np.concatenate([r.counts for r in reqs])
+        k = min(n_srv, n_batched)
+        exact = bool(np.array_equal(got[:k], batched_counts[:k]))
+        out.append({"offered_updates_per_s": round(load, 1),
+                    "updates_per_s": round(st.updates_per_s, 1),
+                    "achieved_over_offered": round(st.updates_per_s / load, 3),
+                    "saturating": saturating,
+                    "p50_update_latency_ms": round(st.p50_ms, 3),
+                    "p99_update_latency_ms": round(st.p99_ms, 3),
+                    "requests": st.requests, "updates": st.updates,
+                    "batches": st.batches, "mean_batch": round(st.mean_batch, 1),
+                    "matches_per_s": round(st.matches / st.elapsed, 1),
+                    "hbm_budget": args.hbm_budget,
+                    "refused_for_memory": st.refused_for_memory,
+                    "counts_equal_batched": exact, "compared_updates": k})
+        log(f"served at {load:.0f}/s offered: {out[-1]}")
+    return out
 
 
 def batched_only(algo, w, queries, ups, args):
@@ -298,9 +352,8 @@ def batched_only(algo, w, queries, ups, args):
 
     def step(i):
         s = i * args.batch
-        n = min(args.batch, n_total - s)
-        m.apply_batch_device(d_ups.ptr + s * 16, n, d_counts.ptr + s * nq * 8)
-        return n
+        m.apply_batch_device(d_ups.ptr + s * 16, args.batch, d_counts.ptr + s * nq * 8)
+        return args.batch
 
     for i in range(args.warmup):
         step(i)
@@ -351,12 +404,28 @@ def ingest_rate(m, w, reps=5):
     return out
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, queries, seconds):
     """The oracle's restatement of the reference's parallel GraphFlow
-    (serial depth-2 expansion, depth-3 task vector, OpenMP loop), all cores,
-    on the first S updates of the same stream for every query."""
+    (serial depth-2 expansion, depth-3 task vector, OpenMP loop) on the first
+    S updates of the same stream for every query, S grown until the sample
+    takes about `seconds`.  Threads: the host CPUs this process may use,
+    capped by OMP_NUM_THREADS where the host sets it (the GPU host's CPU
+    share for one GPU)."""
     from oracle import oracle as O
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    avail = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    cores = min(avail, int(env)) if env.isdigit() and int(env) > 0 else avail
     t0 = time.time()
     dg = O.OGraph(w.vlabels, w.init_src, w.init_dst, w.init_elab)
     log(f"cpu baseline: oracle graph built in {time.time() - t0:.1f}s")
@@ -371,9 +440,14 @@ def cpu_baseline(w, queries, seconds):
         log(f"cpu baseline: {s} updates x {len(qgs)} queries took {tot:.2f}s")
         if tot * 3 >= seconds or s >= w.num_updates:
             break
-        s = int(min(w.num_updates, s * max(2.0, 0.8 * seconds / max(tot, 1e-3))))
-    log(f"cpu baseline: {s} updates x {len(qgs)} queries in {tot:.2f}s on {cores} threads")
+        # at most 3x per step: the cost per update varies along the stream
+        s = int(min(w.num_updates, s * min(3.0, max(1.5, 0.8 * seconds / max(tot, 1e-3)))))
+    model = cpu_model()
+    log(f"cpu baseline: {s} updates x {len(qgs)} queries in {tot:.2f}s on {cores} threads "
+        f"({model}, {avail} CPUs available)")
     return {"value": round(s / tot, 1), "unit": "updates/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "cpus_available": avail,
+            "omp_num_threads_env": env or None,
             "sample": f"first {s} updates of the stream, all {len(qgs)} queries, "
                       f"oracle parallel_graphflow, {cores} OpenMP threads"}
 

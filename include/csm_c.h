@@ -198,12 +198,15 @@ void* csm_stream(csm_engine* e);
 /* Text ingestion on the device.  Parses the reference's input formats, one
  * record per line, into csm_update records:
  *   CSM_PARSE_GRAPH  - query / data graph files (Graph::LoadFromFile,
- *                      graph.cpp:217-249): "t ..." headers skipped,
- *                      "v id label" -> type 'v', "e a b label" -> type 'e'
+ *                      graph.cpp:217-249): the type is the line's first
+ *                      char; "t ..." headers skipped, "v id label" -> type
+ *                      'v', any other char "x a b label" -> type 'e'
  *                      (is_add = 1);
  *   CSM_PARSE_STREAM - update streams (Graph::LoadUpdateStream,
- *                      graph.cpp:251-285): "v"/"-v id label",
- *                      "e"/"-e a b label" (is_add = 0 for '-').
+ *                      graph.cpp:251-285): the type is the first token;
+ *                      "v"/"-v id label" -> type 'v' (is_add = token "v"),
+ *                      any other token "tok a b label" -> type 'e' with
+ *                      is_add = (tok == "e").
  * Blank lines are skipped; ids and labels are decimal uint32.  Returns
  * CSM_E_ARG for a malformed line (its 1-based number in *err_line) and
  * CSM_E_CAPACITY when more than `cap` records result (*n holds the count,

@@ -1,9 +1,12 @@
 // Device parsing of the reference's text formats: graph files
 // (Graph::LoadFromFile, ParaCOSM/CSM/graph/graph.cpp:217-249: "t" header
-// lines, "v id label", "e a b label") and update streams
-// (Graph::LoadUpdateStream, graph.cpp:251-285: "v"/"-v id label",
-// "e"/"-e a b label"), one record per line, into csm_update records
-// (InsertUnit layout, utils/types.h:34-42).
+// lines, "v id label", and any other leading char an edge "e a b label") and
+// update streams (Graph::LoadUpdateStream, graph.cpp:251-285: "v"/"-v id
+// label", and any other leading token an edge "e a b label" with is_add =
+// (token == "e")), one record per line, into csm_update records (InsertUnit
+// layout, utils/types.h:34-42).  The reference reads tokens across line
+// breaks and stops silently at the first field that is not a number; here a
+// record is one line and such a line is an error that names its number.
 //
 // Three passes over the bytes in HBM: newlines counted per 4 KiB tile, their
 // positions written after a scan of the tile counts, then one thread per line
@@ -110,33 +113,36 @@ __global__ void k_parse_records(const uint8_t* __restrict__ text, uint64_t len,
     uint64_t p = s;
     while (!bad && p < e && is_space(text[p])) ++p;
     if (!bad && p < e) {
-      // type token: one letter, optionally after '-'
-      bool neg = false;
-      if (text[p] == '-') { neg = true; ++p; }
-      const uint8_t t = p < e ? text[p++] : 0;
-      if (p < e && !is_space(text[p])) bad = true;
       uint32_t need = 0;
-      if (!bad) {
-        if (mode == PARSE_GRAPH) {
-          if (neg) bad = true;
-          else if (t == 't') need = NONE;         // header: ignored
-          else if (t == 'v') { r.type = 'v'; need = 2; }
-          else if (t == 'e') { r.type = 'e'; need = 3; }
-          else bad = true;
-          r.add = 1;
-        } else {
-          if (t == 'v') { r.type = 'v'; need = 2; }
-          else if (t == 'e') { r.type = 'e'; need = 3; }
-          else bad = true;
-          r.add = neg ? 0 : 1;
-        }
+      bool first_glued = false;   // the first field may follow the type without a space
+      if (mode == PARSE_GRAPH) {
+        // LoadFromFile reads the type as one char (`ifs >> type`, graph.cpp:226):
+        // 't' is a header, 'v' a vertex, and any other char an edge
+        // (graph.cpp:228-246); the next field starts right after that char
+        const uint8_t t = text[p++];
+        first_glued = true;
+        r.add = 1;
+        if (t == 't') need = NONE;
+        else if (t == 'v') { r.type = 'v'; need = 2; }
+        else { r.type = 'e'; need = 3; }
+      } else {
+        // LoadUpdateStream reads a whitespace-delimited token (graph.cpp:263-281):
+        // "v" / "-v" are vertex records, any other token is an edge record
+        // with is_add = (token == "e")
+        const uint64_t t0 = p;
+        while (p < e && !is_space(text[p])) ++p;
+        const uint64_t tl = p - t0;
+        const bool is_v = (tl == 1 && text[t0] == 'v') ||
+                          (tl == 2 && text[t0] == '-' && text[t0 + 1] == 'v');
+        if (is_v) { r.type = 'v'; need = 2; r.add = tl == 1; }
+        else { r.type = 'e'; need = 3; r.add = tl == 1 && text[t0] == 'e'; }
       }
       if (!bad && need != NONE) {
         uint32_t val[3] = {0, 0, 0};
         for (uint32_t f = 0; f < need && !bad; ++f) {
           const uint64_t q0 = p;
           while (p < e && is_space(text[p])) ++p;
-          if (p == q0 || p == e) { bad = true; break; }
+          if ((p == q0 && !(f == 0 && first_glued)) || p == e) { bad = true; break; }
           uint64_t x = 0;
           const uint64_t d0 = p;
           while (p < e && text[p] >= '0' && text[p] <= '9') {

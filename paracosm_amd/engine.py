@@ -314,6 +314,8 @@ class Matching:
         """Device pointers (e.g. DeviceBuffer.ptr) in HBM; results
         stay on the device.  Synchronous with respect to the engine's stream's
         control read-back."""
+        if n < 0:
+            raise ValueError(f"negative batch size {n}")
         self._check(lib().csm_apply_batch_device(self.h, d_updates_ptr, n, d_counts_ptr,
                                                  d_safe_ptr or None, d_hash_ptr or None))
 

@@ -89,7 +89,7 @@ class Server:
         self.refused = 0
 
     def close(self) -> None:
-        for b in (self.h_upd, self.h_cnt, self.h_safe):
+        for b in (self.h_upd, self.h_cnt, self.h_safe, self.d_upd, self.d_cnt, self.d_safe):
             b.close()
 
     def working_set(self, n: int) -> int:

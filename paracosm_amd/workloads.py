@@ -91,7 +91,15 @@ def candidates(d: Data, k: int, density: Optional[str], count: int, round_: int 
                               density)
 
 
-def admit(d: Data, *, sample: int = 4096, budget_per_update: float = 3e3,
+ADMISSION_SAMPLE = 4096
+ADMISSION_BUDGET_PER_UPDATE = 3000
+ADMISSION_RULE = (f"a candidate query is kept only if its exact count over the stream's first "
+                  f"{ADMISSION_SAMPLE} updates is at most {ADMISSION_BUDGET_PER_UPDATE} per "
+                  f"update; candidates in seeded order, first admitted kept")
+
+
+def admit(d: Data, *, sample: int = ADMISSION_SAMPLE,
+          budget_per_update: float = ADMISSION_BUDGET_PER_UPDATE,
           algo: str = "parallel_graphflow", max_rounds: int = 16, log=print) -> List[gen.Query]:
     """Admitted queries for every (size, density) slot of the config."""
     from .engine import Matching, make_updates

@@ -11,29 +11,36 @@ U32MAX = 4294967295
 
 
 def py_parse(text: bytes, graph: bool):
-    """(type, is_add, id1, id2, label) per record, or the first bad line."""
+    """(type, is_add, id1, id2, label) per record, or the first bad line.
+    Graph files take the type from the line's first char ('t' header, 'v'
+    vertex, anything else an edge; graph.cpp:226-246) and the first field may
+    follow it without a space; streams take a whole token ("v"/"-v" vertex,
+    anything else an edge with is_add = token == "e"; graph.cpp:263-281)."""
     out = []
     for no, raw in enumerate(text.split(b"\n"), 1):
-        toks = raw.replace(b"\r", b" ").replace(b"\t", b" ").split()
-        if not toks:
+        line = raw.replace(b"\r", b" ").replace(b"\t", b" ")
+        body = line.lstrip(b" ")
+        if not body:
             continue
-        t = toks[0]
-        if graph and t == b"t":
-            continue
-        if t in (b"v", b"-v"):
-            need, typ = 2, ord("v")
-        elif t in (b"e", b"-e"):
-            need, typ = 3, ord("e")
+        if graph:
+            t, rest = body[:1], body[1:]
+            if t == b"t":
+                continue
+            typ, add = (ord("v"), 1) if t == b"v" else (ord("e"), 1)
+            fields = rest.split()
         else:
+            toks = body.split()
+            t, fields = toks[0], toks[1:]
+            if t in (b"v", b"-v"):
+                typ, add = ord("v"), int(t == b"v")
+            else:
+                typ, add = ord("e"), int(t == b"e")
+        need = 2 if typ == ord("v") else 3
+        if len(fields) != need or not all(x.isdigit() for x in fields):
             return no
-        if graph and t.startswith(b"-"):
-            return no
-        if len(toks) != 1 + need or not all(x.isdigit() for x in toks[1:]):
-            return no
-        vals = [int(x) for x in toks[1:]]
+        vals = [int(x) for x in fields]
         if any(v > U32MAX for v in vals):
             return no
-        add = 0 if t.startswith(b"-") else 1
         if need == 2:
             out.append((typ, add, vals[0], 0, vals[1]))
         else:
@@ -57,7 +64,10 @@ def random_text(rng, n, graph):
             tok = "v" if graph or rng.random() < 0.5 else "-v"
             s = f"{tok}{sp}{rng.integers(0, 1 << 20)}{sp}{rng.integers(0, 64)}"
         else:
-            tok = "e" if graph or rng.random() < 0.7 else "-e"
+            if graph:
+                tok = "e" if rng.random() < 0.9 else "x"
+            else:
+                tok = "e" if rng.random() < 0.6 else ("-e" if rng.random() < 0.8 else "ev")
             s = f"{tok}{sp}{rng.integers(0, 1 << 32)}{sp}{rng.integers(0, 1 << 20)}{sp}{rng.integers(0, 4)}"
         if rng.random() < 0.05:
             s = "\t" + s + " \r"
@@ -83,19 +93,35 @@ def test_parse_matches_restatement(graph, seed):
 
 
 @pytest.mark.parametrize("case,line", [
-    (b"e 1 2 3\nx 1 2 3\n", 2),             # unknown token
     (b"e 1 2\n", 1),                        # missing field
     (b"v 1 2 3\n", 1),                      # extra field
     (b"e 1 2 4294967296\n", 1),             # overflow
     (b"e 1 2 3\n\ne 1 2a 3\n", 3),          # non-digit
-    (b"e 1 2 3\nev 1 2 3\n", 2),            # glued token
-    (b"e 1 2 3\n-t 1\n", 2),
+    (b"e 1 2 3\n-t 1\n", 2),                # an edge record split over lines
 ])
 def test_parse_reports_malformed_line(case, line):
     from paracosm_amd.engine import PARSE_STREAM, ParseError, parse_text
     with pytest.raises(ParseError) as ei:
         parse_text(case, PARSE_STREAM)
     assert ei.value.line == line
+
+
+def test_unknown_tokens_are_edges():
+    """Any token other than v/-v is an edge record, an insertion only for
+    "e" (LoadUpdateStream, graph.cpp:276-281); in graph files any leading
+    char other than t/v is an edge (LoadFromFile, graph.cpp:241-246)."""
+    from paracosm_amd.engine import PARSE_GRAPH, PARSE_STREAM, ParseError, parse_text
+    E = ord("e")
+    assert as_tuples(parse_text(b"x 1 2 3\nev 4 5 6\n-t 7 8 9\ne 1 2 0\n+e 3 4 1",
+                                PARSE_STREAM)) == [
+        (E, 0, 1, 2, 3), (E, 0, 4, 5, 6), (E, 0, 7, 8, 9), (E, 1, 1, 2, 0), (E, 0, 3, 4, 1)]
+    assert as_tuples(parse_text(b"t # 2\nx 1 2 3\nv0 5\ne1 2 3\n", PARSE_GRAPH)) == [
+        (E, 1, 1, 2, 3), (ord("v"), 1, 0, 0, 5), (E, 1, 1, 2, 3)]
+    # the reference stops reading where a field is not a number; here that
+    # line is an error
+    with pytest.raises(ParseError) as ei:
+        parse_text(b"e 1 2 3\n-e 1 2 3\n", PARSE_GRAPH)
+    assert ei.value.line == 2
 
 
 def test_parse_edge_cases():

@@ -1,0 +1,202 @@
+"""Pins the oracle's index and planning functions from their definitions.
+
+The SymBi/TurboFlux index (d1, d2, np1, nc2, n1, n2; SymBi::BuildDCS,
+ParaCOSM/CSM/matching/SymBi/symbi.cpp:246-377) and the SymBi DAG
+(SymBi::BuildDAG, symbi.cpp:113-244) are recomputed here in plain Python from
+the set definitions, with no incremental rule and no code shared with the
+oracle, and compared element by element with orc_index_dump / orc_dag_parents
+on small random graphs.  The GraphFlow matching orders
+(Graphflow::GenerateMatchingOrder, graphflow.cpp:36-160) are checked on a
+hand-worked query whose orders depend on both tie rules.
+"""
+import numpy as np
+import pytest
+
+from paracosm_amd.gen import Query
+from tests.helpers import random_connected_query, random_graph
+
+
+def _adj(nv, src, dst, el):
+    """Per vertex: sorted list of (neighbour, edge label)."""
+    a = [[] for _ in range(nv)]
+    for s, d, l in zip(src.tolist(), dst.tolist(), el.tolist()):
+        a[s].append((d, l))
+        a[d].append((s, l))
+    return [sorted(x) for x in a]
+
+
+def _qadj(q):
+    return _adj(q.k, q.src, q.dst, q.elabels)
+
+
+def dag_from_definition(q, vl, adj):
+    """Root and parents (in neighbour-id order) of SymBi's BFS DAG.
+
+    Root: the endpoint of the query edge with the fewest label-matching
+    directed data edges (first such edge over u ascending, neighbours
+    ascending, counted from the smaller endpoint's side), the endpoint whose
+    label is rarer in the data, on equal frequency the first endpoint if its
+    degree is larger, else the second.  DAG: BFS from the root over
+    neighbours in id order; an edge points from the vertex dequeued first."""
+    qa = _qadj(q)
+    best, mv = None, None
+    for u in range(q.k):
+        for uo, ul in qa[u]:
+            if u > uo:
+                continue
+            f = sum(1 for v in range(len(vl)) if vl[v] == q.vlabels[u]
+                    for w, l in adj[v] if vl[w] == q.vlabels[uo] and l == ul)
+            if best is None or f < best:
+                best, mv = f, (u, uo)
+    f0 = int(np.sum(vl == q.vlabels[mv[0]]))
+    f1 = int(np.sum(vl == q.vlabels[mv[1]]))
+    if f0 != f1:
+        root = mv[1] if f0 > f1 else mv[0]
+    else:
+        root = mv[0] if len(qa[mv[0]]) > len(qa[mv[1]]) else mv[1]
+    popped, queued, fifo = set(), {root}, [root]
+    parents = [[] for _ in range(q.k)]
+    while fifo:
+        u = fifo.pop(0)
+        popped.add(u)
+        for uo, _ in qa[u]:
+            if uo not in queued:
+                queued.add(uo)
+                fifo.append(uo)
+            if uo in popped:
+                parents[u].append(uo)
+    return root, parents
+
+
+def index_from_definition(q, vl, adj, parents):
+    """The index over a DAG given by `parents`, from its definitions:
+
+    S(v, u->x)     = neighbours w of v with L(w) = L(x) and l(v,w) = l(u,x)
+    n1[u<-p][v]    = |{w in S(v, u->p) : d1[p][w]}|
+    np1[u][v]      = #parents p with n1[u<-p][v] > 0
+    d1[u][v]       = L(v) = L(u) and np1[u][v] = #parents(u)
+    n2[u->x][v]    = |{w in S(v, u->x) : d2[x][w]}| for x a parent or child
+    nc2[u][v]      = #children c with n2[u->c][v] > 0
+    d2[u][v]       = d1[u][v] and nc2[u][v] = #children(u)
+    with every counter 0 where L(v) != L(u)."""
+    k, nv = q.k, len(vl)
+    qa = _qadj(q)
+    lab = {(u, x): l for u in range(k) for x, l in qa[u]}
+    children = [[c for c in range(k) if u in parents[c]] for u in range(k)]
+    # topological order: repeatedly take vertices whose parents are placed
+    order, placed = [], set()
+    while len(order) < k:
+        for u in range(k):
+            if u not in placed and all(p in placed for p in parents[u]):
+                order.append(u)
+                placed.add(u)
+
+    def S(v, u, x):
+        return [w for w, l in adj[v] if vl[w] == q.vlabels[x] and l == lab[(u, x)]]
+
+    d1 = np.zeros((k, nv), np.uint32)
+    d2 = np.zeros((k, nv), np.uint32)
+    np1 = np.zeros((k, nv), np.uint32)
+    nc2 = np.zeros((k, nv), np.uint32)
+    n1 = {}
+    n2 = {}
+    for u in order:
+        for p in parents[u]:
+            n1[(u, p)] = np.zeros(nv, np.uint32)
+        for v in range(nv):
+            if vl[v] != q.vlabels[u]:
+                continue
+            for p in parents[u]:
+                n1[(u, p)][v] = sum(1 for w in S(v, u, p) if d1[p][w])
+                np1[u][v] += n1[(u, p)][v] > 0
+            d1[u][v] = np1[u][v] == len(parents[u])
+    for u in reversed(order):
+        for c in children[u]:
+            n2[(u, c)] = np.zeros(nv, np.uint32)
+        for v in range(nv):
+            if vl[v] != q.vlabels[u]:
+                continue
+            for c in children[u]:
+                n2[(u, c)][v] = sum(1 for w in S(v, u, c) if d2[c][w])
+                nc2[u][v] += n2[(u, c)][v] > 0
+            d2[u][v] = d1[u][v] and nc2[u][v] == len(children[u])
+    for u in range(k):
+        for p in parents[u]:
+            n2[(u, p)] = np.zeros(nv, np.uint32)
+            for v in range(nv):
+                if vl[v] == q.vlabels[u]:
+                    n2[(u, p)][v] = sum(1 for w in S(v, u, p) if d2[p][w])
+    n1_rows = np.asarray([n1[(u, p)] for u in range(k) for p in parents[u]]).reshape(-1, nv)
+    n2_rows = np.asarray([n2.get((u, x), np.zeros(nv, np.uint32))
+                          for u in range(k) for x, _ in qa[u]]).reshape(-1, nv)
+    return {1: d1, 2: d2, 3: np1, 4: nc2, 5: n1_rows, 6: n2_rows}
+
+
+def _case(seed):
+    rng = np.random.default_rng(9000 + seed)
+    nv = 12 + seed % 9
+    vl, s, d, el = random_graph(nv, 0.3 + 0.05 * (seed % 4), 2 + seed % 2, 1 + seed % 2, rng)
+    q = random_connected_query(3 + seed % 4, seed % 3, 2 + seed % 2, 1 + seed % 2, rng)
+    return vl, s, d, el, q
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_symbi_dag_from_definition(oracle_lib, seed):
+    O = oracle_lib
+    vl, s, d, el, q = _case(seed)
+    root, par = dag_from_definition(q, vl, _adj(len(vl), s, d, el))
+    got_root, got_par = O.dag_parents(O.OGraph(vl, s, d, el), O.OGraph.from_query(q), q.k,
+                                      tree=False)
+    assert (got_root, got_par) == (root, par)
+
+
+@pytest.mark.parametrize("tree", [False, True])
+@pytest.mark.parametrize("seed", range(12))
+def test_index_from_definition(oracle_lib, tree, seed):
+    """Every row of the oracle's from-scratch index equals the definition, on
+    SymBi's DAG and on TurboFlux's spanning tree (parents taken from the
+    oracle; the tree rule is pinned in test_oracle.py)."""
+    O = oracle_lib
+    vl, s, d, el, q = _case(seed)
+    dg, qg = O.OGraph(vl, s, d, el), O.OGraph.from_query(q)
+    root, par = O.dag_parents(dg, qg, q.k, tree=tree)
+    want = index_from_definition(q, vl, _adj(len(vl), s, d, el), par)
+    nv = len(vl)
+    assert int(O.index_dump(dg, qg, 0, nv, 1, tree=tree)[0]) == root
+    for what in range(1, 7):
+        rows = want[what].shape[0]
+        got = O.index_dump(dg, qg, what, nv, rows, tree=tree)
+        np.testing.assert_array_equal(got, want[what], err_msg=f"what={what}")
+
+
+def test_index_definition_is_not_trivial():
+    """The cases above include d1 without d2 and partial counters, so a
+    dropped term or a wrong parent/child role changes some row."""
+    d1_not_d2 = partial = 0
+    for seed in range(12):
+        vl, s, d, el, q = _case(seed)
+        root, par = dag_from_definition(q, vl, _adj(len(vl), s, d, el))
+        idx = index_from_definition(q, vl, _adj(len(vl), s, d, el), par)
+        d1_not_d2 += int(np.sum((idx[1] == 1) & (idx[2] == 0)))
+        partial += int(np.sum((idx[3] > 0) & (idx[1] == 0)))
+    assert d1_not_d2 > 10 and partial > 0
+
+
+def test_graphflow_orders_tie_rules(oracle_lib):
+    """Hand-worked orders (graphflow.cpp:36-160) for the query
+    0-1, 0-2, 1-3, 2-3, 2-4 (degrees 2, 2, 3, 2, 1).
+
+    Order 0 starts at the first vertex of largest degree (2); each next vertex
+    has the most already-placed neighbours, ties to the strictly larger degree
+    and otherwise to the smaller id: 2, 0, 1, 3, 4.  Its backward lists are
+    [2], [0], [1, 2], [2], so orders 1..4 start on the edges (1,0), (3,1),
+    (3,2), (4,2) (the lower_bound walk over the offsets [0, 1, 2, 4, 5]).
+    Order 2 takes 2 before 0 (equal adjacency, larger degree); orders 0, 3
+    and 4 take the smaller id among equal degrees."""
+    O = oracle_lib
+    q = Query(vlabels=np.zeros(5, np.uint32), src=np.asarray([0, 0, 1, 2, 2], np.uint32),
+              dst=np.asarray([1, 2, 3, 3, 4], np.uint32), elabels=np.zeros(5, np.uint32))
+    g = O.OGraph(np.zeros(2, np.uint32), [0], [1], [0])
+    got = O.graphflow_orders(g, O.OGraph.from_query(q), 5, 5).tolist()
+    assert got == [[2, 0, 1, 3, 4], [1, 0, 2, 3, 4], [3, 1, 2, 0, 4], [3, 2, 0, 1, 4],
+                   [4, 2, 0, 1, 3]]
