@@ -35,15 +35,16 @@ HBM_PEAK_GBS = 8000.0
 
 
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_search_c2_hbm_traffic.md")
+COUNTERS_PROFILE = os.path.join(HERE, "profiles", "k_search_c2_counters.md")
 
 
-def profiled_traffic(config, algo, batch, avg_launch_us):
-    """HBM bytes per search launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when it describes this workload and its average launch
-    duration is within 25 % of the live one (otherwise the kernel changed
-    since it was profiled: None)."""
+def profile_figures(path, config, algo, batch, avg_launch_us):
+    """The machine-readable line of a committed rocprofv3 summary, when it
+    describes this workload and its average launch duration is within 25 %
+    of the live one (otherwise the kernel changed since it was profiled:
+    None)."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             line = [x for x in f if x.startswith("machine-readable:")][-1]
     except (OSError, IndexError):
         return None
@@ -53,7 +54,29 @@ def profiled_traffic(config, algo, batch, avg_launch_us):
     prof_us = float(kv["avg_launch_us"])
     if not avg_launch_us or abs(prof_us - avg_launch_us) > 0.25 * avg_launch_us:
         return None
-    return int(kv["traffic_bytes_per_launch"])
+    return kv
+
+
+def profiled_traffic(config, algo, batch, avg_launch_us):
+    """HBM bytes per search launch from the committed FETCH_SIZE / WRITE_SIZE
+    summary (None when it does not describe this kernel and workload)."""
+    kv = profile_figures(TRAFFIC_PROFILE, config, algo, batch, avg_launch_us)
+    return int(kv["traffic_bytes_per_launch"]) if kv else None
+
+
+def profiled_l2(config, algo, batch, avg_launch_us):
+    """L2 figures of the search from the committed counter summary: what
+    bounds the kernel (its HBM traffic is a few % of the algorithmic bytes)."""
+    kv = profile_figures(COUNTERS_PROFILE, config, algo, batch, avg_launch_us)
+    if not kv:
+        return None
+    return {"l2_hit_rate": float(kv["l2_hit"]), "l2_busy": float(kv["l2_busy"]),
+            "l1_to_l2_latency_cycles": float(kv["l1_l2_latency_cycles"]),
+            "l2_read_requests_per_launch": int(kv["l2_read_req_per_launch"]),
+            "l2_read_requests_per_candidate": float(kv["l2_read_req_per_candidate"]),
+            "wave_wait_fraction": float(kv["wave_wait"]),
+            "wave_issue_fraction": float(kv["wave_issue"]),
+            "source": os.path.relpath(COUNTERS_PROFILE, HERE)}
 
 
 def log(*a):
@@ -234,11 +257,17 @@ def main() -> int:
     avg_ns = k_ns / max(launches, 1)
     achieved = (per_launch_bytes / avg_ns) if avg_ns else 0.0  # bytes/ns == GB/s
     traffic = profiled_traffic(args.config, args.algo, args.batch, avg_ns / 1e3)
+    l2 = profiled_l2(args.config, args.algo, args.batch, avg_ns / 1e3)
+    cands = st1["candidates"] - st0["candidates"]
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "traffic_source": (os.path.relpath(TRAFFIC_PROFILE, HERE) if traffic else None),
             "kernel": "k_search", "avg_launch_us": round(avg_ns / 1e3, 2),
             "alg_bytes_per_launch": int(per_launch_bytes), "launches": int(launches),
+            "candidates_per_launch": int(cands / max(launches, 1)),
+            "traffic_over_alg_bytes": (round(traffic / per_launch_bytes, 4)
+                                       if traffic and per_launch_bytes else None),
+            "l2": l2,
             "kernel_share_of_step": round(k_ns / 1e9 / elapsed, 4) if elapsed else None}
 
     sat = next((x for x in served if x.get("saturating")), None)

@@ -57,7 +57,13 @@ typedef struct csm_update {
 /* Options; mirror the reference CLI flags (main.cpp:681-703) plus device
  * knobs.  Zero-initialise and set what you need. */
 typedef struct csm_opts {
-  uint64_t max_results;     /* --max-results; 0 = unbounded                */
+  uint64_t max_results;     /* --max-results: per update and query, the
+                               search stops once the count reaches this and
+                               the count reported is min(exact, max_results)
+                               (CSM status bit CSM_USTAT_CAPPED); 0 = none.
+                               The reference stops each update's search at
+                               max_num_results_ (parallel_graphflow.cpp:386,
+                               2101) and keeps the count it had reached.    */
   uint32_t print_prep;      /* --print-prep                                */
   uint32_t print_enum;      /* --print-enum (unsupported on device)        */
   uint32_t homo;            /* --homo: drop the injectivity test           */
@@ -65,7 +71,16 @@ typedef struct csm_opts {
   uint32_t auto_tuning;     /* --auto-tuning                               */
   uint32_t max_batch;       /* max updates per csm_apply_batch; 0 = 65534  */
   uint32_t want_hash;       /* compute per-update embedding-set hashes     */
-  double time_limit_s;      /* --time-limit per batch; 0 = none            */
+  double time_limit_s;      /* --time-limit: wall-clock seconds per engine
+                               call that searches (apply_batch, add/remove
+                               edge, initial); 0 = none.  When it expires the
+                               search stops, every update is still applied to
+                               the graph and index, the call returns
+                               CSM_E_TIME_LIMIT and csm_update_status flags
+                               the updates whose counts are partial (lower
+                               bounds).  Replaces execute_with_time_limit
+                               (utils/utils.h:19-36) and the reach_time_limit
+                               polls (parallel_graphflow.cpp:288,386-387).  */
   uint64_t device_budget;   /* bytes of HBM the engine may use; 0 = 90%    */
   uint32_t split_chunk;     /* candidates per depth-2 task; 0 = autotuned  */
   uint32_t device;          /* HIP device ordinal                          */
@@ -144,6 +159,19 @@ int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t
 int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
                            uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash);
 int csm_sync(csm_engine* e);
+
+/* Per update of the last csm_apply_batch / csm_apply_batch_device call (or
+ * entry 0 after csm_initial): bit CSM_USTAT_TIME_LIMIT = the time limit
+ * stopped part of its search (its counts are lower bounds); bit
+ * CSM_USTAT_CAPPED = a count of it reached max_results.  No reference
+ * counterpart (the reference reports one flag for the whole run). */
+#define CSM_USTAT_TIME_LIMIT 1
+#define CSM_USTAT_CAPPED 2
+int csm_update_status(const csm_engine* e, uint8_t* status, size_t n);
+
+/* Sets csm_opts.time_limit_s for the following calls (the CLI hands each
+ * call what is left of the run's --time-limit).  No reference counterpart. */
+int csm_set_time_limit(csm_engine* e, double seconds);
 
 /* Per query: 1 if the last batch stopped that query's search because its
  * match budget was exceeded (its counts for that batch are then partial).

@@ -188,7 +188,6 @@ int main(int argc, char** argv) {
     std::cerr << "-q, -d and -u are required\n";
     return 1;
   }
-  (void)initial_time_limit;
 
   auto t0 = std::chrono::steady_clock::now();
   std::cout << "----------- Loading graphs ------------" << std::endl;
@@ -210,7 +209,7 @@ int main(int argc, char** argv) {
   o.threads = threads;
   o.auto_tuning = auto_tuning;
   o.max_batch = batch;
-  o.time_limit_s = time_limit;
+  o.time_limit_s = 0;   // set per phase below
   o.device = device;
   o.reference_orders = reference_orders;
   csm_graph cq = as_graph(q), cd = as_graph(d);
@@ -229,10 +228,18 @@ int main(int argc, char** argv) {
     std::cout << "----------- Initial Matching ----------" << std::endl;
     t0 = std::chrono::steady_clock::now();
     uint64_t n0 = 0;
+    csm_set_time_limit(e, initial_time_limit);
     rc = csm_initial(e, &n0);
-    if (rc) return die(e, rc);
+    const bool timeout = rc == CSM_E_TIME_LIMIT;
+    if (timeout) std::cout << "Timeout " << initial_time_limit << "s" << std::endl;
+    else if (rc) return die(e, rc);
     std::cout << "Initial Matching: " << ms_since(t0) << "ms" << std::endl;
     std::cout << n0 << " matches. \n";
+    // main.cpp:779-790: a timed-out initial matching ends the run
+    if (timeout) {
+      csm_destroy(e);
+      return 1;
+    }
   }
 
   std::cout << "--------- Incremental Matching --------" << std::endl;
@@ -246,7 +253,7 @@ int main(int argc, char** argv) {
     csm_destroy(e);
     return -1;
   }
-  size_t num_v = 0, num_e = 0, unsafe = 0, classified_unsafe = 0;
+  size_t num_v = 0, num_e = 0, unsafe = 0, classified_unsafe = 0, partial = 0;
   t0 = std::chrono::steady_clock::now();
   if (compat) {
     // BatchUpdates3 (main.cpp:253-397)
@@ -300,15 +307,32 @@ int main(int argc, char** argv) {
   } else {
     const size_t B = batch ? batch : 65534;
     std::vector<uint64_t> pos, neg;
-    std::vector<uint8_t> safe;
+    std::vector<uint8_t> safe, status;
     const size_t nq = csm_num_queries(e);
+    // --time-limit covers the whole incremental phase: each call gets what
+    // is left; once it is spent every later search stops at once while the
+    // updates are still applied (the reference's reach_time_limit polls)
+    const auto t_end = t0 + std::chrono::duration<double>(time_limit);
+    bool timeout = false;
     for (size_t s = 0; s < ups.size(); s += B) {
       size_t m = std::min(B, ups.size() - s);
       pos.assign(m * nq, 0);
       neg.assign(m * nq, 0);
       safe.assign(m, 0);
+      status.assign(m, 0);
+      if (time_limit > 0) {
+        double left = std::chrono::duration<double>(t_end - std::chrono::steady_clock::now()).count();
+        csm_set_time_limit(e, left > 1e-9 ? left : 1e-9);
+      }
       rc = csm_apply_batch(e, ups.data() + s, m, pos.data(), neg.data(), safe.data(), nullptr);
-      if (rc) return die(e, rc);
+      if (rc == CSM_E_TIME_LIMIT) {
+        if (!timeout) std::cout << "Timeout " << time_limit << "s" << std::endl;
+        timeout = true;
+      } else if (rc) {
+        return die(e, rc);
+      }
+      csm_update_status(e, status.data(), m);
+      for (size_t i = 0; i < m; ++i) partial += (status[i] & CSM_USTAT_TIME_LIMIT) != 0;
       for (size_t i = 0; i < m; ++i) {
         const csm_update& u = ups[s + i];
         if (u.type == 'v') ++num_v; else ++num_e;
@@ -328,6 +352,8 @@ int main(int argc, char** argv) {
   std::cout << ptot << " positive matches.\n";
   std::cout << ntot << " negative matches.\n";
   std::cout << classified_unsafe << " updates classified unsafe by the label filter.\n";
+  if (time_limit > 0 && !compat)
+    std::cout << partial << " updates with partial counts (time limit).\n";
   size_t me = 0, mv = 0;
   csm_memory(e, &me, &mv);
   uint64_t dev = 0;

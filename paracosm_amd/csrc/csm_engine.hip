@@ -17,6 +17,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -175,6 +176,19 @@ struct csm_engine {
   bool ev_pending = false;
   uint64_t dev_bytes = 0;
 
+  // time limit (csm_opts.time_limit_s per engine call): a word in pinned,
+  // fine-grained host memory that the search polls (h_abort, mapped at
+  // d_abort_map); the host sets it once the deadline has passed
+  uint32_t* h_abort = nullptr;
+  uint32_t* d_abort_map = nullptr;
+  double deadline = 0;       // steady-clock seconds; 0 = no limit in force
+  bool timed_out = false;
+  int call_depth = 0;
+  // per update of the last call: USTAT_* bits (d_ustatus per batch)
+  uint32_t* d_ustatus = nullptr;
+  std::vector<uint8_t> h_ustatus;
+  uint32_t* h_ustatus_pin = nullptr;
+
   std::vector<DevBuf> allocs;
 };
 
@@ -188,6 +202,48 @@ int fail(csm_engine* e, int code, const char* fmt, ...) {
   va_end(ap);
   if (e) e->err = buf;
   return code;
+}
+
+inline double steady_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// One engine call that may search.  The outermost call of a nesting starts
+// the time limit's clock (csm_opts.time_limit_s; the reference's watchdog,
+// utils/utils.h:19-36, polled as reach_time_limit in the searches,
+// parallel_graphflow.cpp:288,386-387) and turns a successful return into
+// CSM_E_TIME_LIMIT when the limit fired.
+struct CallScope {
+  csm_engine* e;
+  bool outer;
+  explicit CallScope(csm_engine* e_) : e(e_), outer(e_ && e_->call_depth++ == 0) {
+    if (!outer) return;
+    e->timed_out = false;
+    e->deadline = e->opts.time_limit_s > 0 ? steady_s() + e->opts.time_limit_s : 0.0;
+    if (e->h_abort) __atomic_store_n(e->h_abort, 0u, __ATOMIC_SEQ_CST);
+  }
+  int finish(int rc) {
+    if (outer && rc == CSM_OK && e->timed_out) {
+      e->err = "time limit reached; updates flagged in csm_update_status have partial counts";
+      rc = CSM_E_TIME_LIMIT;
+    }
+    return rc;
+  }
+  ~CallScope() {
+    if (!e) return;
+    if (--e->call_depth == 0) {
+      e->deadline = 0;
+      if (e->h_abort) __atomic_store_n(e->h_abort, 0u, __ATOMIC_SEQ_CST);
+    }
+  }
+};
+
+// past the deadline: raise the device-visible abort word (once)
+inline void check_deadline(csm_engine* e) {
+  if (e->deadline > 0 && !e->timed_out && steady_s() >= e->deadline) {
+    e->timed_out = true;
+    __atomic_store_n(e->h_abort, 1u, __ATOMIC_SEQ_CST);
+  }
 }
 
 #define HIP_TRY(e, call)                                                               \
@@ -576,8 +632,10 @@ int alloc_batch(csm_engine* e) {
       (rc = dalloc(e, &e->d_keys2, 2 * (size_t)B)) || (rc = dalloc(e, &e->d_vals, 2 * (size_t)B)) ||
       (rc = dalloc(e, &e->d_vals2, 2 * (size_t)B)) || (rc = dalloc(e, &e->d_runs, 2 * (size_t)B)) ||
       (rc = dalloc(e, &e->d_dirty[0], 2 * (size_t)B + 2)) ||
-      (rc = dalloc(e, &e->d_dirty[1], 2 * (size_t)B + 2)) || (rc = dalloc(e, &e->d_safe, B)))
+      (rc = dalloc(e, &e->d_dirty[1], 2 * (size_t)B + 2)) || (rc = dalloc(e, &e->d_safe, B)) ||
+      (rc = dalloc(e, &e->d_ustatus, (size_t)B + 1)))
     return rc;
+  HIP_TRY(e, hipMemset(e->d_ustatus, 0, ((size_t)B + 1) * 4));
   e->seed_cap = std::max<uint32_t>(1u << 16, std::min<uint64_t>(16ull * B, 1u << 22));
   e->chunk_cap = 1u << 26;
   // spill buffers: each XCD writes into its own eighth, and a heavy subtree
@@ -593,6 +651,12 @@ int alloc_batch(csm_engine* e) {
     return rc;
   HIP_TRY(e, hipHostMalloc((void**)&e->h_upd, (size_t)B * sizeof(UpdateRec)));
   HIP_TRY(e, hipHostMalloc((void**)&e->h_safe, B));
+  HIP_TRY(e, hipHostMalloc((void**)&e->h_ustatus_pin, ((size_t)B + 1) * 4));
+  // the abort word: coherent (fine-grained) pinned memory mapped into the
+  // device address space, so that a running kernel sees the host's store
+  HIP_TRY(e, hipHostMalloc((void**)&e->h_abort, 64, hipHostMallocMapped | hipHostMallocCoherent));
+  *e->h_abort = 0;
+  HIP_TRY(e, hipHostGetDevicePointer((void**)&e->d_abort_map, e->h_abort, 0));
   return CSM_OK;
 }
 
@@ -633,6 +697,18 @@ int ensure_results(csm_engine* e) {
 
 int read_ctrl(csm_engine* e) {
   HIP_TRY(e, hipMemcpyAsync(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, e->stream));
+  if (e->deadline > 0) {
+    // under a time limit the host watches the clock while the device works
+    while (true) {
+      hipError_t st = hipStreamQuery(e->stream);
+      if (st == hipSuccess) break;
+      if (st != hipErrorNotReady)
+        return fail(e, CSM_E_DEVICE, "hipStreamQuery failed: %s", hipGetErrorString(st));
+      check_deadline(e);
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    return CSM_OK;
+  }
   HIP_TRY(e, hipStreamSynchronize(e->stream));
   return CSM_OK;
 }
@@ -714,6 +790,9 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
   sa.spill_cap = e->spill_cap;
   sa.spill_chunk_cap = e->spill_chunk_cap;
   sa.budget_iters = e->budget_iters;
+  sa.habort = e->deadline > 0 ? e->d_abort_map : nullptr;
+  sa.ustatus = e->d_ustatus;
+  sa.ucap = e->opts.max_results;
   const bool h = hashes != nullptr;
   int out = 0;
   static const bool trace = getenv("CSM_TRACE_ROUNDS") != nullptr;
@@ -722,6 +801,7 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
     sa.spill_owner = e->d_spill_owner[out];
     sa.xregions = round > 0 ? 1u : 0u;
     uint32_t chunks_in = trace ? e->h_ctrl->nspill_chunks : 0;
+    check_deadline(e);   // after the limit, rounds only drain their chunks
     launch_round(e, sa, versioned, h);
     int rc = read_ctrl(e);
     if (rc) return rc;
@@ -849,6 +929,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   const size_t nq = e->queries.size();
   HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
   if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
+  k_fill_u32<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_ustatus, n, 0u);
   if (e->opts.match_budget) {
     HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
     HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
@@ -903,6 +984,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
             HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
             HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
           }
+          k_fill_u32<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_ustatus, n, 0u);
         }
         e->opts.split_chunk = e->tune_cands[i].first;
         e->budget_iters = e->tune_cands[i].second;
@@ -917,6 +999,9 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
       if ((rc = search(e, true, counts, hashes))) return rc;
       if (e->tune_active && n >= TUNE_MIN_BATCH) ++e->tune_idx;   // the settling batch
     }
+    if (e->opts.max_results)
+      k_clamp_counts<<<grid_for((uint64_t)n * nq, 256), 256, 0, e->stream>>>(
+          counts, n, (uint32_t)nq, e->opts.max_results, e->d_ustatus);
   }
   // 6. ... and deletions (after it)
   if (uses_index(e) && e->d_d2)
@@ -1210,6 +1295,7 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
   uint32_t key01 = (q.dlab[ub]) * e->nel + dense_of(e->el_map, q.g.elabel(ua, ub));
   const size_t nq = e->queries.size();
   HIP_TRY(e, hipMemsetAsync(e->d_counts, 0, nq * 8, e->stream));
+  k_fill_u32<<<1, 64, 0, e->stream>>>(e->d_ustatus, 1, 0u);
   uint32_t v0 = 0, step = e->nv;
   while (v0 < e->nv) {
     uint32_t v1 = (uint32_t)std::min<uint64_t>((uint64_t)v0 + step, e->nv);
@@ -1230,8 +1316,15 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
     if ((rc = search(e, true, e->d_counts, nullptr, 0xFFFEu))) return rc;
     v0 = v1;
   }
+  if (e->opts.max_results)
+    k_clamp_counts<<<1, 64, 0, e->stream>>>(e->d_counts, 1, (uint32_t)nq, e->opts.max_results,
+                                            e->d_ustatus);
   unsigned long long c = 0;
-  HIP_TRY(e, hipMemcpy(&c, e->d_counts, 8, hipMemcpyDeviceToHost));  // update 0, query 0
+  uint32_t st = 0;
+  HIP_TRY(e, hipMemcpyAsync(&c, e->d_counts, 8, hipMemcpyDeviceToHost, e->stream));  // query 0
+  HIP_TRY(e, hipMemcpyAsync(&st, e->d_ustatus, 4, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  e->h_ustatus.assign(1, (uint8_t)st);
   *total = c;
   return CSM_OK;
 }
@@ -1569,7 +1662,14 @@ int csm_preprocess(csm_engine* e) {
   return CSM_OK;
 }
 
+static int initial_impl(csm_engine* e, uint64_t* num_results);
+
 int csm_initial(csm_engine* e, uint64_t* num_results) {
+  CallScope cs(e);
+  return cs.finish(initial_impl(e, num_results));
+}
+
+static int initial_impl(csm_engine* e, uint64_t* num_results) {
   if (!e || !num_results) return CSM_E_ARG;
   int rc = csm_preprocess(e);
   if (rc) return rc;
@@ -1658,12 +1758,13 @@ int csm_remove_vertex(csm_engine* e, uint32_t id) {
   return remove_vertex(e, id);
 }
 
-int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
-                           uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash) {
+static int apply_batch_device_impl(csm_engine* e, const csm_update* d_updates, size_t n,
+                                   uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash) {
   if (!e || (n && (!d_updates || !d_counts))) return CSM_E_ARG;
   if (n > e->max_batch) return fail(e, CSM_E_CAPACITY, "batch of %zu > max_batch %u", n, e->max_batch);
   if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
   if (d_hash && !e->opts.want_hash) d_hash = nullptr;
+  e->h_ustatus.assign(n, 0);
   if (n == 0) return CSM_OK;
   HIP_TRY(e, hipSetDevice(e->dev));
   k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
@@ -1676,9 +1777,30 @@ int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
   if (e->opts.match_budget)
     HIP_TRY(e, hipMemcpyAsync(e->h_qabort.data(), e->d_qabort, e->queries.size() * 4,
                               hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(e->h_ustatus_pin, e->d_ustatus, n * 4, hipMemcpyDeviceToHost,
+                            e->stream));
   if ((rc = read_ctrl(e))) return rc;
+  for (size_t i = 0; i < n; ++i) e->h_ustatus[i] = (uint8_t)e->h_ustatus_pin[i];
   accumulate_stats(e);
   return check_errors(e);
+}
+
+int csm_apply_batch_device(csm_engine* e, const csm_update* d_updates, size_t n,
+                           uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash) {
+  CallScope cs(e);
+  return cs.finish(apply_batch_device_impl(e, d_updates, n, d_counts, d_safe, d_hash));
+}
+
+int csm_update_status(const csm_engine* e, uint8_t* status, size_t n) {
+  if (!e || (n && !status)) return CSM_E_ARG;
+  for (size_t i = 0; i < n; ++i) status[i] = i < e->h_ustatus.size() ? e->h_ustatus[i] : 0;
+  return CSM_OK;
+}
+
+int csm_set_time_limit(csm_engine* e, double seconds) {
+  if (!e || !(seconds >= 0)) return CSM_E_ARG;
+  e->opts.time_limit_s = seconds;
+  return CSM_OK;
 }
 
 int csm_query_status(const csm_engine* e, uint8_t* aborted, size_t n) {
@@ -1693,13 +1815,15 @@ int csm_sync(csm_engine* e) {
   return CSM_OK;
 }
 
-int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t* pos,
-                    uint64_t* neg, uint8_t* safe, uint64_t* emb_hash) {
+static int apply_batch_impl(csm_engine* e, const csm_update* updates, size_t n, uint64_t* pos,
+                            uint64_t* neg, uint8_t* safe, uint64_t* emb_hash,
+                            std::vector<uint8_t>& status) {
   if (!e || (n && !updates)) return CSM_E_ARG;
   if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
   HIP_TRY(e, hipSetDevice(e->dev));
   const size_t nq = e->queries.size();
   size_t done = 0;
+  status.assign(n, 0);
   while (done < n) {
     // a vertex deletion is applied on its own, in stream order
     if (updates[done].type == 'v' && !updates[done].is_add) {
@@ -1736,6 +1860,7 @@ int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t
                                     reinterpret_cast<uint64_t*>(e->d_counts), e->d_safe,
                                     hash ? reinterpret_cast<uint64_t*>(e->d_hashes) : nullptr);
     if (rc) return rc;
+    for (size_t i = 0; i < m && i < e->h_ustatus.size(); ++i) status[done + i] = e->h_ustatus[i];
     HIP_TRY(e, hipMemcpyAsync(e->h_counts, e->d_counts, m * nq * 8, hipMemcpyDeviceToHost, e->stream));
     if (hash)
       HIP_TRY(e, hipMemcpyAsync(e->h_hashes, e->d_hashes, m * nq * 8, hipMemcpyDeviceToHost, e->stream));
@@ -1758,6 +1883,15 @@ int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t
     done += m;
   }
   return CSM_OK;
+}
+
+int csm_apply_batch(csm_engine* e, const csm_update* updates, size_t n, uint64_t* pos,
+                    uint64_t* neg, uint8_t* safe, uint64_t* emb_hash) {
+  CallScope cs(e);
+  std::vector<uint8_t> status;
+  int rc = apply_batch_impl(e, updates, n, pos, neg, safe, emb_hash, status);
+  if (e) e->h_ustatus.swap(status);
+  return cs.finish(rc);
 }
 
 int csm_totals(const csm_engine* e, uint64_t* initial, uint64_t* positive, uint64_t* negative) {
@@ -1982,6 +2116,8 @@ void csm_destroy(csm_engine* e) {
   if (e->h_hashes) (void)hipHostFree(e->h_hashes);
   if (e->h_safe) (void)hipHostFree(e->h_safe);
   if (e->h_upd) (void)hipHostFree(e->h_upd);
+  if (e->h_ustatus_pin) (void)hipHostFree(e->h_ustatus_pin);
+  if (e->h_abort) (void)hipHostFree(e->h_abort);
   for (auto& ev : e->ev_search)
     if (ev) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);

@@ -839,7 +839,18 @@ struct SearchArgs {
   unsigned long long* qcount; // per-query running match count (nullable)
   uint32_t* qabort;           // per-query abort flags (nullable)
   unsigned long long budget;  // per-query match budget in this launch (0: none)
+  // time limit: a host-written word in fine-grained host memory (nullable:
+  // no limit); once it reads non-zero, waves stop their tasks and drain the
+  // remaining chunks, flagging each affected update with USTAT_TIME_LIMIT
+  const uint32_t* habort;
+  uint32_t* ustatus;          // per update of the batch: USTAT_* bits
+  unsigned long long ucap;    // per (update, query) result cap (0: none)
 };
+
+constexpr uint32_t USTAT_TIME_LIMIT = 1u;  // search stopped: counts are lower bounds
+constexpr uint32_t USTAT_CAPPED = 2u;      // a count reached the per-update result cap
+// real-time counter ticks (100 MHz) between two polls of the host abort word
+constexpr uint64_t ABORT_POLL_TICKS = 2000;
 
 // Per-wave DFS state in LDS, sized for queries of up to MK vertices.
 template <int MK>
@@ -968,6 +979,19 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
   // XCD y's waves wrote the tasks of the previous round
   const uint32_t xn_lane = (XCD_REGIONS && a.xregions && lane < NXCD) ? a.ctrl->xn[lane] : 0u;
   const uint32_t nwaves = gridDim.x * (uint32_t)search_waves<MK>();
+  // the host's time-limit word, polled at most every ABORT_POLL_TICKS
+  uint32_t aborted = 0;
+  uint64_t last_poll = 0;
+  auto poll_abort = [&]() CSM_AINL -> bool {
+    if (!a.habort || aborted) return aborted != 0;
+    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+    if (now - last_poll < ABORT_POLL_TICKS) return false;
+    last_poll = now;
+    uint32_t v = 0;
+    if (lane == 0) v = __hip_atomic_load(a.habort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    aborted = uniform(__shfl(v, 0));
+    return aborted != 0;
+  };
   while (true) {
     PROF_BEGIN(fetch);
     uint32_t nch = uniform(__atomic_load_n(&a.ctrl->nchunks, __ATOMIC_RELAXED));
@@ -1022,6 +1046,19 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     const uint32_t pos = t + 1 + a.pos_offset;
     const uint32_t q = uniform(P->q);
     if (a.qabort && __atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) continue;
+    if (poll_abort()) {
+      // time limit: this chunk is not searched
+      if (lane == 0) atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
+      continue;
+    }
+    if (a.ucap) {
+      // the count of (t, q) has reached the cap: nothing left to find
+      unsigned long long cur = 0;
+      if (lane == 0)
+        cur = __hip_atomic_load(&a.counts[(size_t)t * a.nq + q], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+      if (uniform(__shfl((uint32_t)(cur >= a.ucap), 0))) continue;
+    }
     ++n_tasks;
     if (qo != S.pqo) {
       if (lane < MK) {
@@ -1048,21 +1085,32 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     uint32_t work = 0;          // budget units used by this task (one per 64 candidates)
     bool may_spill = false;
     bool stop = false;
+    bool lost = false;          // stopped by the time limit: work was dropped
     const uint32_t L1 = k - 1, L2 = k - 2;   // last and second-to-last positions
 
-    // publish the running count; stop if the query's budget is exhausted
+    // publish the running count; stop if the query's budget is exhausted or
+    // the update's count has reached the result cap
     auto flush = [&](bool final_) CSM_AINL {
       unsigned long long d = cnt - flushed;
-      if (lane == 0 && d) {
-        atomicAdd(&a.counts[(size_t)t * a.nq + q], d);
-        if (a.qcount) {
-          unsigned long long tot = atomicAdd(&a.qcount[q], d) + d;
-          if (a.budget && tot > a.budget) atomicOr(&a.qabort[q], 1u);
+      uint32_t capped = 0;
+      if (lane == 0) {
+        unsigned long long tot_u = 0;
+        if (d) {
+          tot_u = atomicAdd(&a.counts[(size_t)t * a.nq + q], d) + d;
+          if (a.qcount) {
+            unsigned long long tot = atomicAdd(&a.qcount[q], d) + d;
+            if (a.budget && tot > a.budget) atomicOr(&a.qabort[q], 1u);
+          }
+        } else if (a.ucap && !final_) {
+          tot_u = __hip_atomic_load(&a.counts[(size_t)t * a.nq + q], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
         }
+        capped = a.ucap && tot_u >= a.ucap;
       }
       flushed = cnt;
       if (!final_ && a.qabort)
         stop = uniform(__atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) != 0;
+      if (!final_ && a.ucap && uniform(__shfl(capped, 0))) stop = true;
     };
 
     // test a candidate entry e of position lv's u_min segment (joins with the
@@ -1579,7 +1627,11 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       }
       if (++ticks == 256) {
         ticks = 0;
-        if (cnt - flushed > (1ull << 20) || a.qabort) flush(false);
+        if (cnt - flushed > (1ull << 20) || a.qabort || a.ucap) flush(false);
+        if (poll_abort()) {
+          stop = true;
+          lost = true;
+        }
       }
       uint32_t cur = S.cur[L], end = S.end[L];
       if (L == L2) {
@@ -1634,6 +1686,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     }
     flush(true);
     if (lane == 0 && HASH && hsum) atomicAdd(&a.hashes[(size_t)t * a.nq + q], hsum);
+    if (lane == 0 && lost) atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
   }
   if (lane == 0) {
     atomicAdd(&a.ctrl->stat_tasks, n_tasks);
@@ -1655,6 +1708,27 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
 }
 
 // ------------------------------------------------------------ batch control
+// counts of n updates x nq queries at or above the result cap become the cap
+// (the search stops a (update, query) pair once its count reaches the cap,
+// so the result is min(exact count, cap) whatever the order of the tasks)
+__global__ void k_clamp_counts(unsigned long long* __restrict__ counts, uint32_t n, uint32_t nq,
+                               unsigned long long cap, uint32_t* __restrict__ ustatus) {
+  const uint64_t total = (uint64_t)n * nq;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (counts[i] >= cap) {
+      counts[i] = cap;
+      atomicOr(&ustatus[i / nq], USTAT_CAPPED);
+    }
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 __global__ void k_batch_begin(Ctrl* c) {
 #ifdef CSM_PHASE_PROF
   if (threadIdx.x == 0) { c->prof[30] = ~0ull; c->prof[31] = 0; }

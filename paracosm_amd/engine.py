@@ -30,6 +30,9 @@ CSM_E_UNSUPPORTED = -6
 CSM_E_TIME_LIMIT = -7
 CSM_E_ADMISSION = -8
 
+USTAT_TIME_LIMIT = 1   # the time limit stopped part of the update's search
+USTAT_CAPPED = 2       # a count of the update reached max_results
+
 ALGORITHMS = ("parallel_graphflow", "parallel_symbi", "parallel_turboflux",
               "graphflow", "symbi", "turboflux")
 
@@ -87,6 +90,8 @@ def lib():
             "csm_apply_batch_device": (ctypes.c_int, [vp, vp, sz, vp, vp, vp]),
             "csm_sync": (ctypes.c_int, [vp]),
             "csm_query_status": (ctypes.c_int, [vp, vp, sz]),
+            "csm_update_status": (ctypes.c_int, [vp, vp, sz]),
+            "csm_set_time_limit": (ctypes.c_int, [vp, ctypes.c_double]),
             "csm_totals": (ctypes.c_int, [vp, P(u64), P(u64), P(u64)]),
             "csm_memory": (ctypes.c_int, [vp, P(sz), P(sz)]),
             "csm_device_bytes": (ctypes.c_int, [vp, P(u64)]),
@@ -119,7 +124,8 @@ def exported_symbols() -> Sequence[str]:
     return ("csm_create", "csm_register_query", "csm_num_queries", "csm_preprocess",
             "csm_initial", "csm_classify", "csm_add_edge", "csm_remove_edge",
             "csm_add_vertex", "csm_remove_vertex", "csm_apply_batch",
-            "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_totals", "csm_memory",
+            "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_update_status",
+            "csm_set_time_limit", "csm_totals", "csm_memory",
             "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
             "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
             "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_parse_text",
@@ -161,12 +167,13 @@ class Matching:
                  homo: bool = False, threads: int = 0, auto_tuning: int = 1,
                  max_batch: int = 0, want_hash: bool = False, device_budget: int = 0,
                  split_chunk: int = 0, device: int = 0, match_budget: int = 0,
-                 spill_budget: int = 0, reference_orders: bool = False):
+                 spill_budget: int = 0, reference_orders: bool = False,
+                 time_limit_s: float = 0.0):
         L = lib()
         self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         d = _GraphArgs(vlabels, src, dst, elabels)
         o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
-                  auto_tuning, max_batch, int(want_hash), 0.0, device_budget, split_chunk,
+                  auto_tuning, max_batch, int(want_hash), time_limit_s, device_budget, split_chunk,
                   device, match_budget, spill_budget, int(reference_orders))
         err = ctypes.c_int(0)
         self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
@@ -261,6 +268,15 @@ class Matching:
         self._check(lib().csm_query_status(self.h, out.ctypes.data, out.size))
         return out[: self.num_queries]
 
+    def update_status(self, n: int) -> np.ndarray:
+        """USTAT_* bits per update of the last batch call (n entries)."""
+        out = np.zeros(max(1, n), np.uint8)
+        self._check(lib().csm_update_status(self.h, out.ctypes.data, n))
+        return out[:n]
+
+    def set_time_limit(self, seconds: float) -> None:
+        self._check(lib().csm_set_time_limit(self.h, float(seconds)))
+
     def totals(self):
         a, b, c = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
         self._check(lib().csm_totals(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
@@ -295,29 +311,41 @@ class Matching:
     def stream(self) -> int:
         return int(lib().csm_stream(self.h) or 0)
 
-    def apply_batch(self, updates: np.ndarray):
+    def _check_partial(self, rc: int, partial_ok: bool) -> bool:
+        """True if the call hit the time limit (allowed by partial_ok)."""
+        if rc == CSM_E_TIME_LIMIT and partial_ok:
+            return True
+        self._check(rc)
+        return False
+
+    def apply_batch(self, updates: np.ndarray, partial_ok: bool = False):
         """Host arrays in, (pos, neg, safe, hash) out; pos/neg/hash are
-        [n, num_queries] uint64, safe is [n] uint8."""
+        [n, num_queries] uint64, safe is [n] uint8.  With partial_ok, a call
+        stopped by the time limit returns its (partial) results instead of
+        raising; update_status(n) says which updates they concern."""
         u = np.ascontiguousarray(updates, dtype=UPDATE_DTYPE)
         n, nq = len(u), self.num_queries
         pos = np.zeros((n, nq), np.uint64)
         neg = np.zeros((n, nq), np.uint64)
         safe = np.zeros(n, np.uint8)
         hsh = np.zeros((n, nq), np.uint64)
-        self._check(lib().csm_apply_batch(self.h, u.ctypes.data, n, pos.ctypes.data,
-                                          neg.ctypes.data, safe.ctypes.data,
-                                          hsh.ctypes.data if self.want_hash else None))
+        rc = lib().csm_apply_batch(self.h, u.ctypes.data, n, pos.ctypes.data,
+                                   neg.ctypes.data, safe.ctypes.data,
+                                   hsh.ctypes.data if self.want_hash else None)
+        self._check_partial(rc, partial_ok)
         return pos, neg, safe, (hsh if self.want_hash else None)
 
     def apply_batch_device(self, d_updates_ptr: int, n: int, d_counts_ptr: int,
-                           d_safe_ptr: int = 0, d_hash_ptr: int = 0) -> None:
+                           d_safe_ptr: int = 0, d_hash_ptr: int = 0,
+                           partial_ok: bool = False) -> bool:
         """Device pointers (e.g. DeviceBuffer.ptr) in HBM; results
         stay on the device.  Synchronous with respect to the engine's stream's
         control read-back."""
         if n < 0:
             raise ValueError(f"negative batch size {n}")
-        self._check(lib().csm_apply_batch_device(self.h, d_updates_ptr, n, d_counts_ptr,
-                                                 d_safe_ptr or None, d_hash_ptr or None))
+        rc = lib().csm_apply_batch_device(self.h, d_updates_ptr, n, d_counts_ptr,
+                                          d_safe_ptr or None, d_hash_ptr or None)
+        return self._check_partial(rc, partial_ok)
 
     def sync(self) -> None:
         self._check(lib().csm_sync(self.h))
