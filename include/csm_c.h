@@ -96,6 +96,9 @@ typedef struct csm_opts {
                                estimated candidate counts from the data
                                graph's label statistics.  Counts are the same
                                either way; only the search effort differs.  */
+  uint32_t want_work;       /* 1: count the candidate entries each update's
+                               search tests (csm_update_work), the size of
+                               its search tree                             */
 } csm_opts;
 
 /* Creates an engine and uploads the data graph.  `algo` is one of the
@@ -169,13 +172,21 @@ int csm_sync(csm_engine* e);
 #define CSM_USTAT_CAPPED 2
 int csm_update_status(const csm_engine* e, uint8_t* status, size_t n);
 
+/* Per update of the last csm_apply_batch_device call (needs want_work):
+ * the candidate entries its search tested over all queries, a measure of the
+ * size of its search tree.  csm_apply_batch keeps those of its last slice.
+ * No reference counterpart (the reference's intermediate counters,
+ * matching.h:28-33, are not printed). */
+int csm_update_work(const csm_engine* e, uint64_t* work, size_t n);
+
 /* Sets csm_opts.time_limit_s for the following calls (the CLI hands each
  * call what is left of the run's --time-limit).  No reference counterpart. */
 int csm_set_time_limit(csm_engine* e, double seconds);
 
-/* Per query: 1 if the last batch stopped that query's search because its
- * match budget was exceeded (its counts for that batch are then partial).
- * No reference counterpart (the reference's --max-results caps one update). */
+/* Per query, for the last batch: bit 1 if its search stopped because its
+ * match budget was exceeded, bit 2 if the time limit dropped part of its
+ * search (its counts for that batch are then partial).  No reference
+ * counterpart (the reference's --max-results caps one update). */
 int csm_query_status(const csm_engine* e, uint8_t* aborted, size_t n);
 
 /* matching::GetNumInitialResults/Positive/Negative (matching.h:60-62). */

@@ -1,10 +1,21 @@
-"""Figures of one benchmark workload beside the headline bench: the fraction
-of updates the label filter certifies safe (Classify, as
-parallel_graphflow.cpp:1990-2016), the device memory the engine holds for the
-graph, indexes and standing queries, and the batched update rate of a
-plugin over a sample of the stream.
+"""Figures of one benchmark workload beside the headline bench.
+
+--mode rate (default): the fraction of updates the label filter certifies
+safe (Classify, as parallel_graphflow.cpp:1990-2016), the device memory the
+engine holds for the graph, indexes and standing queries, and the batched
+update rate of a plugin over a sample of the stream.
 
     python -m paracosm_amd.config_report --config c5 --algo parallel_symbi
+
+--mode large: large queries (config c3: 20 random-walk queries each of 12, 16
+and 24 vertices on the scale-20 graph, 100k insertions).  Every query runs
+alone on its own engine over the stream under a time limit for the whole
+stream (the reference's --time-limit); a query succeeds if its search never
+hit the limit.  Reported per size: success rate, time and matches, and the
+histogram of per-update search-tree sizes (candidate entries tested, log2
+buckets).  No query admission rule applies.
+
+    python -m paracosm_amd.config_report --config c3 --mode large --time-limit 60
 
 Prints one JSON line; progress goes to stderr.
 """
@@ -35,7 +46,14 @@ def main(argv=None) -> int:
     ap.add_argument("--sample", type=int, default=1_000_000, help="stream updates run")
     ap.add_argument("--batch", type=int, default=62500)
     ap.add_argument("--scale", type=int, default=None)
+    ap.add_argument("--mode", default="rate", choices=("rate", "large"))
+    ap.add_argument("--time-limit", type=float, default=60.0,
+                    help="large: seconds per query for the whole stream sample")
+    ap.add_argument("--per-size", type=int, default=0,
+                    help="large: queries per size (0 = the config's)")
     args = ap.parse_args(argv)
+    if args.mode == "large":
+        return large_queries(args)
 
     t0 = time.time()
     d = workloads.build(args.config, seed=args.seed, scale=args.scale, progress=log)
@@ -96,6 +114,90 @@ def main(argv=None) -> int:
         "stats": m.stats(),
     }
     m.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def histogram_log2(x: np.ndarray) -> dict:
+    """Counts of zeros ("0") and of values in [2^j, 2^(j+1)) ("2^j")."""
+    x = np.asarray(x, np.uint64)
+    out = {"0": int((x == 0).sum())}
+    nz = x[x > 0].astype(np.float64)
+    if nz.size:
+        b = np.floor(np.log2(nz)).astype(np.int64)
+        for j, c in zip(*np.unique(b, return_counts=True)):
+            out[f"2^{int(j)}"] = int(c)
+    return out
+
+
+def large_queries(args) -> int:
+    """Config c3: per-query success under a time limit, and the per-update
+    search-tree size histogram per query size."""
+    from . import workloads
+    from .engine import USTAT_TIME_LIMIT, Matching, make_updates
+    t0 = time.time()
+    d = workloads.build(args.config, seed=args.seed, scale=args.scale, progress=log)
+    w = d.w
+    n = min(args.sample, w.num_updates)
+    log(f"workload {args.config}: {w.nv} vertices, {len(w.init_src)} initial edges, "
+        f"{w.num_updates} updates in {time.time() - t0:.1f}s; running {n}")
+    ups = make_updates(w.upd_add[:n], w.upd_v1[:n], w.upd_v2[:n], w.upd_label[:n])
+    per = args.per_size or d.cfg.per_size
+    sizes = {}
+    for k in d.cfg.query_sizes:
+        qs = workloads.candidates(d, k, None, per, 0)
+        rows = []
+        work_all = []
+        for i, q in enumerate(qs):
+            tb = time.time()
+            m = Matching(args.algo, q, w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                         max_batch=args.batch, want_work=True)
+            m.Preprocessing()
+            build = time.time() - tb
+            d_ups = m.device_buffer(min(n, args.batch) * 16)
+            d_cnt = m.device_buffer(min(n, args.batch) * 8)
+            cnt = np.zeros(min(n, args.batch), np.uint64)
+            matches, partial, done = 0, 0, 0
+            t_run = time.perf_counter()
+            work = []
+            for s0 in range(0, n, args.batch):
+                kk = min(args.batch, n - s0)
+                left = args.time_limit - (time.perf_counter() - t_run)
+                m.set_time_limit(max(left, 1e-9))
+                d_ups.upload(ups[s0:s0 + kk].view(np.uint8))
+                hit = m.apply_batch_device(d_ups.ptr, kk, d_cnt.ptr, partial_ok=True)
+                d_cnt.download(cnt.view(np.uint8)[: kk * 8])
+                matches += int(cnt[:kk].sum())
+                st = m.update_status(kk)
+                partial += int(((st & USTAT_TIME_LIMIT) != 0).sum())
+                work.append(m.update_work(kk))
+                done += kk
+                if hit:
+                    break
+            elapsed = time.perf_counter() - t_run
+            m.close()
+            ok = partial == 0 and done == n
+            wk = np.concatenate(work) if work else np.zeros(0, np.uint64)
+            work_all.append(wk)
+            rows.append({"query": i, "edges": q.num_edges, "success": ok,
+                         "seconds": round(elapsed, 3), "updates_searched": done,
+                         "matches": matches, "partial_updates": partial,
+                         "max_update_work": int(wk.max()) if wk.size else 0,
+                         "engine_build_s": round(build, 2)})
+            log(f"k={k} query {i}: {rows[-1]}")
+        wk = np.concatenate(work_all) if work_all else np.zeros(0, np.uint64)
+        succ = sum(r["success"] for r in rows)
+        sizes[str(k)] = {
+            "queries": len(rows), "success": succ, "success_rate": round(succ / max(len(rows), 1), 3),
+            "median_seconds": float(np.median([r["seconds"] for r in rows])) if rows else None,
+            "matches_successful": int(sum(r["matches"] for r in rows if r["success"])),
+            "update_work_histogram_log2": histogram_log2(wk),
+            "per_query": rows,
+        }
+    out = {"config": d.cfg.description, "mode": "large", "algo": args.algo, "seed": args.seed,
+           "updates": n, "batch": args.batch, "time_limit_s_per_query": args.time_limit,
+           "work_unit": "candidate entries tested by the search of one update",
+           "sizes": sizes}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -131,6 +131,8 @@ struct csm_engine {
   unsigned long long* d_qcount = nullptr;  // per-query running matches (budget)
   uint32_t* d_qabort = nullptr;
   std::vector<uint32_t> h_qabort;
+  uint32_t* d_qtl = nullptr;             // per query: time limit dropped work
+  std::vector<uint32_t> h_qtl;
   size_t q_cap = 0;
   uint8_t* d_safe = nullptr;
   unsigned long long* h_counts = nullptr;  // pinned
@@ -187,6 +189,11 @@ struct csm_engine {
   // per update of the last call: USTAT_* bits (d_ustatus per batch)
   uint32_t* d_ustatus = nullptr;
   std::vector<uint8_t> h_ustatus;
+  // per update of the last call: candidate entries the search tested
+  // (csm_opts.want_work)
+  unsigned long long* d_uwork = nullptr;
+  unsigned long long* h_uwork_pin = nullptr;
+  std::vector<uint64_t> h_uwork;
   uint32_t* h_ustatus_pin = nullptr;
 
   std::vector<DevBuf> allocs;
@@ -636,6 +643,11 @@ int alloc_batch(csm_engine* e) {
       (rc = dalloc(e, &e->d_ustatus, (size_t)B + 1)))
     return rc;
   HIP_TRY(e, hipMemset(e->d_ustatus, 0, ((size_t)B + 1) * 4));
+  if (e->opts.want_work) {
+    if ((rc = dalloc(e, &e->d_uwork, (size_t)B + 1))) return rc;
+    HIP_TRY(e, hipMemset(e->d_uwork, 0, ((size_t)B + 1) * 8));
+    HIP_TRY(e, hipHostMalloc((void**)&e->h_uwork_pin, ((size_t)B + 1) * 8));
+  }
   e->seed_cap = std::max<uint32_t>(1u << 16, std::min<uint64_t>(16ull * B, 1u << 22));
   e->chunk_cap = 1u << 26;
   // spill buffers: each XCD writes into its own eighth, and a heavy subtree
@@ -670,6 +682,7 @@ int ensure_results(csm_engine* e) {
       if (a.p == e->d_counts && a.bytes >= need * 8) big = true;
     if (big && e->queries.size() <= e->q_cap) {
       e->h_qabort.assign(e->queries.size(), 0);
+      e->h_qtl.assign(e->queries.size(), 0);
       return CSM_OK;
     }
     dfree(e, e->d_counts);
@@ -683,13 +696,17 @@ int ensure_results(csm_engine* e) {
   if (e->queries.size() > e->q_cap) {
     dfree(e, e->d_qcount);
     dfree(e, e->d_qabort);
+    dfree(e, e->d_qtl);
     e->q_cap = std::max<size_t>(64, e->queries.size() * 2);
-    if ((rc = dalloc(e, &e->d_qcount, e->q_cap)) || (rc = dalloc(e, &e->d_qabort, e->q_cap)))
+    if ((rc = dalloc(e, &e->d_qcount, e->q_cap)) || (rc = dalloc(e, &e->d_qabort, e->q_cap)) ||
+        (rc = dalloc(e, &e->d_qtl, e->q_cap)))
       return rc;
+    HIP_TRY(e, hipMemset(e->d_qtl, 0, e->q_cap * 4));
     HIP_TRY(e, hipMemset(e->d_qcount, 0, e->q_cap * 8));
     HIP_TRY(e, hipMemset(e->d_qabort, 0, e->q_cap * 4));
   }
   e->h_qabort.assign(e->queries.size(), 0);
+  e->h_qtl.assign(e->queries.size(), 0);
   HIP_TRY(e, hipHostMalloc((void**)&e->h_counts, need * 8));
   HIP_TRY(e, hipHostMalloc((void**)&e->h_hashes, need * 8));
   return CSM_OK;
@@ -793,6 +810,8 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
   sa.habort = e->deadline > 0 ? e->d_abort_map : nullptr;
   sa.ustatus = e->d_ustatus;
   sa.ucap = e->opts.max_results;
+  sa.uwork = e->opts.want_work ? e->d_uwork : nullptr;
+  sa.qtl = e->d_qtl;
   const bool h = hashes != nullptr;
   int out = 0;
   static const bool trace = getenv("CSM_TRACE_ROUNDS") != nullptr;
@@ -930,6 +949,10 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
   if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
   k_fill_u32<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_ustatus, n, 0u);
+  k_fill_u32<<<1, 256, 0, e->stream>>>(e->d_qtl, nq, 0u);
+  if (e->d_uwork)
+    k_fill_u32<<<grid_for(2ull * n, 256), 256, 0, e->stream>>>(
+        reinterpret_cast<uint32_t*>(e->d_uwork), 2ull * n, 0u);
   if (e->opts.match_budget) {
     HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
     HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
@@ -985,6 +1008,9 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
             HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
           }
           k_fill_u32<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_ustatus, n, 0u);
+          if (e->d_uwork)
+            k_fill_u32<<<grid_for(2ull * n, 256), 256, 0, e->stream>>>(
+                reinterpret_cast<uint32_t*>(e->d_uwork), 2ull * n, 0u);
         }
         e->opts.split_chunk = e->tune_cands[i].first;
         e->budget_iters = e->tune_cands[i].second;
@@ -1296,6 +1322,7 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
   const size_t nq = e->queries.size();
   HIP_TRY(e, hipMemsetAsync(e->d_counts, 0, nq * 8, e->stream));
   k_fill_u32<<<1, 64, 0, e->stream>>>(e->d_ustatus, 1, 0u);
+  k_fill_u32<<<1, 256, 0, e->stream>>>(e->d_qtl, nq, 0u);
   uint32_t v0 = 0, step = e->nv;
   while (v0 < e->nv) {
     uint32_t v1 = (uint32_t)std::min<uint64_t>((uint64_t)v0 + step, e->nv);
@@ -1765,6 +1792,7 @@ static int apply_batch_device_impl(csm_engine* e, const csm_update* d_updates, s
   if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
   if (d_hash && !e->opts.want_hash) d_hash = nullptr;
   e->h_ustatus.assign(n, 0);
+  e->h_uwork.assign(n, 0);
   if (n == 0) return CSM_OK;
   HIP_TRY(e, hipSetDevice(e->dev));
   k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
@@ -1779,8 +1807,15 @@ static int apply_batch_device_impl(csm_engine* e, const csm_update* d_updates, s
                               hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(e, hipMemcpyAsync(e->h_ustatus_pin, e->d_ustatus, n * 4, hipMemcpyDeviceToHost,
                             e->stream));
+  HIP_TRY(e, hipMemcpyAsync(e->h_qtl.data(), e->d_qtl, e->queries.size() * 4,
+                            hipMemcpyDeviceToHost, e->stream));
+  if (e->d_uwork)
+    HIP_TRY(e, hipMemcpyAsync(e->h_uwork_pin, e->d_uwork, n * 8, hipMemcpyDeviceToHost,
+                              e->stream));
   if ((rc = read_ctrl(e))) return rc;
   for (size_t i = 0; i < n; ++i) e->h_ustatus[i] = (uint8_t)e->h_ustatus_pin[i];
+  e->h_uwork.assign(e->h_uwork_pin ? e->h_uwork_pin : nullptr,
+                    e->h_uwork_pin ? e->h_uwork_pin + n : nullptr);
   accumulate_stats(e);
   return check_errors(e);
 }
@@ -1797,6 +1832,14 @@ int csm_update_status(const csm_engine* e, uint8_t* status, size_t n) {
   return CSM_OK;
 }
 
+int csm_update_work(const csm_engine* e, uint64_t* work, size_t n) {
+  if (!e || (n && !work)) return CSM_E_ARG;
+  if (!e->opts.want_work) return fail(const_cast<csm_engine*>(e), CSM_E_UNSUPPORTED,
+                                      "per-update work needs csm_opts.want_work");
+  for (size_t i = 0; i < n; ++i) work[i] = i < e->h_uwork.size() ? e->h_uwork[i] : 0;
+  return CSM_OK;
+}
+
 int csm_set_time_limit(csm_engine* e, double seconds) {
   if (!e || !(seconds >= 0)) return CSM_E_ARG;
   e->opts.time_limit_s = seconds;
@@ -1805,7 +1848,8 @@ int csm_set_time_limit(csm_engine* e, double seconds) {
 
 int csm_query_status(const csm_engine* e, uint8_t* aborted, size_t n) {
   if (!e || !aborted) return CSM_E_ARG;
-  for (size_t i = 0; i < n && i < e->h_qabort.size(); ++i) aborted[i] = e->h_qabort[i] ? 1 : 0;
+  for (size_t i = 0; i < n && i < e->h_qabort.size(); ++i)
+    aborted[i] = (e->h_qabort[i] ? 1 : 0) | (i < e->h_qtl.size() && e->h_qtl[i] ? 2 : 0);
   return CSM_OK;
 }
 
@@ -2117,6 +2161,7 @@ void csm_destroy(csm_engine* e) {
   if (e->h_safe) (void)hipHostFree(e->h_safe);
   if (e->h_upd) (void)hipHostFree(e->h_upd);
   if (e->h_ustatus_pin) (void)hipHostFree(e->h_ustatus_pin);
+  if (e->h_uwork_pin) (void)hipHostFree(e->h_uwork_pin);
   if (e->h_abort) (void)hipHostFree(e->h_abort);
   for (auto& ev : e->ev_search)
     if (ev) (void)hipEventDestroy(ev);

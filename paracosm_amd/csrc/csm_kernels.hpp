@@ -845,6 +845,8 @@ struct SearchArgs {
   const uint32_t* habort;
   uint32_t* ustatus;          // per update of the batch: USTAT_* bits
   unsigned long long ucap;    // per (update, query) result cap (0: none)
+  unsigned long long* uwork;  // per update: candidate entries tested (nullable)
+  uint32_t* qtl;              // per query: 1 once the time limit dropped some of its work
 };
 
 constexpr uint32_t USTAT_TIME_LIMIT = 1u;  // search stopped: counts are lower bounds
@@ -1048,7 +1050,10 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     if (a.qabort && __atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) continue;
     if (poll_abort()) {
       // time limit: this chunk is not searched
-      if (lane == 0) atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
+      if (lane == 0) {
+        atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
+        atomicOr(&a.qtl[q], 1u);
+      }
       continue;
     }
     if (a.ucap) {
@@ -1060,6 +1065,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       if (uniform(__shfl((uint32_t)(cur >= a.ucap), 0))) continue;
     }
     ++n_tasks;
+    const unsigned long long cands_at_start = n_cands;
     if (qo != S.pqo) {
       if (lane < MK) {
         S.pnb[lane] = P->nb[lane];
@@ -1686,7 +1692,12 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     }
     flush(true);
     if (lane == 0 && HASH && hsum) atomicAdd(&a.hashes[(size_t)t * a.nq + q], hsum);
-    if (lane == 0 && lost) atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
+    if (lane == 0 && lost) {
+      atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
+      atomicOr(&a.qtl[q], 1u);
+    }
+    if (lane == 0 && a.uwork && n_cands > cands_at_start)
+      atomicAdd(&a.uwork[t], n_cands - cands_at_start);
   }
   if (lane == 0) {
     atomicAdd(&a.ctrl->stat_tasks, n_tasks);

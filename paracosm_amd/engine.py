@@ -57,7 +57,7 @@ class _Opts(ctypes.Structure):
                 ("time_limit_s", ctypes.c_double), ("device_budget", ctypes.c_uint64),
                 ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("match_budget", ctypes.c_uint64), ("spill_budget", ctypes.c_uint32),
-                ("reference_orders", ctypes.c_uint32)]
+                ("reference_orders", ctypes.c_uint32), ("want_work", ctypes.c_uint32)]
 
 
 UPDATE_DTYPE = np.dtype([("type", np.uint8), ("is_add", np.uint8), ("pad", np.uint16),
@@ -92,6 +92,7 @@ def lib():
             "csm_query_status": (ctypes.c_int, [vp, vp, sz]),
             "csm_update_status": (ctypes.c_int, [vp, vp, sz]),
             "csm_set_time_limit": (ctypes.c_int, [vp, ctypes.c_double]),
+            "csm_update_work": (ctypes.c_int, [vp, vp, sz]),
             "csm_totals": (ctypes.c_int, [vp, P(u64), P(u64), P(u64)]),
             "csm_memory": (ctypes.c_int, [vp, P(sz), P(sz)]),
             "csm_device_bytes": (ctypes.c_int, [vp, P(u64)]),
@@ -125,7 +126,7 @@ def exported_symbols() -> Sequence[str]:
             "csm_initial", "csm_classify", "csm_add_edge", "csm_remove_edge",
             "csm_add_vertex", "csm_remove_vertex", "csm_apply_batch",
             "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_update_status",
-            "csm_set_time_limit", "csm_totals", "csm_memory",
+            "csm_set_time_limit", "csm_update_work", "csm_totals", "csm_memory",
             "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
             "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
             "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_parse_text",
@@ -168,13 +169,13 @@ class Matching:
                  max_batch: int = 0, want_hash: bool = False, device_budget: int = 0,
                  split_chunk: int = 0, device: int = 0, match_budget: int = 0,
                  spill_budget: int = 0, reference_orders: bool = False,
-                 time_limit_s: float = 0.0):
+                 time_limit_s: float = 0.0, want_work: bool = False):
         L = lib()
         self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         d = _GraphArgs(vlabels, src, dst, elabels)
         o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
                   auto_tuning, max_batch, int(want_hash), time_limit_s, device_budget, split_chunk,
-                  device, match_budget, spill_budget, int(reference_orders))
+                  device, match_budget, spill_budget, int(reference_orders), int(want_work))
         err = ctypes.c_int(0)
         self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
                               ctypes.byref(o), ctypes.byref(err))
@@ -263,7 +264,8 @@ class Matching:
         return int(lib().csm_num_queries(self.h))
 
     def query_status(self) -> np.ndarray:
-        """1 per query whose search the last batch stopped at the match budget."""
+        """Per query of the last batch: bit 1 = stopped at the match budget,
+        bit 2 = the time limit dropped part of its search."""
         out = np.zeros(max(1, self.num_queries), np.uint8)
         self._check(lib().csm_query_status(self.h, out.ctypes.data, out.size))
         return out[: self.num_queries]
@@ -272,6 +274,13 @@ class Matching:
         """USTAT_* bits per update of the last batch call (n entries)."""
         out = np.zeros(max(1, n), np.uint8)
         self._check(lib().csm_update_status(self.h, out.ctypes.data, n))
+        return out[:n]
+
+    def update_work(self, n: int) -> np.ndarray:
+        """Candidate entries tested per update of the last device batch
+        (the engine needs want_work=True)."""
+        out = np.zeros(max(1, n), np.uint64)
+        self._check(lib().csm_update_work(self.h, out.ctypes.data, n))
         return out[:n]
 
     def set_time_limit(self, seconds: float) -> None:

@@ -1,0 +1,198 @@
+"""Parity at the benchmark workloads' real sizes.
+
+Each workload is built at its full size (paracosm_amd.workloads): the
+scale-14 stream of config c1 is checked whole against the oracle; the
+scale-20 (c2) and scale-23 (c5) graphs are checked against the oracle on a
+prefix of the stream that the oracle finishes in seconds, and over the whole
+stream (c2) or a long prefix (c5) through properties that do not need it:
+the three plugins count the same embeddings per update and query, the counts
+do not depend on the batch size, and the per-update plain path agrees.  The
+label-filter flags (Classify) are checked against the oracle's classifier.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, len(os.sched_getaffinity(0)))
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+def _built(name, **kw):
+    from paracosm_amd import workloads
+    t0 = time.time()
+    d = workloads.build(name, seed=1, **kw)
+    _log(f"{name}: built {d.w.nv} vertices, {len(d.w.init_src)} edges, "
+         f"{d.w.num_updates} updates in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    qs = workloads.admit(d, log=_log)
+    _log(f"{name}: admitted {len(qs)} queries in {time.time() - t0:.1f}s")
+    return d, qs
+
+
+def _run_device(algo, d, qs, n, batch, **kw):
+    """Counts [n, nq] and safe flags [n] of the first n updates, in device
+    batches of `batch` updates resident in HBM."""
+    from paracosm_amd.engine import Matching, make_updates
+    w = d.w
+    nq = len(qs)
+    m = Matching(algo, qs[0], w.vlabels, w.init_src, w.init_dst, w.init_elab, max_batch=batch, **kw)
+    for q in qs[1:]:
+        m.register_query(q)
+    m.Preprocessing()
+    ups = make_updates(w.upd_add[:n], w.upd_v1[:n], w.upd_v2[:n], w.upd_label[:n])
+    d_ups = m.device_buffer(n * 16)
+    d_ups.upload(ups.view(np.uint8))
+    d_cnt = m.device_buffer(n * nq * 8)
+    d_safe = m.device_buffer(n)
+    t0 = time.time()
+    for s in range(0, n, batch):
+        k = min(batch, n - s)
+        m.apply_batch_device(d_ups.ptr + 16 * s, k, d_cnt.ptr + 8 * nq * s, d_safe.ptr + s)
+    m.sync()
+    dt = time.time() - t0
+    cnt = d_cnt.download(np.zeros(n * nq, np.uint64)).reshape(n, nq)
+    safe = d_safe.download(np.zeros(n, np.uint8))
+    m.close()
+    _log(f"{algo}: {n} updates in batches of {batch}: {dt:.2f}s, {int(cnt.sum())} matches")
+    return cnt, safe
+
+
+def _oracle_counts(O, d, qs, n, algo="parallel_graphflow"):
+    """Oracle counts [n, nq] of the first n updates (pos or neg by type)."""
+    w = d.w
+    dg = O.OGraph(w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    out = np.zeros((n, len(qs)), np.uint64)
+    t0 = time.time()
+    for j, q in enumerate(qs):
+        r = O.run(algo, dg, O.OGraph.from_query(q), w.upd_add, w.upd_v1, w.upd_v2, w.upd_label,
+                  threads=THREADS, limit=n)
+        out[:, j] = np.where(w.upd_add[:n] == 1, r.pos, r.neg)
+    _log(f"oracle {algo}: {n} updates x {len(qs)} queries in {time.time() - t0:.1f}s "
+         f"on {THREADS} threads")
+    return out
+
+
+def _oracle_safe(O, d, qs, n):
+    """Classify per update (safe iff safe for every query), from the oracle's
+    classifier evaluated once per distinct label triple."""
+    w = d.w
+    l1 = w.vlabels[w.upd_v1[:n]].astype(np.int64)
+    l2 = w.vlabels[w.upd_v2[:n]].astype(np.int64)
+    el = w.upd_label[:n].astype(np.int64)
+    trip = np.stack([l1, l2, el], axis=1)
+    uniq, inv = np.unique(trip, axis=0, return_inverse=True)
+    qgs = [O.OGraph.from_query(q) for q in qs]
+    safe_u = np.asarray([all(O.classify(g, int(a), int(b), int(c)) for g in qgs)
+                         for a, b, c in uniq], np.uint8)
+    return safe_u[inv.reshape(-1)]
+
+
+# ------------------------------------------------------------------ config 1
+@pytest.fixture(scope="module")
+def c1():
+    return _built("c1")
+
+
+@pytest.mark.timeout(600)
+def test_c1_full_stream_matches_oracle(oracle_lib, c1):
+    """Config 1 whole: 11,396 insertions on the scale-14 graph, 5 queries of 6
+    vertices, bit-exact per update and query, in one batch and in batches of
+    1,000; safe flags equal the oracle's classifier."""
+    d, qs = c1
+    n = d.w.num_updates
+    ref = _oracle_counts(oracle_lib, d, qs, n, algo="graphflow")
+    for batch in (65534, 1000):
+        cnt, safe = _run_device("parallel_graphflow", d, qs, n, batch)
+        np.testing.assert_array_equal(cnt, ref)
+    np.testing.assert_array_equal(safe, _oracle_safe(oracle_lib, d, qs, n))
+    assert ref.sum() > 10**7 and safe.mean() > 0.5
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("algo", ["parallel_symbi", "parallel_turboflux"])
+def test_c1_other_plugins(oracle_lib, c1, algo):
+    d, qs = c1
+    n = d.w.num_updates
+    ref, _ = _run_device("parallel_graphflow", d, qs, n, 65534)
+    cnt, _ = _run_device(algo, d, qs, n, 4096)
+    np.testing.assert_array_equal(cnt, ref)
+
+
+def test_c1_plain_path(c1):
+    """The plain per-update path (csm_add_edge, one update per launch
+    sequence) on the first 400 insertions equals the batched counts."""
+    from paracosm_amd.engine import Matching
+    d, qs = c1
+    w = d.w
+    ref, _ = _run_device("parallel_graphflow", d, qs, 400, 65534)
+    m = Matching("parallel_graphflow", qs[0], w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    for q in qs[1:]:
+        m.register_query(q)
+    m.Preprocessing()
+    got = [m.AddEdge(int(w.upd_v1[i]), int(w.upd_v2[i]), int(w.upd_label[i])) for i in range(400)]
+    m.close()
+    np.testing.assert_array_equal(np.asarray(got, np.uint64), ref.sum(axis=1))
+
+
+# ------------------------------------------------------------------ config 2
+@pytest.fixture(scope="module")
+def c2():
+    d, qs = _built("c2")
+    cnt, safe = _run_device("parallel_graphflow", d, qs, d.w.num_updates, 40000)
+    return d, qs, cnt, safe
+
+
+@pytest.mark.timeout(600)
+def test_c2_prefix_matches_oracle(oracle_lib, c2):
+    """Scale-20 graph, 10 queries of 8 vertices, 80/20 insert/delete: the
+    first 1,000 updates (inside the first 40,000-update batch) bit-exact
+    against the oracle, and the first batch's safe flags."""
+    d, qs, cnt, safe = c2
+    n = 4000
+    ref = _oracle_counts(oracle_lib, d, qs, n)
+    np.testing.assert_array_equal(cnt[:n], ref)
+    assert ref.sum() > 0 and (d.w.upd_add[:n] == 0).any()
+    np.testing.assert_array_equal(safe[:40000], _oracle_safe(oracle_lib, d, qs, 40000))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("algo", ["parallel_symbi", "parallel_turboflux"])
+def test_c2_whole_stream_plugins_agree(c2, algo):
+    """All 10^6 updates: SymBi (DCS/DCG index maintained over the stream) and
+    TurboFlux count exactly what GraphFlow counts, per update and query."""
+    d, qs, cnt, _ = c2
+    got, _ = _run_device(algo, d, qs, d.w.num_updates, 40000)
+    np.testing.assert_array_equal(got, cnt)
+
+
+@pytest.mark.timeout(600)
+def test_c2_batch_size_invariance(c2):
+    d, qs, cnt, _ = c2
+    n = 200_000
+    got, _ = _run_device("parallel_graphflow", d, qs, n, 7919)
+    np.testing.assert_array_equal(got, cnt[:n])
+
+
+# ------------------------------------------------------------------ config 5
+@pytest.mark.timeout(900)
+def test_c5_matches_oracle_and_plugins_agree(oracle_lib):
+    """Scale 23 (8,388,608 vertices, ~124M initial edges), 64 vertex labels,
+    4 edge labels, 8 dense queries of 10 vertices: the first 300 updates
+    against the oracle, the first 10^6 updates GraphFlow = SymBi per update
+    and query, and the first batch's safe flags."""
+    d, qs = _built("c5")
+    n = 1_000_000
+    cnt, safe = _run_device("parallel_symbi", d, qs, n, 62500)
+    ref = _oracle_counts(oracle_lib, d, qs, 300)
+    np.testing.assert_array_equal(cnt[:300], ref)
+    np.testing.assert_array_equal(safe[:62500], _oracle_safe(oracle_lib, d, qs, 62500))
+    gf, _ = _run_device("parallel_graphflow", d, qs, n, 62500)
+    np.testing.assert_array_equal(gf, cnt)
+    _log(f"c5: safe fraction {safe.mean():.4f}, matches {int(cnt.sum())}")

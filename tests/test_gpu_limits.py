@@ -91,6 +91,7 @@ def test_time_limit_returns_promptly_with_flags():
               "counted", int(pos.sum()), flush=True)
         assert dt < 3.0
         assert (st & USTAT_TIME_LIMIT).any()
+        assert m.query_status().tolist() == [2]   # the time limit dropped its work
         with pytest.raises(CsmError) as ei:
             m.apply_batch(u[32:40])
         assert ei.value.code == CSM_E_TIME_LIMIT
