@@ -152,7 +152,7 @@ def c2():
 @pytest.mark.timeout(600)
 def test_c2_prefix_matches_oracle(oracle_lib, c2):
     """Scale-20 graph, 10 queries of 8 vertices, 80/20 insert/delete: the
-    first 1,000 updates (inside the first 40,000-update batch) bit-exact
+    first 4,000 updates (inside the first 40,000-update batch) bit-exact
     against the oracle, and the first batch's safe flags."""
     d, qs, cnt, safe = c2
     n = 4000
