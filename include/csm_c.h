@@ -266,6 +266,17 @@ int csm_parse_text(int device, int mode, const char* text, size_t len, csm_updat
 int csm_parse_text_device(csm_engine* e, int mode, const char* d_text, size_t len,
                           csm_update* d_out, size_t cap, size_t* n, uint64_t* err_line);
 
+/* csm_parse_text_device with no device allocation.  No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its working space is a fixed scratch array of 12 MiB per GPU
+ * inside the library, so a call takes at most 8 MiB of text and 262,144
+ * lines (larger inputs: CSM_E_CAPACITY, nothing parsed).  Calls on one GPU
+ * share that array: calls that overlap on one GPU race and must be
+ * serialised by the caller.  Results equal csm_parse_text_device's. */
+int csm_parse_text_device_static(csm_engine* e, int mode, const char* d_text, size_t len,
+                                 csm_update* d_out, size_t cap, size_t* n, uint64_t* err_line);
+
 const char* csm_last_error(const csm_engine* e);
 void csm_destroy(csm_engine* e);
 

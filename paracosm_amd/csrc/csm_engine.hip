@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -195,6 +196,9 @@ struct csm_engine {
   unsigned long long* h_uwork_pin = nullptr;
   std::vector<uint64_t> h_uwork;
   uint32_t* h_ustatus_pin = nullptr;
+
+  // pinned words for the allocation-free parse's read-backs
+  uint64_t* h_parse = nullptr;
 
   std::vector<DevBuf> allocs;
 };
@@ -1499,6 +1503,124 @@ int parse_text_on(hipStream_t st, int mode, const uint8_t* d_text, uint64_t len,
   return done(rc);
 }
 
+// ---------------------------------------------- allocation-free text parse
+// csm_parse_text_device_static parses text already in HBM with no device
+// allocation: its working space is this file-scope array, PARSE_SCRATCH_BYTES
+// per GPU (each device holds its own copy until the process ends).  Host code
+// must never pass the array's name to a kernel or copy: its device address on
+// each GPU comes from hipGetSymbolAddress at engine creation (scratch_init)
+// and is kept per device in g_scratch.  Every call on one GPU shares that
+// GPU's array, so calls that overlap on one GPU race; callers serialise them.
+constexpr size_t PARSE_SCRATCH_BYTES = 12u << 20;
+constexpr uint64_t PARSE_STATIC_MAX_TEXT = 8u << 20;   // bytes of text per call
+constexpr uint64_t PARSE_STATIC_MAX_LINES = 1u << 18;  // lines per call
+static __device__ __align__(256) uint8_t scratch[PARSE_SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static uint8_t* g_scratch[SCRATCH_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// with `dev` current: look up this GPU's copy of the scratch array (once)
+int scratch_init(int dev) {
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return CSM_E_CAPACITY;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev]) return CSM_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CSM_E_DEVICE;
+  g_scratch[dev] = static_cast<uint8_t*>(p);
+  return CSM_OK;
+}
+
+__global__ void k_parse_flags_init(unsigned long long* flags) {
+  if (threadIdx.x == 0) {
+    flags[0] = 0;
+    flags[1] = ~0ull;
+  }
+}
+
+int parse_static(csm_engine* e, int mode, const uint8_t* d_text, uint64_t len,
+                 csm_update* d_out, size_t cap, size_t* n, uint64_t* err_line) {
+  *n = 0;
+  if (err_line) *err_line = 0;
+  if (mode != PARSE_GRAPH && mode != PARSE_STREAM) return fail(e, CSM_E_ARG, "unknown parse mode");
+  if (len > PARSE_STATIC_MAX_TEXT)
+    return fail(e, CSM_E_CAPACITY, "text of %llu bytes exceeds the %llu-byte limit of the "
+                "allocation-free parse", (unsigned long long)len,
+                (unsigned long long)PARSE_STATIC_MAX_TEXT);
+  if (len == 0) return CSM_OK;
+  if ((reinterpret_cast<uintptr_t>(d_text) & 15u) != 0)
+    return fail(e, CSM_E_ARG, "text must be 16-byte aligned in device memory");
+  uint8_t* base = e->dev < SCRATCH_MAX_DEVICES ? g_scratch[e->dev] : nullptr;
+  if (!base) return fail(e, CSM_E_DEVICE, "scratch of the allocation-free parse is not set up");
+  hipStream_t st = e->stream;
+  // bump allocation inside the scratch array, 256-byte aligned
+  size_t used = 0;
+  auto take = [&](size_t bytes) -> uint8_t* {
+    size_t at = (used + 255) & ~size_t(255);
+    if (at + bytes > PARSE_SCRATCH_BYTES) return nullptr;
+    used = at + bytes;
+    return base + at;
+  };
+  const uint64_t ntiles = (len + PARSE_TILE - 1) / PARSE_TILE;
+  auto* tcount = reinterpret_cast<uint32_t*>(take(ntiles * 4));
+  auto* toff = reinterpret_cast<uint32_t*>(take(ntiles * 4));
+  auto* flags = reinterpret_cast<unsigned long long*>(take(16));
+  auto* count = reinterpret_cast<uint64_t*>(take(8));
+  size_t need = 0;
+  if (rocprim::exclusive_scan(nullptr, need, tcount, toff, 0u, (size_t)ntiles,
+                              rocprim::plus<uint32_t>(), st) != hipSuccess)
+    return fail(e, CSM_E_DEVICE, "scan size query failed");
+  void* tmp = take(need);
+  if (!tcount || !toff || !flags || !count || !tmp)
+    return fail(e, CSM_E_CAPACITY, "allocation-free parse: scratch too small");
+  k_parse_count<<<(uint32_t)ntiles, PARSE_BS, 0, st>>>(d_text, len, tcount);
+  HIP_TRY(e, rocprim::exclusive_scan(tmp, need, tcount, toff, 0u, (size_t)ntiles,
+                                     rocprim::plus<uint32_t>(), st));
+  uint64_t* hp = e->h_parse;
+  hp[0] = hp[1] = hp[2] = 0;
+  HIP_TRY(e, hipMemcpyAsync(&hp[0], toff + ntiles - 1, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(e, hipMemcpyAsync(&hp[1], tcount + ntiles - 1, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(e, hipMemcpyAsync(&hp[2], d_text + len - 1, 1, hipMemcpyDeviceToHost, st));
+  HIP_TRY(e, hipStreamSynchronize(st));
+  const uint64_t n_nl = (hp[0] & 0xFFFFFFFFull) + (hp[1] & 0xFFFFFFFFull);
+  const uint64_t n_lines = n_nl + ((hp[2] & 0xFFu) != '\n' ? 1 : 0);
+  if (n_lines > PARSE_STATIC_MAX_LINES)
+    return fail(e, CSM_E_CAPACITY, "%llu lines exceed the %llu-line limit of the "
+                "allocation-free parse", (unsigned long long)n_lines,
+                (unsigned long long)PARSE_STATIC_MAX_LINES);
+  auto* lend = reinterpret_cast<uint64_t*>(take(std::max<uint64_t>(n_nl, 1) * 8));
+  auto* rec = reinterpret_cast<UpdateRec*>(take(n_lines * sizeof(UpdateRec)));
+  auto* rec2 = reinterpret_cast<UpdateRec*>(take(n_lines * sizeof(UpdateRec)));
+  auto* keep = take(n_lines);
+  size_t need2 = 0;
+  if (rocprim::select(nullptr, need2, rec, keep, rec2, count, (size_t)n_lines, st) != hipSuccess)
+    return fail(e, CSM_E_DEVICE, "select size query failed");
+  void* tmp2 = take(need2);
+  if (!lend || !rec || !rec2 || !keep || !tmp2)
+    return fail(e, CSM_E_CAPACITY, "allocation-free parse: scratch too small");
+  // flags start at (no error, no bad line) on every call: the array keeps
+  // the last call's contents
+  k_parse_flags_init<<<1, 64, 0, st>>>(flags);
+  k_parse_lines<<<(uint32_t)ntiles, PARSE_BS, 0, st>>>(d_text, len, toff, lend);
+  k_parse_records<<<grid_for(n_lines, 256, 65536), 256, 0, st>>>(
+      d_text, len, lend, n_nl, n_lines, mode, rec, keep, reinterpret_cast<uint32_t*>(flags),
+      flags + 1);
+  HIP_TRY(e, rocprim::select(tmp2, need2, rec, keep, rec2, count, (size_t)n_lines, st));
+  HIP_TRY(e, hipMemcpyAsync(&hp[3], flags, 16, hipMemcpyDeviceToHost, st));
+  HIP_TRY(e, hipMemcpyAsync(&hp[5], count, 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(e, hipStreamSynchronize(st));
+  if (hp[3] & 1u) {
+    if (err_line) *err_line = hp[4] + 1;
+    return fail(e, CSM_E_ARG, "malformed line %llu", (unsigned long long)(hp[4] + 1));
+  }
+  *n = hp[5];
+  if (hp[5] > cap) return fail(e, CSM_E_CAPACITY, "output capacity too small");
+  if (hp[5]) {
+    HIP_TRY(e, hipMemcpyAsync(d_out, rec2, hp[5] * sizeof(UpdateRec), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(e, hipStreamSynchronize(st));
+  }
+  return CSM_OK;
+}
+
 // After the tuning batch: keep the candidate with the least search time for
 // the engine's lifetime.
 void tune_finish(csm_engine* e) {
@@ -1619,6 +1741,11 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
     return nullptr;
   }
   if (hipHostMalloc((void**)&e->h_ctrl, sizeof(Ctrl)) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  if (hipHostMalloc((void**)&e->h_parse, 8 * sizeof(uint64_t)) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  // the allocation-free parse's scratch on this device, and room for error
+  // texts so that setting one allocates nothing
+  if ((rc = scratch_init(e->dev))) { set(rc); return nullptr; }
+  e->err.reserve(512);
   if (hipMemcpy(e->d_vlab, e->h_vlab.data(), (size_t)e->nv_cap * 4, hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(e->d_elmap, e->el_map.data(), e->el_map.size() * 4, hipMemcpyHostToDevice) !=
@@ -2105,6 +2232,14 @@ int csm_parse_text_device(csm_engine* e, int mode, const char* d_text, size_t le
   return CSM_OK;
 }
 
+int csm_parse_text_device_static(csm_engine* e, int mode, const char* d_text, size_t len,
+                                 csm_update* d_out, size_t cap, size_t* n, uint64_t* err_line) {
+  if (!e || !n || (len && !d_text)) return CSM_E_ARG;
+  HIP_TRY(e, hipSetDevice(e->dev));
+  return parse_static(e, mode, reinterpret_cast<const uint8_t*>(d_text), len, d_out, cap, n,
+                      err_line);
+}
+
 int csm_parse_text(int device, int mode, const char* text, size_t len, csm_update* out,
                    size_t cap, size_t* n, uint64_t* err_line) {
   if (!n || (len && !text)) return CSM_E_ARG;
@@ -2163,6 +2298,7 @@ void csm_destroy(csm_engine* e) {
   if (e->h_ustatus_pin) (void)hipHostFree(e->h_ustatus_pin);
   if (e->h_uwork_pin) (void)hipHostFree(e->h_uwork_pin);
   if (e->h_abort) (void)hipHostFree(e->h_abort);
+  if (e->h_parse) (void)hipHostFree(e->h_parse);
   for (auto& ev : e->ev_search)
     if (ev) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);

@@ -110,6 +110,8 @@ def lib():
                                               P(u64)]),
             "csm_parse_text_device": (ctypes.c_int, [vp, ctypes.c_int, vp, sz, vp, sz, P(sz),
                                                      P(u64)]),
+            "csm_parse_text_device_static": (ctypes.c_int, [vp, ctypes.c_int, vp, sz, vp, sz,
+                                                            P(sz), P(u64)]),
             "csm_last_error": (ctypes.c_char_p, [vp]),
             "csm_destroy": (None, [vp]),
         }
@@ -130,7 +132,8 @@ def exported_symbols() -> Sequence[str]:
             "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
             "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
             "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_parse_text",
-            "csm_parse_text_device", "csm_last_error", "csm_destroy")
+            "csm_parse_text_device", "csm_parse_text_device_static", "csm_last_error",
+            "csm_destroy")
 
 
 def _u32(a) -> np.ndarray:
@@ -360,12 +363,13 @@ class Matching:
         self._check(lib().csm_sync(self.h))
 
     def parse_text_device(self, d_text_ptr: int, nbytes: int, mode: int, d_out_ptr: int,
-                          cap: int) -> int:
+                          cap: int, static: bool = False) -> int:
         """Text in HBM to device records (see csm_parse_text_device); returns
-        the record count."""
+        the record count.  static=True: the allocation-free entry point
+        (csm_parse_text_device_static)."""
         n, bad = ctypes.c_size_t(0), ctypes.c_uint64(0)
-        rc = lib().csm_parse_text_device(self.h, mode, d_text_ptr, nbytes, d_out_ptr, cap,
-                                         ctypes.byref(n), ctypes.byref(bad))
+        f = lib().csm_parse_text_device_static if static else lib().csm_parse_text_device
+        rc = f(self.h, mode, d_text_ptr, nbytes, d_out_ptr, cap, ctypes.byref(n), ctypes.byref(bad))
         if rc == -1 and bad.value:
             raise ParseError(rc, int(bad.value))
         self._check(rc)
