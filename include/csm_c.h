@@ -99,6 +99,8 @@ typedef struct csm_opts {
   uint32_t want_work;       /* 1: count the candidate entries each update's
                                search tests (csm_update_work), the size of
                                its search tree                             */
+  uint32_t no_graphs;       /* 1: never replay steady-state batches as HIP
+                               graphs (every launch issued one by one)     */
 } csm_opts;
 
 /* Creates an engine and uploads the data graph.  `algo` is one of the
@@ -166,10 +168,16 @@ int csm_sync(csm_engine* e);
 /* Per update of the last csm_apply_batch / csm_apply_batch_device call (or
  * entry 0 after csm_initial): bit CSM_USTAT_TIME_LIMIT = the time limit
  * stopped part of its search (its counts are lower bounds); bit
- * CSM_USTAT_CAPPED = a count of it reached max_results.  No reference
- * counterpart (the reference reports one flag for the whole run). */
+ * CSM_USTAT_CAPPED = a count of it reached max_results; bit
+ * CSM_USTAT_CERTIFIED = it passed the label filter (Classify) but was
+ * certified before its search to change no count: for every seed, some other
+ * query neighbour of a seed endpoint has no candidate with its labels among
+ * that endpoint's neighbours (the safe-update classifier's second stage).
+ * No reference counterpart (the reference reports one flag for the whole
+ * run). */
 #define CSM_USTAT_TIME_LIMIT 1
 #define CSM_USTAT_CAPPED 2
+#define CSM_USTAT_CERTIFIED 4
 int csm_update_status(const csm_engine* e, uint8_t* status, size_t n);
 
 /* Per update of the last csm_apply_batch_device call (needs want_work):
@@ -195,7 +203,8 @@ int csm_totals(const csm_engine* e, uint64_t* initial, uint64_t* positive, uint6
 /* matching::GetMemoryCost (matching.h:53): edges and vertices held. */
 int csm_memory(const csm_engine* e, size_t* num_edges, size_t* num_vertices);
 
-/* Device bytes currently held by the engine (admission control). */
+/* Device bytes currently held by the engine (admission control).  Reads one
+ * word, so a serving thread may call it while a batch call runs. */
 int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
 
 /* Engine statistics (cumulative): [0] search tasks run, [1] candidate
@@ -205,7 +214,7 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
  * the engine stream), [6] search launches timed, [7] most search rounds one
  * batch needed, [8] split_chunk and [9] spill budget in use, [10] 1 while
  * the autotuner (auto_tuning != 0 with neither knob set) is still trying
- * candidates on large batches. */
+ * candidates on large batches, [11] batches replayed as HIP graphs. */
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
 
 /* Diagnostic read-back of the candidate index of one query (parallel_symbi
@@ -233,6 +242,28 @@ int csm_copy_device(csm_engine* e, void* dst, const void* src, size_t bytes);
 
 /* The HIP stream the engine launches on (hipStream_t). */
 void* csm_stream(csm_engine* e);
+
+/* Overlapping copies with the engine's work (serving).  A second HIP stream,
+ * the transfer stream, carries asynchronous copies; markers (slots
+ * 0..CSM_MARKS-1) order the two streams: csm_mark records a marker on a
+ * stream, csm_wait_mark makes a stream wait for it on the device, and
+ * csm_sync_mark waits for it on the host.  Host memory passed to
+ * csm_copy_async must be pinned (csm_host_alloc).  These calls touch only the
+ * transfer stream and the markers, so a second host thread may make them
+ * while a batch call runs, provided the two threads use different slots.
+ * No reference counterpart (the reference's driver is a serial loop,
+ * main.cpp:253-397). */
+#define CSM_STREAM_ENGINE 0
+#define CSM_STREAM_TRANSFER 1
+#define CSM_MARKS 16
+int csm_copy_async(csm_engine* e, void* dst, const void* src, size_t bytes);
+int csm_mark(csm_engine* e, int stream, uint32_t slot);
+int csm_wait_mark(csm_engine* e, int stream, uint32_t slot);
+int csm_sync_mark(csm_engine* e, uint32_t slot);
+
+/* Free and total device memory (hipMemGetInfo) on the engine's device: the
+ * serving loop's HBM admission.  No reference counterpart. */
+int csm_mem_info(csm_engine* e, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* Text ingestion on the device.  Parses the reference's input formats, one
  * record per line, into csm_update records:

@@ -126,7 +126,21 @@ struct Graph {
 struct Opts {
   bool homo = false;
   bool want_hash = false;
+  // per-update result cap (0: none): the search of an update stops once its
+  // count reaches the cap, and the count reported is min(exact, cap); the
+  // reference checks max_num_results_ after each candidate
+  // (parallel_graphflow.cpp:386) or seed (graphflow.cpp:328, 349) and keeps
+  // the count it had reached
+  uint64_t max_results = 0;
+  // time limit of one update's search (steady-clock seconds; 0: none): the
+  // reference's reach_time_limit polls (graphflow.cpp:329, 350); once it
+  // passes, the search unwinds and the update is reported unfinished
+  double deadline = 0;
 };
+
+inline double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 // ------------------------------------------------------------- GraphFlow
 // ParaCOSM/CSM/matching/GraphFlow/graphflow.cpp
@@ -137,6 +151,8 @@ struct GraphFlow {
   uint32_t k;
   std::vector<std::vector<uint32_t>> ovs, ocsr, ooff;  // order_vs_/csrs_/offs_
   std::vector<uint8_t> visited;
+  mutable uint64_t polls = 0;
+  mutable bool timed_out = false;
 
   GraphFlow(const Graph& q_, Graph& d_, Opts o_) : q(q_), d(d_), o(o_), k(q_.nv()) {
     uint32_t ne = (uint32_t)q.ne;
@@ -204,6 +220,10 @@ struct GraphFlow {
   // graphflow.cpp:181-265 (visited via the m array when visited is null)
   void find(uint32_t oi, uint32_t depth, std::vector<uint32_t>& m, uint64_t& cnt,
             uint64_t& h, std::vector<uint8_t>& vis) const {
+    if (o.deadline > 0) {
+      if (timed_out) return;
+      if ((++polls & 4095) == 0 && now_s() >= o.deadline) { timed_out = true; return; }
+    }
     uint32_t u = ovs[oi][depth];
     uint32_t u_min = NOT_EXIST, u_min_label = NOT_EXIST, u_min_size = UINT32_MAX;
     const auto& qn = q.nb[u];
@@ -237,6 +257,8 @@ struct GraphFlow {
       }
       vis[v] = 0;
       m[u] = NOT_EXIST;
+      if (o.max_results && cnt >= o.max_results) return;
+      if (timed_out) return;
     }
   }
 
@@ -271,7 +293,10 @@ struct GraphFlow {
       }
       visited[s[1]] = visited[s[2]] = 0;
       m[u1] = m[u2] = NOT_EXIST;
+      if (o.max_results && cnt >= o.max_results) break;
+      if (timed_out) break;
     }
+    if (o.max_results && cnt > o.max_results) cnt = o.max_results;
   }
 
   int add_edge(uint32_t v1, uint32_t v2, uint32_t l, uint64_t& cnt, uint64_t& h) {
@@ -1053,13 +1078,16 @@ uint64_t orc_brute_count_filtered(const orc_graph* d, const orc_graph* q, int ho
 int orc_run(const char* algo, const orc_graph* data, const orc_graph* query, size_t n,
             const uint8_t* add, const uint32_t* v1, const uint32_t* v2, const uint32_t* lab,
             uint64_t* pos, uint64_t* neg, uint64_t* hash, int threads, int homo,
-            uint64_t* initial, int check_index, double* seconds, size_t* fail_at) {
+            uint64_t* initial, int check_index, double* seconds, size_t* fail_at,
+            uint64_t max_results) {
   Graph d = data->g;
   const Graph& q = query->g;
   Opts o;
   o.homo = homo != 0;
   o.want_hash = hash != nullptr;
+  o.max_results = max_results;
   std::string a(algo);
+  if (max_results && a != "graphflow") return -4;   // the cap is restated for GraphFlow only
   auto t0 = std::chrono::steady_clock::now();
   auto finish = [&](int rc) {
     if (seconds)
@@ -1134,6 +1162,43 @@ int orc_run(const char* algo, const orc_graph* data, const orc_graph* query, siz
     return finish(0);
   }
   return -1;
+}
+
+// Sequential GraphFlow over a stream with a per-update result cap and time
+// limits: each update's search gets at most `update_s` seconds and the whole
+// run `total_s` (0: none); an update whose search ran out of time has
+// unfinished[i] = 1 (its count is a lower bound), and once the total is spent
+// the remaining updates are applied without a search (unfinished).  The graph
+// is updated either way.  Returns 0, or -2 at a deletion of an absent edge.
+int orc_graphflow_limited(const orc_graph* data, const orc_graph* query, size_t n,
+                          const uint8_t* add, const uint32_t* v1, const uint32_t* v2,
+                          const uint32_t* lab, uint64_t* count, uint8_t* unfinished,
+                          uint64_t max_results, double update_s, double total_s) {
+  Graph d = data->g;
+  Opts o;
+  o.max_results = max_results;
+  GraphFlow gf(query->g, d, o);
+  const double t_end = total_s > 0 ? now_s() + total_s : 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t c = 0, h = 0;
+    const double t = now_s();
+    bool skip = t_end > 0 && t >= t_end;
+    double dl = update_s > 0 ? t + update_s : 0.0;
+    if (t_end > 0 && (dl == 0.0 || dl > t_end)) dl = t_end;
+    gf.o.deadline = dl;
+    gf.timed_out = false;
+    uint32_t l = lab[i];
+    if (!add[i]) {
+      l = d.elabel(v1[i], v2[i]);
+      if (l == NOT_EXIST) return -2;
+    }
+    if (add[i]) d.add_edge(v1[i], v2[i], l);
+    if (!skip) gf.search_update(v1[i], v2[i], l, c, h);
+    if (!add[i]) d.remove_edge(v1[i], v2[i]);
+    count[i] = c;
+    unfinished[i] = (skip || gf.timed_out) ? 1 : 0;
+  }
+  return 0;
 }
 
 // SymBi index built from scratch on `data` (symbi.cpp:246-377), dumped in the

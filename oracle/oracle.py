@@ -44,7 +44,7 @@ def lib():
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                               ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-                              ctypes.POINTER(ctypes.c_size_t)]
+                              ctypes.POINTER(ctypes.c_size_t), ctypes.c_uint64]
         L.orc_symbi_dump.restype = ctypes.c_long
         L.orc_symbi_dump.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, _u32p,
                                      ctypes.c_size_t]
@@ -55,6 +55,11 @@ def lib():
         L.orc_d2_filtered_count.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.orc_dag_parents.restype = ctypes.c_uint32
         L.orc_dag_parents.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _u32p]
+        L.orc_graphflow_limited.restype = ctypes.c_int
+        L.orc_graphflow_limited.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                            _u8p, _u32p, _u32p, _u32p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double,
+                                            ctypes.c_double]
         L.orc_graphflow_orders.restype = ctypes.c_int
         L.orc_graphflow_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p]
         _lib = L
@@ -103,7 +108,8 @@ class RunResult:
 
 def run(algo: str, data: OGraph, query: OGraph, add, v1, v2, lab, *, threads: int = 1,
         homo: bool = False, want_hash: bool = False, want_initial: bool = False,
-        check_index: bool = False, limit: int | None = None) -> RunResult:
+        check_index: bool = False, limit: int | None = None,
+        max_results: int = 0) -> RunResult:
     add = np.ascontiguousarray(np.asarray(add, dtype=np.uint8))
     v1, v2, lab = _u32(v1), _u32(v2), _u32(lab)
     n = len(v1) if limit is None else min(limit, len(v1))
@@ -117,7 +123,9 @@ def run(algo: str, data: OGraph, query: OGraph, add, v1, v2, lab, *, threads: in
                        pos.ctypes.data, neg.ctypes.data,
                        hsh.ctypes.data if hsh is not None else None,
                        threads, int(homo), ctypes.byref(init) if want_initial else None,
-                       int(check_index), ctypes.byref(secs), ctypes.byref(fail))
+                       int(check_index), ctypes.byref(secs), ctypes.byref(fail), max_results)
+    if rc == -4:
+        raise ValueError("max_results is supported for algo 'graphflow' only")
     if rc == -1:
         raise ValueError(f"unknown algorithm {algo!r}")
     if rc == -2:
@@ -127,6 +135,24 @@ def run(algo: str, data: OGraph, query: OGraph, add, v1, v2, lab, *, threads: in
     if rc != 0:
         raise RuntimeError(f"oracle returned {rc}")
     return RunResult(pos, neg, hsh, int(init.value) if want_initial else None, secs.value)
+
+
+def graphflow_limited(data: OGraph, query: OGraph, add, v1, v2, lab, *, limit: int,
+                      max_results: int = 0, update_seconds: float = 0.0,
+                      total_seconds: float = 0.0):
+    """Sequential GraphFlow over the first `limit` updates with a per-update
+    result cap and time limits: (counts, unfinished) where unfinished marks
+    the updates whose search ran out of time (see orc_graphflow_limited)."""
+    add = np.ascontiguousarray(np.asarray(add, dtype=np.uint8))
+    v1, v2, lab = _u32(v1), _u32(v2), _u32(lab)
+    n = min(limit, len(v1))
+    cnt = np.zeros(n, np.uint64)
+    unf = np.zeros(n, np.uint8)
+    rc = lib().orc_graphflow_limited(data.h, query.h, n, add, v1, v2, lab, cnt.ctypes.data,
+                                     unf.ctypes.data, max_results, update_seconds, total_seconds)
+    if rc == -2:
+        raise RuntimeError("deletion error")
+    return cnt, unf.astype(bool)
 
 
 def graphflow_orders(data: OGraph, query: OGraph, k: int, ne: int) -> np.ndarray:

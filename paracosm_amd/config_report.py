@@ -17,6 +17,17 @@ buckets).  No query admission rule applies.
 
     python -m paracosm_amd.config_report --config c3 --mode large --time-limit 60
 
+--mode serve: the serving loop (config c4: scale-21 graph, 70/30 stream, 64
+standing random-walk queries of 6-16 vertices, 6 per size in seeded order,
+no admission rule).  Requests of log-uniform size in 1..4096 arrive as a
+Poisson process at each offered load; batches are admitted by free HBM and
+each batch's search runs under a time limit (--time-limit, per batch), so
+every request completes.  Reported per load: updates/s, p50/p99 latency,
+the fraction of updates whose counts are complete, and per query the
+fraction of batches it completed within the limit.
+
+    python -m paracosm_amd.config_report --config c4 --mode serve --time-limit 0.2
+
 Prints one JSON line; progress goes to stderr.
 """
 from __future__ import annotations
@@ -46,7 +57,12 @@ def main(argv=None) -> int:
     ap.add_argument("--sample", type=int, default=1_000_000, help="stream updates run")
     ap.add_argument("--batch", type=int, default=62500)
     ap.add_argument("--scale", type=int, default=None)
-    ap.add_argument("--mode", default="rate", choices=("rate", "large"))
+    ap.add_argument("--mode", default="rate", choices=("rate", "large", "serve"))
+    ap.add_argument("--loads", default="1e4,1e5,1e6,1e7",
+                    help="serve: offered loads in updates/s")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="serve: seconds of arrivals per offered load")
+    ap.add_argument("--queries", type=int, default=64, help="serve: standing queries")
     ap.add_argument("--time-limit", type=float, default=60.0,
                     help="large: seconds per query for the whole stream sample")
     ap.add_argument("--per-size", type=int, default=0,
@@ -54,6 +70,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.mode == "large":
         return large_queries(args)
+    if args.mode == "serve":
+        return serve_loads(args)
 
     t0 = time.time()
     d = workloads.build(args.config, seed=args.seed, scale=args.scale, progress=log)
@@ -198,6 +216,67 @@ def large_queries(args) -> int:
            "updates": n, "batch": args.batch, "time_limit_s_per_query": args.time_limit,
            "work_unit": "candidate entries tested by the search of one update",
            "sizes": sizes}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def serve_loads(args) -> int:
+    """Config c4: the serving loop over all standing queries at several
+    offered loads, each on a fresh engine from the initial graph."""
+    from . import workloads
+    from .engine import Matching, make_updates
+    from .serve import Server, poisson_requests
+    t0 = time.time()
+    d = workloads.build(args.config, seed=args.seed, scale=args.scale, progress=log)
+    w = d.w
+    log(f"workload {args.config}: {w.nv} vertices, {len(w.init_src)} initial edges, "
+        f"{w.num_updates} updates in {time.time() - t0:.1f}s")
+    per = args.per_size or d.cfg.per_size
+    qs = []
+    for k in d.cfg.query_sizes:
+        qs += workloads.candidates(d, k, None, per, 0)
+    qs = qs[: args.queries]
+    log(f"{len(qs)} standing queries, k = {[q.k for q in qs]}")
+    ups = make_updates(w.upd_add, w.upd_v1, w.upd_v2, w.upd_label)
+    loads = []
+    for tok in [x for x in args.loads.split(",") if x]:
+        load = float(tok)
+        n = int(min(args.serve_seconds * load, w.num_updates))
+        tb = time.time()
+        m = Matching(args.algo, qs[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                     max_batch=args.batch, time_limit_s=args.time_limit, split_chunk=64,
+                     spill_budget=512)
+        for q in qs[1:]:
+            m.register_query(q)
+        m.Preprocessing()
+        build = time.time() - tb
+        reqs = poisson_requests(ups[:n], load, seed=args.seed)
+        srv = Server(m, max_batch=args.batch)
+        st = srv.serve(reqs)
+        budget = srv.hbm_budget
+        srv.close()
+        engine_bytes = m.device_bytes()
+        m.close()
+        qpb = np.asarray(st.query_partial_batches, np.float64)
+        loads.append({
+            "offered_updates_per_s": load, "updates": st.updates, "requests": st.requests,
+            "batches": st.batches, "mean_batch": round(st.mean_batch, 1),
+            "updates_per_s": round(st.updates_per_s, 1),
+            "achieved_over_offered": round(st.updates_per_s / load, 3),
+            "p50_update_latency_ms": round(st.p50_ms, 3),
+            "p99_update_latency_ms": round(st.p99_ms, 3),
+            "complete_update_fraction": round(1.0 - st.partial_updates / max(st.updates, 1), 5),
+            "queries_complete_in_every_batch": int((qpb == 0).sum()),
+            "query_batch_completion": [round(1.0 - x / max(st.batches, 1), 4) for x in qpb],
+            "matches_counted": st.matches, "overlapped_uploads": st.overlapped_uploads,
+            "hbm_budget": budget, "engine_device_bytes": engine_bytes,
+            "refused_for_memory": st.refused_for_memory, "engine_build_s": round(build, 2)})
+        log(f"load {load:g}: " + json.dumps({k: v for k, v in loads[-1].items()
+                                             if k != "query_batch_completion"}))
+    out = {"config": d.cfg.description, "mode": "serve", "algo": args.algo, "seed": args.seed,
+           "queries": len(qs), "query_sizes": [q.k for q in qs],
+           "time_limit_s_per_batch": args.time_limit, "max_batch": args.batch,
+           "admission": "none for queries; batches admitted by free HBM", "loads": loads}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -193,12 +193,33 @@ struct csm_engine {
   // per update of the last call: candidate entries the search tested
   // (csm_opts.want_work)
   unsigned long long* d_uwork = nullptr;
+  SearchLimits* d_limits = nullptr;      // the search's SearchLimits (device copy)
   unsigned long long* h_uwork_pin = nullptr;
   std::vector<uint64_t> h_uwork;
   uint32_t* h_ustatus_pin = nullptr;
 
+  // steady-state batches replayed as HIP graphs (see run_batch_graph); gen
+  // counts device allocations, whose addresses a captured graph bakes in
+  uint64_t gen = 0;
+  struct GraphKey {
+    uint32_t n, nq, dcur, cur, hash, nv, chunk, budget, budgeted, capped;
+    uint64_t gen;
+    bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof *this) == 0; }
+  };
+  struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec = nullptr;
+    uint32_t rounds = 0;      // search rounds inside the graph
+    bool bad = false;         // capture failed: this key runs eagerly
+  };
+  std::vector<GraphEntry> graphs;
+  uint64_t graph_launches = 0;
   // pinned words for the allocation-free parse's read-backs
   uint64_t* h_parse = nullptr;
+  // transfer stream (default priority, like the engine stream) and markers
+  // for overlapping host-device copies with the engine's work
+  hipStream_t xfer = nullptr;
+  hipEvent_t marks[CSM_MARKS] = {};
 
   std::vector<DevBuf> allocs;
 };
@@ -276,6 +297,7 @@ int dalloc(csm_engine* e, T** p, size_t count) {
     return fail(e, CSM_E_DEVICE, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st));
   e->allocs.push_back({q, bytes});
   e->dev_bytes += bytes;
+  ++e->gen;
   *p = static_cast<T*>(q);
   return CSM_OK;
 }
@@ -676,6 +698,24 @@ int alloc_batch(csm_engine* e) {
   return CSM_OK;
 }
 
+// The search's SearchLimits: the addresses of the time-limit word, the
+// status, work and per-query arrays, and the result cap (all fixed between
+// reallocations, which call this again).
+int upload_limits(csm_engine* e) {
+  if (!e->d_limits) {
+    int rc = dalloc(e, &e->d_limits, 1);
+    if (rc) return rc;
+  }
+  SearchLimits l;
+  l.habort = e->d_abort_map;
+  l.ustatus = e->d_ustatus;
+  l.qtl = e->d_qtl;
+  l.uwork = e->d_uwork;
+  l.ucap = e->opts.max_results;
+  HIP_TRY(e, hipMemcpy(e->d_limits, &l, sizeof l, hipMemcpyHostToDevice));
+  return CSM_OK;
+}
+
 int ensure_results(csm_engine* e) {
   size_t need = (size_t)e->max_batch * std::max<size_t>(1, e->queries.size());
   static_assert(sizeof(unsigned long long) == 8, "u64");
@@ -713,7 +753,7 @@ int ensure_results(csm_engine* e) {
   e->h_qtl.assign(e->queries.size(), 0);
   HIP_TRY(e, hipHostMalloc((void**)&e->h_counts, need * 8));
   HIP_TRY(e, hipHostMalloc((void**)&e->h_hashes, need * 8));
-  return CSM_OK;
+  return upload_limits(e);
 }
 
 int read_ctrl(csm_engine* e) {
@@ -785,17 +825,12 @@ void launch_round(csm_engine* e, SearchArgs& sa, bool versioned, bool h) {
 void collect_search_time(csm_engine* e);
 int read_ctrl(csm_engine* e);
 
-// Search rounds: round 0 runs the seeds; a task that exceeds its iteration
-// budget spills the rest of its DFS into the next round's task list, and
-// rounds repeat until nothing spills.
-int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned long long* hashes,
-           uint32_t pos_offset = 0) {
+// The search arguments of one batch (round buffers are set per round).
+SearchArgs search_args(csm_engine* e, unsigned long long* counts, unsigned long long* hashes,
+                       uint32_t pos_offset) {
   SearchArgs sa;
   sa.g = view(e);
   sa.plans = e->d_plans;
-  sa.tasks = e->d_seeds;
-  sa.chunk_owner = e->d_chunk_owner;
-  sa.chunk_cap = e->chunk_cap;
   sa.ctrl = e->d_ctrl;
   sa.counts = counts;
   sa.hashes = hashes;
@@ -811,18 +846,42 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
   sa.spill_cap = e->spill_cap;
   sa.spill_chunk_cap = e->spill_chunk_cap;
   sa.budget_iters = e->budget_iters;
-  sa.habort = e->deadline > 0 ? e->d_abort_map : nullptr;
-  sa.ustatus = e->d_ustatus;
-  sa.ucap = e->opts.max_results;
-  sa.uwork = e->opts.want_work ? e->d_uwork : nullptr;
-  sa.qtl = e->d_qtl;
+  sa.lflags = (e->deadline > 0 ? LIM_TIME : 0u) | (e->opts.max_results ? LIM_CAP : 0u) |
+              (e->opts.want_work ? LIM_WORK : 0u);
+  sa.lim = e->d_limits;
+  return sa;
+}
+
+// Round r reads the tasks round r-1 spilled (round 0: the seeds) and spills
+// into the other buffer of the ping-pong pair.
+void round_buffers(csm_engine* e, SearchArgs& sa, uint32_t round) {
+  if (round == 0) {
+    sa.tasks = e->d_seeds;
+    sa.chunk_owner = e->d_chunk_owner;
+    sa.chunk_cap = e->chunk_cap;
+  } else {
+    const int in = (int)((round - 1) & 1u);
+    sa.tasks = e->d_spill[in];
+    sa.chunk_owner = e->d_spill_owner[in];
+    sa.chunk_cap = e->spill_chunk_cap;
+  }
+  const int out = (int)(round & 1u);
+  sa.spill = e->d_spill[out];
+  sa.spill_owner = e->d_spill_owner[out];
+  sa.xregions = round > 0 ? 1u : 0u;
+}
+
+// Search rounds from `first` on: a task that exceeds its iteration budget
+// spills the rest of its DFS into the next round's task list, and rounds
+// repeat until nothing spills.  The round's chunk counts are in place on the
+// device (k_seeds for round 0, k_next_round after each round).
+int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned long long* hashes,
+           uint32_t pos_offset = 0, uint32_t first = 0) {
+  SearchArgs sa = search_args(e, counts, hashes, pos_offset);
   const bool h = hashes != nullptr;
-  int out = 0;
   static const bool trace = getenv("CSM_TRACE_ROUNDS") != nullptr;
-  for (uint32_t round = 0;; ++round) {
-    sa.spill = e->d_spill[out];
-    sa.spill_owner = e->d_spill_owner[out];
-    sa.xregions = round > 0 ? 1u : 0u;
+  for (uint32_t round = first;; ++round) {
+    round_buffers(e, sa, round);
     uint32_t chunks_in = trace ? e->h_ctrl->nspill_chunks : 0;
     check_deadline(e);   // after the limit, rounds only drain their chunks
     launch_round(e, sa, versioned, h);
@@ -860,10 +919,6 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
       return CSM_OK;
     }
     k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl, e->spill_chunk_cap);
-    sa.tasks = e->d_spill[out];
-    sa.chunk_owner = e->d_spill_owner[out];
-    sa.chunk_cap = e->spill_chunk_cap;
-    out = 1 - out;
     if (round > 100000) return fail(e, CSM_E_DEVICE, "search did not converge");
   }
 }
@@ -904,6 +959,7 @@ void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long 
   a.counts = counts;
   a.hashes = hashes;
   a.safe = safe;
+  a.ustatus = e->d_ustatus;
   k_seeds<<<grid_for((uint64_t)n * WAVE, 256, 8192), 256, 0, e->stream>>>(a);
 }
 
@@ -938,8 +994,9 @@ int symbi_add_vertex(csm_engine* e, uint32_t id);
 int dir_reserve(csm_engine* e, uint32_t n);
 void tune_finish(csm_engine* e);
 
-int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
-              uint8_t* safe, bool search_matches = true) {
+// Host decisions before a batch: room in the adjacency pool (repack) and in
+// the key-directory pool.
+int batch_prepare(csm_engine* e, uint32_t n) {
   int rc;
   // room for this batch's rewrites: at most every list twice plus the new entries
   uint64_t live = e->h_ctrl->live;
@@ -948,27 +1005,36 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
     if ((uint64_t)e->h_ctrl->pool_top + 2 * live + 2ull * n + 1024 > e->pool_cap)
       return fail(e, CSM_E_CAPACITY, "adjacency pool too small");
   }
-  if ((rc = dir_reserve(e, n))) return rc;
+  return dir_reserve(e, n);
+}
+
+// Zero a batch's result arrays with fill kernels (no memset, so that a
+// captured graph replays them exactly).
+void batch_clear(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes) {
   const size_t nq = e->queries.size();
-  HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
-  if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
+  const uint64_t w = 2ull * n * nq;   // u32 words of n x nq u64
+  k_fill_u32<<<grid_for(w, 256), 256, 0, e->stream>>>(reinterpret_cast<uint32_t*>(counts), w, 0u);
+  if (hashes)
+    k_fill_u32<<<grid_for(w, 256), 256, 0, e->stream>>>(reinterpret_cast<uint32_t*>(hashes), w, 0u);
   k_fill_u32<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_ustatus, n, 0u);
-  k_fill_u32<<<1, 256, 0, e->stream>>>(e->d_qtl, nq, 0u);
   if (e->d_uwork)
     k_fill_u32<<<grid_for(2ull * n, 256), 256, 0, e->stream>>>(
         reinterpret_cast<uint32_t*>(e->d_uwork), 2ull * n, 0u);
   if (e->opts.match_budget) {
-    HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
-    HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
+    k_fill_u32<<<1, 256, 0, e->stream>>>(reinterpret_cast<uint32_t*>(e->d_qcount), 2 * nq, 0u);
+    k_fill_u32<<<1, 256, 0, e->stream>>>(e->d_qabort, nq, 0u);
   }
+}
+
+// The graph and index update of a batch whose records are unpacked.
+int batch_update(csm_engine* e, uint32_t n) {
+  int rc;
+  const size_t nq = e->queries.size();
+  k_fill_u32<<<1, 256, 0, e->stream>>>(e->d_qtl, nq, 0u);
   // 1. compact lists dirtied by the previous batch
-  {
-    GraphView g = view(e);
-    k_compact<APPLY_BS><<<2048, APPLY_BS, 0, e->stream>>>(
-        e->d_vrec, e->d_adj[e->cur], e->d_ats[e->cur], e->d_dirty[e->dcur], &e->d_ctrl->ndirty_cur,
-        e->d_ctrl, (uint32_t)e->pool_cap);
-    (void)g;
-  }
+  k_compact<APPLY_BS><<<2048, APPLY_BS, 0, e->stream>>>(
+      e->d_vrec, e->d_adj[e->cur], e->d_ats[e->cur], e->d_dirty[e->dcur], &e->d_ctrl->ndirty_cur,
+      e->d_ctrl, (uint32_t)e->pool_cap);
   // 2. merge insertions
   BatchView b = batch_view(e, n);
   k_batch_keys<<<grid_for(n, 256), 256, 0, e->stream>>>(b, e->d_vlab, e->nel, e->vbits, e->nv,
@@ -996,25 +1062,44 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   // 4. SymBi index: insertions (before the search) ...
   if (uses_index(e) && e->d_d2)
     if ((rc = symbi_batch_insert(e, n))) return rc;
+  return CSM_OK;
+}
+
+// After the search: the result cap, the index's deletions, the batch's end.
+int batch_tail(csm_engine* e, uint32_t n, unsigned long long* counts, bool searched) {
+  int rc;
+  const size_t nq = e->queries.size();
+  if (searched && e->opts.max_results)
+    k_clamp_counts<<<grid_for((uint64_t)n * nq, 256), 256, 0, e->stream>>>(
+        counts, n, (uint32_t)nq, e->opts.max_results, e->d_ustatus);
+  // 6. ... and deletions (after it)
+  if (uses_index(e) && e->d_d2)
+    if ((rc = symbi_batch_delete(e, n))) return rc;
+  // next batch compacts what this one dirtied
+  k_batch_end<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+  e->dcur = 1 - e->dcur;
+  ++e->stats[3];
+  return CSM_OK;
+}
+
+// The batch pipeline; k_batch_begin has run and the updates are unpacked into
+// d_type/d_add/d_v1/d_v2/d_lab.  With search_matches == false the graph and
+// the SymBi index are updated but no match is searched (vertex deletion).
+int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
+              uint8_t* safe, bool search_matches = true) {
+  int rc;
+  if ((rc = batch_prepare(e, n))) return rc;
+  batch_clear(e, n, counts, hashes);
+  if ((rc = batch_update(e, n))) return rc;
   // 5. seeds + search
   if (search_matches) {
     if (e->tune_active && n >= TUNE_MIN_BATCH && e->tune_idx > 0) {
       // autotuning: the search alone runs once per candidate setting on this
       // same batch (identical work and identical counts each time)
-      const size_t nq = e->queries.size();
       for (size_t i = 0; i < e->tune_cands.size(); ++i) {
         if (i) {
           k_search_reset<<<1, 64, 0, e->stream>>>(e->d_ctrl);
-          HIP_TRY(e, hipMemsetAsync(counts, 0, (size_t)n * nq * 8, e->stream));
-          if (hashes) HIP_TRY(e, hipMemsetAsync(hashes, 0, (size_t)n * nq * 8, e->stream));
-          if (e->opts.match_budget) {
-            HIP_TRY(e, hipMemsetAsync(e->d_qcount, 0, nq * 8, e->stream));
-            HIP_TRY(e, hipMemsetAsync(e->d_qabort, 0, nq * 4, e->stream));
-          }
-          k_fill_u32<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_ustatus, n, 0u);
-          if (e->d_uwork)
-            k_fill_u32<<<grid_for(2ull * n, 256), 256, 0, e->stream>>>(
-                reinterpret_cast<uint32_t*>(e->d_uwork), 2ull * n, 0u);
+          batch_clear(e, n, counts, hashes);
         }
         e->opts.split_chunk = e->tune_cands[i].first;
         e->budget_iters = e->tune_cands[i].second;
@@ -1029,17 +1114,164 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
       if ((rc = search(e, true, counts, hashes))) return rc;
       if (e->tune_active && n >= TUNE_MIN_BATCH) ++e->tune_idx;   // the settling batch
     }
-    if (e->opts.max_results)
-      k_clamp_counts<<<grid_for((uint64_t)n * nq, 256), 256, 0, e->stream>>>(
-          counts, n, (uint32_t)nq, e->opts.max_results, e->d_ustatus);
   }
-  // 6. ... and deletions (after it)
-  if (uses_index(e) && e->d_d2)
-    if ((rc = symbi_batch_delete(e, n))) return rc;
-  // next batch compacts what this one dirtied
-  k_batch_end<<<1, 64, 0, e->stream>>>(e->d_ctrl);
-  e->dcur = 1 - e->dcur;
-  ++e->stats[3];
+  return batch_tail(e, n, counts, search_matches);
+}
+
+// ------------------------------------------------------- batches as graphs
+// Steady-state batches (autotuning done, no time limit) run as a HIP graph:
+// k_batch_begin, unpack, the graph and index update, the seeds and a fixed
+// number of search rounds (each followed by k_next_round, so that a round
+// with nothing to do ends at once), replayed with one launch.  The graph
+// reads and writes the engine's own staging buffers (d_upd, d_counts,
+// d_safe, d_hashes), so a caller's buffers are copied in and out around it,
+// and is keyed by everything its launches bake in (batch size, ping-pong
+// buffers, allocations).  Rounds still pending after it continue eagerly.
+// The first batch of a key runs eagerly (sizes and allocations settle), the
+// second is captured.
+constexpr size_t MAX_GRAPHS = 8;
+
+bool graphs_allowed(const csm_engine* e, uint32_t n) {
+  static const bool off = getenv("CSM_NO_GRAPHS") != nullptr || getenv("CSM_TRACE_ROUNDS") != nullptr;
+  // batches the autotuner still scores run eagerly; smaller ones never tune
+  const bool tuning = e->tune_active && n >= TUNE_MIN_BATCH;
+  return !off && !e->opts.no_graphs && !tuning && e->deadline <= 0 && e->algo != ALGO_NONE;
+}
+
+csm_engine::GraphKey graph_key(const csm_engine* e, uint32_t n, bool hash) {
+  csm_engine::GraphKey k;
+  std::memset(&k, 0, sizeof k);
+  k.n = n;
+  k.nq = (uint32_t)e->queries.size();
+  k.dcur = (uint32_t)e->dcur;
+  k.cur = (uint32_t)e->cur;
+  k.hash = hash ? 1u : 0u;
+  k.nv = e->nv;
+  k.chunk = e->opts.split_chunk;
+  k.budget = e->budget_iters;
+  k.budgeted = e->opts.match_budget ? 1u : 0u;
+  k.capped = e->opts.max_results ? 1u : 0u;
+  k.gen = e->gen;
+  return k;
+}
+
+void graph_release(csm_engine::GraphEntry& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+}
+
+// Enqueue (on a capturing stream) one batch's head: begin, unpack, clear,
+// update, seeds, `rounds` search rounds between the search events.
+int enqueue_batch_head(csm_engine* e, uint32_t n, bool hash, uint32_t rounds) {
+  int rc;
+  unsigned long long* counts = e->d_counts;
+  unsigned long long* hashes = hash ? e->d_hashes : nullptr;
+  k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+  k_unpack<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_upd, n, e->d_type, e->d_add, e->d_v1,
+                                                    e->d_v2, e->d_lab, e->d_elmap,
+                                                    (uint32_t)e->el_map.size(), e->d_ctrl);
+  batch_clear(e, n, counts, hashes);
+  if ((rc = batch_update(e, n))) return rc;
+  seeds(e, n, counts, hashes, e->d_safe, true);
+  SearchArgs sa = search_args(e, counts, hashes, 0);
+  const bool symbi = uses_index(e) && e->d_d2 != nullptr;
+  if (hipEventRecord(e->ev_search[0], e->stream) != hipSuccess)
+    return fail(e, CSM_E_DEVICE, "event record in capture failed");
+  for (uint32_t r = 0; r < rounds; ++r) {
+    round_buffers(e, sa, r);
+    if (hash) { if (symbi) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa); }
+    else { if (symbi) launch_search<false, true>(e, sa); else launch_search<false, false>(e, sa); }
+    k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl, e->spill_chunk_cap);
+  }
+  if (hipEventRecord(e->ev_search[1], e->stream) != hipSuccess)
+    return fail(e, CSM_E_DEVICE, "event record in capture failed");
+  return CSM_OK;
+}
+
+// Capture the head of a batch into an executable graph; false (and the entry
+// marked bad) if the capture failed, allocated, or holds a memset node.
+bool graph_capture(csm_engine* e, csm_engine::GraphEntry& g, uint32_t n, bool hash) {
+  const uint64_t gen0 = e->gen;
+  const uint32_t rounds = std::min<uint32_t>(std::max<uint32_t>(e->max_rounds, 2u), 12u);
+  if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    g.bad = true;
+    return false;
+  }
+  const int rc = enqueue_batch_head(e, n, hash, rounds);
+  hipGraph_t graph = nullptr;
+  const hipError_t st = hipStreamEndCapture(e->stream, &graph);
+  bool ok = rc == CSM_OK && st == hipSuccess && graph && e->gen == gen0;
+  if (ok) {
+    size_t nn = 0;
+    ok = hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess;
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (ok && nn) ok = hipGraphGetNodes(graph, nodes.data(), &nn) == hipSuccess;
+    for (size_t i = 0; ok && i < nn; ++i) {
+      hipGraphNodeType t;
+      ok = hipGraphNodeGetType(nodes[i], &t) == hipSuccess && t != hipGraphNodeTypeMemset;
+    }
+  }
+  if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipGetLastError();
+  if (!ok) {
+    graph_release(g);
+    g.bad = true;
+    return false;
+  }
+  g.rounds = rounds;
+  return true;
+}
+
+// A device batch through a graph.  Returns 1 if this batch should run
+// eagerly instead (nothing has been enqueued then).
+int run_batch_graph(csm_engine* e, const csm_update* d_updates, uint32_t n, uint64_t* d_counts,
+                    uint8_t* d_safe, uint64_t* d_hash) {
+  int rc;
+  if ((rc = batch_prepare(e, n))) return rc;
+  const bool hash = d_hash != nullptr;
+  const csm_engine::GraphKey key = graph_key(e, n, hash);
+  csm_engine::GraphEntry* g = nullptr;
+  for (auto& x : e->graphs)
+    if (x.key == key) g = &x;
+  if (!g) {
+    // first batch of this key: eagerly (allocations settle); keep the key
+    if (e->graphs.size() >= MAX_GRAPHS) {
+      graph_release(e->graphs.front());
+      e->graphs.erase(e->graphs.begin());
+    }
+    csm_engine::GraphEntry x;
+    x.key = key;
+    e->graphs.push_back(x);
+    return 1;
+  }
+  if (g->bad) return 1;
+  const size_t nq = e->queries.size();
+  if (reinterpret_cast<const void*>(d_updates) != e->d_upd)
+    HIP_TRY(e, hipMemcpyAsync(e->d_upd, d_updates, (size_t)n * sizeof(UpdateRec),
+                              hipMemcpyDeviceToDevice, e->stream));
+  if (!g->exec && !graph_capture(e, *g, n, hash)) return 1;
+  HIP_TRY(e, hipGraphLaunch(g->exec, e->stream));
+  ++e->graph_launches;
+  e->stats[6] += g->rounds;
+  e->ev_pending = true;
+  if ((rc = read_ctrl(e))) return rc;
+  collect_search_time(e);
+  e->stats[6] -= 1;   // collect_search_time counted the event pair as one launch
+  e->max_rounds = std::max(e->max_rounds, e->h_ctrl->rounds + 1);
+  if (e->h_ctrl->nchunks > 0) {
+    // more rounds than the graph holds: continue eagerly from round g->rounds
+    if ((rc = search(e, true, e->d_counts, hash ? e->d_hashes : nullptr, 0, g->rounds))) return rc;
+  }
+  if ((rc = batch_tail(e, n, e->d_counts, true))) return rc;
+  if (reinterpret_cast<void*>(d_counts) != e->d_counts)
+    HIP_TRY(e, hipMemcpyAsync(d_counts, e->d_counts, (size_t)n * nq * 8, hipMemcpyDeviceToDevice,
+                              e->stream));
+  if (d_safe && d_safe != e->d_safe)
+    HIP_TRY(e, hipMemcpyAsync(d_safe, e->d_safe, n, hipMemcpyDeviceToDevice, e->stream));
+  if (hash && reinterpret_cast<void*>(d_hash) != e->d_hashes)
+    HIP_TRY(e, hipMemcpyAsync(d_hash, e->d_hashes, (size_t)n * nq * 8, hipMemcpyDeviceToDevice,
+                              e->stream));
   return CSM_OK;
 }
 
@@ -1674,6 +1906,9 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
     set(CSM_E_DEVICE);
     return nullptr;
   }
+  if (hipStreamCreateWithFlags(&e->xfer, hipStreamNonBlocking) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
+  for (auto& ev : e->marks)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
 
 
   // label maps from data and query labels
@@ -1922,12 +2157,16 @@ static int apply_batch_device_impl(csm_engine* e, const csm_update* d_updates, s
   e->h_uwork.assign(n, 0);
   if (n == 0) return CSM_OK;
   HIP_TRY(e, hipSetDevice(e->dev));
-  k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
-  k_unpack<<<grid_for(n, 256), 256, 0, e->stream>>>(
-      reinterpret_cast<const UpdateRec*>(d_updates), (uint32_t)n, e->d_type, e->d_add, e->d_v1,
-      e->d_v2, e->d_lab, e->d_elmap, (uint32_t)e->el_map.size(), e->d_ctrl);
-  int rc = run_batch(e, (uint32_t)n, reinterpret_cast<unsigned long long*>(d_counts),
-                     reinterpret_cast<unsigned long long*>(d_hash), d_safe);
+  int rc = 1;
+  if (graphs_allowed(e, (uint32_t)n)) rc = run_batch_graph(e, d_updates, (uint32_t)n, d_counts, d_safe, d_hash);
+  if (rc == 1) {
+    k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+    k_unpack<<<grid_for(n, 256), 256, 0, e->stream>>>(
+        reinterpret_cast<const UpdateRec*>(d_updates), (uint32_t)n, e->d_type, e->d_add, e->d_v1,
+        e->d_v2, e->d_lab, e->d_elmap, (uint32_t)e->el_map.size(), e->d_ctrl);
+    rc = run_batch(e, (uint32_t)n, reinterpret_cast<unsigned long long*>(d_counts),
+                   reinterpret_cast<unsigned long long*>(d_hash), d_safe);
+  }
   if (rc) return rc;
   if (e->opts.match_budget)
     HIP_TRY(e, hipMemcpyAsync(e->h_qabort.data(), e->d_qabort, e->queries.size() * 4,
@@ -2092,6 +2331,7 @@ int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
   if (n_out > 8) out[8] = e->opts.split_chunk;
   if (n_out > 9) out[9] = e->budget_iters;
   if (n_out > 10) out[10] = e->tune_active ? 1 : 0;
+  if (n_out > 11) out[11] = e->graph_launches;
   return CSM_OK;
 }
 
@@ -2167,7 +2407,9 @@ int csm_device_alloc(csm_engine* e, size_t bytes, void** ptr) {
   if (!e || !ptr) return CSM_E_ARG;
   HIP_TRY(e, hipSetDevice(e->dev));
   uint8_t* p = nullptr;
+  const uint64_t gen = e->gen;   // a caller's buffer does not move the engine's own
   int rc = dalloc(e, &p, bytes);
+  e->gen = gen;
   if (rc) return rc;
   *ptr = p;
   return CSM_OK;
@@ -2282,12 +2524,62 @@ int csm_copy_device(csm_engine* e, void* dst, const void* src, size_t bytes) {
 
 void* csm_stream(csm_engine* e) { return e ? (void*)e->stream : nullptr; }
 
+// The transfer-stream entry points touch only the transfer stream and the
+// marker events (never the engine's error text), so one host thread may call
+// them while another runs a batch on the engine.
+static hipStream_t stream_of(csm_engine* e, int which) {
+  return which == CSM_STREAM_ENGINE ? e->stream : e->xfer;
+}
+
+int csm_copy_async(csm_engine* e, void* dst, const void* src, size_t bytes) {
+  if (!e || (bytes && (!dst || !src))) return CSM_E_ARG;
+  if (hipSetDevice(e->dev) != hipSuccess ||
+      hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, e->xfer) != hipSuccess)
+    return CSM_E_DEVICE;
+  return CSM_OK;
+}
+
+int csm_mark(csm_engine* e, int stream, uint32_t slot) {
+  if (!e || slot >= (uint32_t)CSM_MARKS || (stream != CSM_STREAM_ENGINE && stream != CSM_STREAM_TRANSFER))
+    return CSM_E_ARG;
+  if (hipSetDevice(e->dev) != hipSuccess ||
+      hipEventRecord(e->marks[slot], stream_of(e, stream)) != hipSuccess)
+    return CSM_E_DEVICE;
+  return CSM_OK;
+}
+
+int csm_wait_mark(csm_engine* e, int stream, uint32_t slot) {
+  if (!e || slot >= (uint32_t)CSM_MARKS || (stream != CSM_STREAM_ENGINE && stream != CSM_STREAM_TRANSFER))
+    return CSM_E_ARG;
+  if (hipSetDevice(e->dev) != hipSuccess ||
+      hipStreamWaitEvent(stream_of(e, stream), e->marks[slot], 0) != hipSuccess)
+    return CSM_E_DEVICE;
+  return CSM_OK;
+}
+
+int csm_sync_mark(csm_engine* e, uint32_t slot) {
+  if (!e || slot >= (uint32_t)CSM_MARKS) return CSM_E_ARG;
+  if (hipEventSynchronize(e->marks[slot]) != hipSuccess) return CSM_E_DEVICE;
+  return CSM_OK;
+}
+
+int csm_mem_info(csm_engine* e, uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (!e) return CSM_E_ARG;
+  size_t f = 0, t = 0;
+  if (hipSetDevice(e->dev) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess) return CSM_E_DEVICE;
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return CSM_OK;
+}
+
 const char* csm_last_error(const csm_engine* e) { return e ? e->err.c_str() : "null engine"; }
 
 void csm_destroy(csm_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->dev);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (auto& g : e->graphs) graph_release(g);
+  e->graphs.clear();
   for (auto& a : e->allocs) (void)hipFree(a.p);
   e->allocs.clear();
   if (e->h_ctrl) (void)hipHostFree(e->h_ctrl);
@@ -2301,6 +2593,10 @@ void csm_destroy(csm_engine* e) {
   if (e->h_parse) (void)hipHostFree(e->h_parse);
   for (auto& ev : e->ev_search)
     if (ev) (void)hipEventDestroy(ev);
+  if (e->xfer) (void)hipStreamSynchronize(e->xfer);
+  for (auto& ev : e->marks)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->xfer) (void)hipStreamDestroy(e->xfer);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

@@ -75,7 +75,8 @@ struct Ctrl {
   uint32_t nspill;        // tasks spilled by the current search round
   uint32_t nspill_chunks; // their chunks
   uint32_t dir_top;       // key directory slots handed out
-  uint32_t pad[3];
+  uint32_t rounds;        // spilled rounds that had chunks, this batch
+  uint32_t pad[2];
   unsigned long long live;        // live entries (sum of list lengths)
   unsigned long long stat_tasks;
   unsigned long long stat_cands;
@@ -101,6 +102,19 @@ struct PlanRec {
   uint32_t bw[MAXK][MAXNB];      // (backward position) | (segment key << 8)
   uint32_t d2row[MAXK];          // row of position p's query vertex in the d2 bitmap
 };
+
+// Time limit, result cap and work counters of a search (SearchArgs::lim).
+struct SearchLimits {
+  // time limit: a host-written word in fine-grained host memory; once it
+  // reads non-zero, waves stop their tasks and drain the remaining chunks,
+  // flagging each affected update with USTAT_TIME_LIMIT and its query in qtl
+  const uint32_t* habort;
+  uint32_t* ustatus;          // per update of the batch: USTAT_* bits
+  uint32_t* qtl;              // per query: 1 once the time limit dropped some of its work
+  unsigned long long* uwork;  // per update: candidate entries tested
+  unsigned long long ucap;    // per (update, query) result cap
+};
+constexpr uint32_t LIM_TIME = 1u, LIM_CAP = 2u, LIM_WORK = 4u;
 
 // A search task: update t, plan qo, the first `depth` positions fixed in m[],
 // and the candidate range of position `depth` (absolute pool offsets into the
@@ -675,6 +689,13 @@ __global__ void k_dedup(BatchView b, GraphView g, uint32_t* __restrict__ ats) {
 }
 
 // ------------------------------------------------------------------ seeds
+// per-update status bits of a batch (csm_update_status)
+constexpr uint32_t USTAT_TIME_LIMIT = 1u;  // search stopped: counts are lower bounds
+constexpr uint32_t USTAT_CAPPED = 2u;      // a count reached the per-update result cap
+// passed the label filter but certified before its search to change no count:
+// for every seed, some query neighbour of a seed endpoint has no candidate
+constexpr uint32_t USTAT_CERTIFIED = 4u;
+
 struct SeedArgs {
   GraphView g;
   BatchView b;
@@ -698,6 +719,7 @@ struct SeedArgs {
   unsigned long long* counts;
   unsigned long long* hashes;
   uint8_t* safe;
+  uint32_t* ustatus;          // USTAT_CERTIFIED for updates no seed survives (nullable)
 };
 
 // Emit the depth-2 seeds of the lanes with `have` set (one task each, cut into
@@ -767,6 +789,7 @@ __global__ void k_seeds(SeedArgs a) {
     s = uniform(s);
     e = uniform(e);
     if (lane == 0 && a.safe) a.safe[i] = (s == e);
+    bool survived = false;      // some seed of this update passed every check
     for (uint32_t j0 = s; j0 < e; j0 += WAVE) {
       const uint32_t j = j0 + lane;
       bool have = false;
@@ -781,7 +804,20 @@ __global__ void k_seeds(SeedArgs a) {
         bool ok = !(P->symbi && a.d2 &&
                     !(d2_bit(a.d2, P->d2row[0], a.d2_words, m0) &&
                       d2_bit(a.d2, P->d2row[1], a.d2_words, m1)));
+        // neighbourhood certificate: every other query neighbour of the two
+        // seed endpoints needs a candidate among their neighbours with its
+        // label and edge label; an empty segment (in the pool, so at every
+        // position of the batch) rules the seed out
+        for (uint32_t p = 2; ok && p < P->k; ++p)
+          for (uint32_t jj = 0; ok && jj < P->nb[p]; ++jj) {
+            const uint32_t w = P->bw[p][jj];
+            if ((w & 0xFFu) > 1u) continue;
+            uint32_t ry;
+            const uint2 sg = seg_of(a.g, (w & 0xFFu) == 0 ? m0 : m1, w >> 8, &ry);
+            ok = sg.y > sg.x;
+          }
         if (ok && P->k == 2) {
+          survived = true;
           atomicAdd(&a.counts[(size_t)i * a.nq + P->q], 1ull);
           if (a.want_hash) {
             uint32_t mq[2];
@@ -809,9 +845,13 @@ __global__ void k_seeds(SeedArgs a) {
           }
         }
       }
+      survived = survived || have;
       emit_seed_wave(have, i, qo, m0, m1, bj, blo, bhi, ch, a.seeds, a.seed_cap, a.chunk_owner,
                      a.chunk_cap, a.ctrl);
     }
+    // passed the label filter, yet no seed can grow into an embedding
+    if (a.ustatus && s < e && ballot(survived) == 0 && lane == 0)
+      a.ustatus[i] |= USTAT_CERTIFIED;
   }
 }
 
@@ -839,18 +879,13 @@ struct SearchArgs {
   unsigned long long* qcount; // per-query running match count (nullable)
   uint32_t* qabort;           // per-query abort flags (nullable)
   unsigned long long budget;  // per-query match budget in this launch (0: none)
-  // time limit: a host-written word in fine-grained host memory (nullable:
-  // no limit); once it reads non-zero, waves stop their tasks and drain the
-  // remaining chunks, flagging each affected update with USTAT_TIME_LIMIT
-  const uint32_t* habort;
-  uint32_t* ustatus;          // per update of the batch: USTAT_* bits
-  unsigned long long ucap;    // per (update, query) result cap (0: none)
-  unsigned long long* uwork;  // per update: candidate entries tested (nullable)
-  uint32_t* qtl;              // per query: 1 once the time limit dropped some of its work
+  // LIM_* bits: which of the rarely used features below are on; their
+  // pointers and values sit in device memory (lim) so that the hot loop
+  // keeps one word of them in registers
+  uint32_t lflags;
+  const SearchLimits* __restrict__ lim;
 };
 
-constexpr uint32_t USTAT_TIME_LIMIT = 1u;  // search stopped: counts are lower bounds
-constexpr uint32_t USTAT_CAPPED = 2u;      // a count reached the per-update result cap
 // real-time counter ticks (100 MHz) between two polls of the host abort word
 constexpr uint64_t ABORT_POLL_TICKS = 2000;
 
@@ -881,6 +916,11 @@ struct WaveState {
   uint32_t pqo;
   uint32_t pnb[MK], pd2[MK], pqv[MK];
   uint32_t pbw[MK][MK - 1];
+  // rarely used per-wave state, kept out of registers: the time-limit word
+  // as last read and when, and the candidate count at the task's start
+  uint32_t aborted;
+  uint32_t pad_;
+  unsigned long long last_poll, cands0;
 };
 
 // One wavefront runs one depth-2 chunk of a seed at a time: an iterative DFS
@@ -941,8 +981,10 @@ constexpr bool XCD_REGIONS = CSM_XCD_REGIONS && CSM_XCD_SPLIT;
 #ifndef CSM_CHUNK_GRAB
 #define CSM_CHUNK_GRAB 1
 #endif
+// six waves per SIMD for queries of up to 8 vertices: the kernel fits in 80
+// VGPRs only when the compiler is held to it (left at five it takes 96)
 #ifndef CSM_SEARCH_WAVES_PER_EU
-#define CSM_SEARCH_WAVES_PER_EU 5
+#define CSM_SEARCH_WAVES_PER_EU 6
 #endif
 template <int MK, bool VERSIONED, bool HASH, bool SYMBI>
 __global__ __launch_bounds__(search_waves<MK>() * WAVE)
@@ -954,6 +996,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
   if (lane_id() == 0) {
     S.pqo = NONE;
     S.fvalid = 0;
+    S.aborted = 0;
+    S.last_poll = 0;
   }
   for (uint32_t i = lane; i < MK * (MK - 1); i += WAVE) S.pvx[i / (MK - 1)][i % (MK - 1)] = NONE;
   wave_sync();
@@ -982,17 +1026,18 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
   const uint32_t xn_lane = (XCD_REGIONS && a.xregions && lane < NXCD) ? a.ctrl->xn[lane] : 0u;
   const uint32_t nwaves = gridDim.x * (uint32_t)search_waves<MK>();
   // the host's time-limit word, polled at most every ABORT_POLL_TICKS
-  uint32_t aborted = 0;
-  uint64_t last_poll = 0;
   auto poll_abort = [&]() CSM_AINL -> bool {
-    if (!a.habort || aborted) return aborted != 0;
+    if (!(a.lflags & LIM_TIME)) return false;
+    if (S.aborted) return true;
     const uint64_t now = __builtin_amdgcn_s_memrealtime();
-    if (now - last_poll < ABORT_POLL_TICKS) return false;
-    last_poll = now;
+    if (now - S.last_poll < ABORT_POLL_TICKS) return false;
     uint32_t v = 0;
-    if (lane == 0) v = __hip_atomic_load(a.habort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    aborted = uniform(__shfl(v, 0));
-    return aborted != 0;
+    if (lane == 0) {
+      v = __hip_atomic_load(a.lim->habort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      S.last_poll = now;
+      S.aborted = v;
+    }
+    return uniform(__shfl(v, 0)) != 0;
   };
   while (true) {
     PROF_BEGIN(fetch);
@@ -1051,21 +1096,21 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     if (poll_abort()) {
       // time limit: this chunk is not searched
       if (lane == 0) {
-        atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
-        atomicOr(&a.qtl[q], 1u);
+        atomicOr(&a.lim->ustatus[t], USTAT_TIME_LIMIT);
+        atomicOr(&a.lim->qtl[q], 1u);
       }
       continue;
     }
-    if (a.ucap) {
+    if (a.lflags & LIM_CAP) {
       // the count of (t, q) has reached the cap: nothing left to find
-      unsigned long long cur = 0;
+      uint32_t full = 0;
       if (lane == 0)
-        cur = __hip_atomic_load(&a.counts[(size_t)t * a.nq + q], __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-      if (uniform(__shfl((uint32_t)(cur >= a.ucap), 0))) continue;
+        full = __hip_atomic_load(&a.counts[(size_t)t * a.nq + q], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) >= a.lim->ucap;
+      if (uniform(__shfl(full, 0))) continue;
     }
     ++n_tasks;
-    const unsigned long long cands_at_start = n_cands;
+    if ((a.lflags & LIM_WORK) && lane == 0) S.cands0 = n_cands;
     if (qo != S.pqo) {
       if (lane < MK) {
         S.pnb[lane] = P->nb[lane];
@@ -1098,25 +1143,29 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     // the update's count has reached the result cap
     auto flush = [&](bool final_) CSM_AINL {
       unsigned long long d = cnt - flushed;
-      uint32_t capped = 0;
-      if (lane == 0) {
-        unsigned long long tot_u = 0;
-        if (d) {
-          tot_u = atomicAdd(&a.counts[(size_t)t * a.nq + q], d) + d;
-          if (a.qcount) {
-            unsigned long long tot = atomicAdd(&a.qcount[q], d) + d;
-            if (a.budget && tot > a.budget) atomicOr(&a.qabort[q], 1u);
-          }
-        } else if (a.ucap && !final_) {
-          tot_u = __hip_atomic_load(&a.counts[(size_t)t * a.nq + q], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0 && d) {
+        if (!(a.lflags & LIM_CAP)) atomicAdd(&a.counts[(size_t)t * a.nq + q], d);
+        if (a.qcount) {
+          unsigned long long tot = atomicAdd(&a.qcount[q], d) + d;
+          if (a.budget && tot > a.budget) atomicOr(&a.qabort[q], 1u);
         }
-        capped = a.ucap && tot_u >= a.ucap;
+      }
+      if (a.lflags & LIM_CAP) {
+        // add the count and see the update's running total (the cap stops
+        // the task once it is reached)
+        uint32_t capped = 0;
+        if (lane == 0) {
+          unsigned long long* c = &a.counts[(size_t)t * a.nq + q];
+          const unsigned long long tot_u =
+              d ? atomicAdd(c, d) + d
+                : __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          capped = tot_u >= a.lim->ucap;
+        }
+        if (!final_ && uniform(__shfl(capped, 0))) stop = true;
       }
       flushed = cnt;
       if (!final_ && a.qabort)
         stop = uniform(__atomic_load_n(&a.qabort[q], __ATOMIC_RELAXED)) != 0;
-      if (!final_ && a.ucap && uniform(__shfl(capped, 0))) stop = true;
     };
 
     // test a candidate entry e of position lv's u_min segment (joins with the
@@ -1633,7 +1682,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       }
       if (++ticks == 256) {
         ticks = 0;
-        if (cnt - flushed > (1ull << 20) || a.qabort || a.ucap) flush(false);
+        if (cnt - flushed > (1ull << 20) || a.qabort || (a.lflags & LIM_CAP)) flush(false);
         if (poll_abort()) {
           stop = true;
           lost = true;
@@ -1693,16 +1742,19 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     flush(true);
     if (lane == 0 && HASH && hsum) atomicAdd(&a.hashes[(size_t)t * a.nq + q], hsum);
     if (lane == 0 && lost) {
-      atomicOr(&a.ustatus[t], USTAT_TIME_LIMIT);
-      atomicOr(&a.qtl[q], 1u);
+      atomicOr(&a.lim->ustatus[t], USTAT_TIME_LIMIT);
+      atomicOr(&a.lim->qtl[q], 1u);
     }
-    if (lane == 0 && a.uwork && n_cands > cands_at_start)
-      atomicAdd(&a.uwork[t], n_cands - cands_at_start);
+    if (lane == 0 && (a.lflags & LIM_WORK) && n_cands > S.cands0)
+      atomicAdd(&a.lim->uwork[t], n_cands - S.cands0);
   }
   if (lane == 0) {
-    atomicAdd(&a.ctrl->stat_tasks, n_tasks);
-    atomicAdd(&a.ctrl->stat_cands, n_cands);
-    atomicAdd(&a.ctrl->stat_bytes, n_bytes + 4ull * n_cands);
+    // a wave of an empty round adds nothing (and skips three contended atomics)
+    if (n_tasks) {
+      atomicAdd(&a.ctrl->stat_tasks, n_tasks);
+      atomicAdd(&a.ctrl->stat_cands, n_cands);
+      atomicAdd(&a.ctrl->stat_bytes, n_bytes + 4ull * n_cands);
+    }
 #ifdef CSM_PHASE_PROF
     for (int i = 0; i < 8; ++i) {
       atomicAdd(&a.ctrl->prof[i], prof_acc[i]);
@@ -1760,6 +1812,7 @@ __global__ void k_batch_begin(Ctrl* c) {
     c->ndirty_next = 0;
     c->err = 0;
     c->err_at = 0xFFFFFFFFu;
+    c->rounds = 0;
   }
 }
 
@@ -1803,6 +1856,7 @@ __global__ void k_next_round(Ctrl* c, uint32_t spill_chunk_cap) {
     } else {
       c->nchunks = c->nspill_chunks;
     }
+    if (c->nchunks) ++c->rounds;
     c->chunk_head = 0;
     for (uint32_t x = 0; x < NXCD; ++x) c->xhead[x * XHEAD_STRIDE] = 0;
     c->nspill = 0;

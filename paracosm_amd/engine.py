@@ -30,8 +30,12 @@ CSM_E_UNSUPPORTED = -6
 CSM_E_TIME_LIMIT = -7
 CSM_E_ADMISSION = -8
 
+STREAM_ENGINE, STREAM_TRANSFER = 0, 1   # csm_mark / csm_wait_mark streams
+MARKS = 16
+
 USTAT_TIME_LIMIT = 1   # the time limit stopped part of the update's search
 USTAT_CAPPED = 2       # a count of the update reached max_results
+USTAT_CERTIFIED = 4    # certified before its search to change no count
 
 ALGORITHMS = ("parallel_graphflow", "parallel_symbi", "parallel_turboflux",
               "graphflow", "symbi", "turboflux")
@@ -57,7 +61,8 @@ class _Opts(ctypes.Structure):
                 ("time_limit_s", ctypes.c_double), ("device_budget", ctypes.c_uint64),
                 ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("match_budget", ctypes.c_uint64), ("spill_budget", ctypes.c_uint32),
-                ("reference_orders", ctypes.c_uint32), ("want_work", ctypes.c_uint32)]
+                ("reference_orders", ctypes.c_uint32), ("want_work", ctypes.c_uint32),
+                ("no_graphs", ctypes.c_uint32)]
 
 
 UPDATE_DTYPE = np.dtype([("type", np.uint8), ("is_add", np.uint8), ("pad", np.uint16),
@@ -106,6 +111,11 @@ def lib():
             "csm_copy_to_host": (ctypes.c_int, [vp, vp, vp, sz]),
             "csm_copy_device": (ctypes.c_int, [vp, vp, vp, sz]),
             "csm_stream": (vp, [vp]),
+            "csm_copy_async": (ctypes.c_int, [vp, vp, vp, sz]),
+            "csm_mark": (ctypes.c_int, [vp, ctypes.c_int, u32]),
+            "csm_wait_mark": (ctypes.c_int, [vp, ctypes.c_int, u32]),
+            "csm_sync_mark": (ctypes.c_int, [vp, u32]),
+            "csm_mem_info": (ctypes.c_int, [vp, P(u64), P(u64)]),
             "csm_parse_text": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, vp, sz, vp, sz, P(sz),
                                               P(u64)]),
             "csm_parse_text_device": (ctypes.c_int, [vp, ctypes.c_int, vp, sz, vp, sz, P(sz),
@@ -131,7 +141,8 @@ def exported_symbols() -> Sequence[str]:
             "csm_set_time_limit", "csm_update_work", "csm_totals", "csm_memory",
             "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
             "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
-            "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_parse_text",
+            "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_copy_async", "csm_mark",
+            "csm_wait_mark", "csm_sync_mark", "csm_mem_info", "csm_parse_text",
             "csm_parse_text_device", "csm_parse_text_device_static", "csm_last_error",
             "csm_destroy")
 
@@ -172,13 +183,15 @@ class Matching:
                  max_batch: int = 0, want_hash: bool = False, device_budget: int = 0,
                  split_chunk: int = 0, device: int = 0, match_budget: int = 0,
                  spill_budget: int = 0, reference_orders: bool = False,
-                 time_limit_s: float = 0.0, want_work: bool = False):
+                 time_limit_s: float = 0.0, want_work: bool = False,
+                 no_graphs: bool = False):
         L = lib()
         self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         d = _GraphArgs(vlabels, src, dst, elabels)
         o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
                   auto_tuning, max_batch, int(want_hash), time_limit_s, device_budget, split_chunk,
-                  device, match_budget, spill_budget, int(reference_orders), int(want_work))
+                  device, match_budget, spill_budget, int(reference_orders), int(want_work),
+                  int(no_graphs))
         err = ctypes.c_int(0)
         self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
                               ctypes.byref(o), ctypes.byref(err))
@@ -295,12 +308,12 @@ class Matching:
         return int(a.value), int(b.value), int(c.value)
 
     def stats(self):
-        out = (ctypes.c_uint64 * 11)()
-        self._check(lib().csm_stats(self.h, out, 11))
+        out = (ctypes.c_uint64 * 12)()
+        self._check(lib().csm_stats(self.h, out, 12))
         return {"tasks": out[0], "candidates": out[1], "alg_bytes": out[2], "batches": out[3],
                 "repacks": out[4], "search_ns": out[5], "search_launches": out[6],
                 "max_rounds": out[7], "split_chunk": out[8], "spill_budget": out[9],
-                "tuning": out[10]}
+                "tuning": out[10], "graph_batches": out[11]}
 
     def device_bytes(self) -> int:
         b = ctypes.c_uint64(0)
@@ -322,6 +335,33 @@ class Matching:
 
     def stream(self) -> int:
         return int(lib().csm_stream(self.h) or 0)
+
+    # -- overlapping copies (transfer stream + markers; see csm_copy_async)
+    def copy_async(self, dst: int, src: int, nbytes: int) -> None:
+        rc = lib().csm_copy_async(self.h, dst, src, nbytes)
+        if rc:
+            raise CsmError(rc, "csm_copy_async failed")
+
+    def mark(self, stream: int, slot: int) -> None:
+        rc = lib().csm_mark(self.h, stream, slot)
+        if rc:
+            raise CsmError(rc, "csm_mark failed")
+
+    def wait_mark(self, stream: int, slot: int) -> None:
+        rc = lib().csm_wait_mark(self.h, stream, slot)
+        if rc:
+            raise CsmError(rc, "csm_wait_mark failed")
+
+    def sync_mark(self, slot: int) -> None:
+        rc = lib().csm_sync_mark(self.h, slot)
+        if rc:
+            raise CsmError(rc, "csm_sync_mark failed")
+
+    def mem_info(self):
+        """(free, total) device bytes on the engine's device."""
+        f, t = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._check(lib().csm_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return int(f.value), int(t.value)
 
     def _check_partial(self, rc: int, partial_ok: bool) -> bool:
         """True if the call hit the time limit (allowed by partial_ok)."""

@@ -2,29 +2,40 @@
 
 Update requests (each a small batch of edge updates) arrive over time.  The
 server keeps the graph, the indexes and the standing queries on the device
-and runs a loop:
+and overlaps the stages of consecutive batches:
 
-* every request that has arrived and fits is admitted into the next device
-  batch, in arrival order: up to ``max_batch`` updates, and only while the
-  device bytes the engine holds plus the batch's working set stay inside
-  ``hbm_budget`` (requests that do not fit wait for the next batch);
-* the batch is uploaded, searched and read back; each of its requests
-  completes when the batch does.
+* the admission thread (the caller's) gathers every request that has
+  arrived, in arrival order, into the next batch: up to ``max_batch``
+  updates, and only while the engine's device bytes plus the working sets of
+  the batches in flight and of this one stay inside ``hbm_budget``;
+  requests that do not fit wait for a later batch;
+* it stages the batch in a pinned slot and starts its upload on the
+  engine's transfer stream, while the device is still searching the
+  previous batch;
+* the device thread makes the engine stream wait for that upload, runs the
+  batch (graph update, index update, search), and starts the download of
+  its counts on the transfer stream, where it overlaps the next batch's
+  search;
+* back on the admission thread, each finished batch's download is waited
+  for and its requests completed.
 
-Because admitted requests are concatenated in arrival order, every update
-is answered exactly as if the stream were applied one update at a time --
-the same result alone and in the mix (tests/test_gpu_serve.py).  Latency of
-an update = completion time of its batch - arrival time of its request.
+Batches run on the engine one after another in arrival order, and every
+update sees exactly the graph before it, so every update is answered as if
+the stream were applied one update at a time -- the same result alone and in
+the mix (tests/test_gpu_serve.py).  Latency of an update = completion time of
+its batch - arrival time of its request.
 """
 from __future__ import annotations
 
 import dataclasses
+import queue
+import threading
 import time
 from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .engine import UPDATE_DTYPE, Matching
+from .engine import MARKS, STREAM_ENGINE, STREAM_TRANSFER, UPDATE_DTYPE, Matching
 
 REC_BYTES = UPDATE_DTYPE.itemsize
 
@@ -36,6 +47,7 @@ class Request:
     done: float = float("nan")     # completion time (s from start)
     counts: Optional[np.ndarray] = None   # [n, num_queries] exact counts
     safe: Optional[np.ndarray] = None
+    status: Optional[np.ndarray] = None   # [n] USTAT_* bits (time limit, result cap)
 
 
 @dataclasses.dataclass
@@ -51,6 +63,10 @@ class ServeStats:
     max_ms: float
     mean_batch: float
     refused_for_memory: int
+    overlapped_uploads: int = 0    # uploads started while another batch was on the device
+    partial_updates: int = 0       # updates whose search the engine's time limit stopped
+    # per query: batches in which the time limit dropped part of its search
+    query_partial_batches: Optional[List[int]] = None
 
 
 def poisson_requests(updates: np.ndarray, offered_per_s: float, seed: int = 0,
@@ -73,88 +89,208 @@ def poisson_requests(updates: np.ndarray, offered_per_s: float, seed: int = 0,
     return out
 
 
-class Server:
-    def __init__(self, engine: Matching, *, max_batch: int = 65534, hbm_budget: int = 0):
-        self.m = engine
-        self.nq = engine.num_queries
-        self.max_batch = max_batch
-        self.hbm_budget = hbm_budget
-        # device and pinned buffers through the engine's own HIP runtime
-        self.d_upd = engine.device_buffer(max_batch * REC_BYTES)
-        self.d_cnt = engine.device_buffer(max_batch * self.nq * 8)
-        self.d_safe = engine.device_buffer(max_batch)
-        self.h_upd = engine.host_buffer(max_batch * REC_BYTES)
-        self.h_cnt = engine.host_buffer(max_batch * self.nq * 8)
-        self.h_safe = engine.host_buffer(max_batch)
-        self.refused = 0
+class _Slot:
+    """Pinned staging and device buffers of one batch in flight, and its
+    three markers (uploaded, searched, downloaded)."""
+
+    def __init__(self, m: Matching, idx: int, max_batch: int, nq: int):
+        self.idx = idx
+        self.d_upd = m.device_buffer(max_batch * REC_BYTES)
+        self.d_cnt = m.device_buffer(max_batch * nq * 8)
+        self.d_safe = m.device_buffer(max_batch)
+        self.h_upd = m.host_buffer(max_batch * REC_BYTES)
+        self.h_cnt = m.host_buffer(max_batch * nq * 8)
+        self.h_safe = m.host_buffer(max_batch)
+        self.mk_up, self.mk_done, self.mk_dl = 3 * idx, 3 * idx + 1, 3 * idx + 2
+        self.reqs: Sequence[Request] = ()
+        self.n = 0
+        self.error: Optional[BaseException] = None
+        self.ustatus: Optional[np.ndarray] = None
+        self.qstatus: Optional[np.ndarray] = None
 
     def close(self) -> None:
         for b in (self.h_upd, self.h_cnt, self.h_safe, self.d_upd, self.d_cnt, self.d_safe):
             b.close()
 
+
+class Server:
+    """hbm_budget: device bytes the engine plus the batches in flight may
+    hold; 0 = 90 % of the device's memory."""
+
+    def __init__(self, engine: Matching, *, max_batch: int = 65534, hbm_budget: int = 0,
+                 slots: int = 2):
+        assert 1 <= slots and 3 * slots <= MARKS
+        self.m = engine
+        self.nq = engine.num_queries
+        self.max_batch = max_batch
+        self.slots = [_Slot(engine, i, max_batch, self.nq) for i in range(slots)]
+        if not hbm_budget:
+            hbm_budget = int(0.9 * engine.mem_info()[1])
+        self.hbm_budget = hbm_budget
+        self.refused = 0
+
+    def close(self) -> None:
+        for s in self.slots:
+            s.close()
+        self.slots = []
+
     def working_set(self, n: int) -> int:
         """Device bytes a batch of n updates needs beyond the engine's own."""
         return n * (REC_BYTES + 8 * self.nq + 1)
 
-    def fits(self, n: int) -> bool:
-        if not self.hbm_budget:
-            return True
-        return self.m.device_bytes() + self.working_set(n) <= self.hbm_budget
+    def fits(self, n: int, in_flight: int = 0) -> bool:
+        return (self.m.device_bytes() + self.working_set(in_flight) + self.working_set(n)
+                <= self.hbm_budget)
 
-    def run_batch(self, reqs: Sequence[Request], t0: float) -> None:
+    # -- the three stages
+    def _stage(self, slot: _Slot, reqs: Sequence[Request]) -> None:
         ups = np.concatenate([r.updates for r in reqs]) if len(reqs) > 1 else reqs[0].updates
         n = len(ups)
         nb = n * REC_BYTES
-        self.h_upd.array[:nb] = np.ascontiguousarray(ups, UPDATE_DTYPE).view(np.uint8)
-        self.d_upd.upload(self.h_upd.array[:nb])
-        self.m.apply_batch_device(self.d_upd.ptr, n, self.d_cnt.ptr, self.d_safe.ptr)
-        self.d_cnt.download(self.h_cnt.array[: n * self.nq * 8])
-        self.d_safe.download(self.h_safe.array[:n])
-        done = time.perf_counter() - t0
-        cnt = self.h_cnt.array[: n * self.nq * 8].view(np.uint64).reshape(n, self.nq).copy()
-        safe = self.h_safe.array[:n].copy()
+        slot.h_upd.array[:nb] = np.ascontiguousarray(ups, UPDATE_DTYPE).view(np.uint8)
+        slot.reqs, slot.n, slot.error = reqs, n, None
+        self.m.copy_async(slot.d_upd.ptr, slot.h_upd.ptr, nb)
+        self.m.mark(STREAM_TRANSFER, slot.mk_up)
+
+    def _device_loop(self, todo: "queue.Queue", done: "queue.Queue") -> None:
+        m = self.m
+        while True:
+            slot = todo.get()
+            if slot is None:
+                done.put(None)
+                return
+            try:
+                m.wait_mark(STREAM_ENGINE, slot.mk_up)
+                # a time limit (csm_opts.time_limit_s) is a normal outcome
+                # here: the batch is applied and its stopped updates flagged
+                m.apply_batch_device(slot.d_upd.ptr, slot.n, slot.d_cnt.ptr, slot.d_safe.ptr,
+                                     partial_ok=True)
+                slot.ustatus = m.update_status(slot.n)
+                slot.qstatus = m.query_status()
+                # the batch call has synchronised its results: download them
+                # on the transfer stream while the next batch searches
+                m.mark(STREAM_ENGINE, slot.mk_done)
+                m.wait_mark(STREAM_TRANSFER, slot.mk_done)
+                m.copy_async(slot.h_cnt.ptr, slot.d_cnt.ptr, slot.n * self.nq * 8)
+                m.copy_async(slot.h_safe.ptr, slot.d_safe.ptr, slot.n)
+                m.mark(STREAM_TRANSFER, slot.mk_dl)
+            except BaseException as exc:   # handed to the caller's thread
+                slot.error = exc
+            done.put(slot)
+
+    def _complete(self, slot: _Slot, t0: float) -> None:
+        self.m.sync_mark(slot.mk_dl)
+        t = time.perf_counter() - t0
+        n = slot.n
+        cnt = slot.h_cnt.array[: n * self.nq * 8].view(np.uint64).reshape(n, self.nq).copy()
+        safe = slot.h_safe.array[:n].copy()
         o = 0
-        for r in reqs:
+        for r in slot.reqs:
             k = len(r.updates)
             r.counts = cnt[o:o + k]
             r.safe = safe[o:o + k]
-            r.done = done
+            r.status = slot.ustatus[o:o + k]
+            r.done = t
             o += k
+        self._qpartial += (slot.qstatus & 2) != 0
 
     def serve(self, requests: List[Request]) -> ServeStats:
         """Serve requests in arrival order in real time (requests must be
         sorted by arrival)."""
+        todo: "queue.Queue" = queue.Queue()
+        done: "queue.Queue" = queue.Queue()
+        free = list(self.slots)
+        self._qpartial = np.zeros(self.nq, np.int64)
+        in_flight: dict = {}       # slot idx -> updates in flight
+        dev = threading.Thread(target=self._device_loop, args=(todo, done), daemon=True)
         t0 = time.perf_counter()
+        dev.start()
         i = 0
         batches = 0
+        overlapped = 0
         n_req = len(requests)
-        while i < n_req:
-            now = time.perf_counter() - t0
-            if requests[i].arrival > now:
-                time.sleep(min(requests[i].arrival - now, 0.05))
-                continue
-            batch: List[Request] = []
-            size = 0
-            while i < n_req and requests[i].arrival <= now:
-                k = len(requests[i].updates)
-                if batch and size + k > self.max_batch:
+        error: Optional[BaseException] = None
+
+        def reap(block: bool, timeout: float = 0.05) -> None:
+            nonlocal error
+            try:
+                slot = done.get(block=block, timeout=timeout if block else None)
+            except queue.Empty:
+                return
+            if slot is None:
+                return
+            if slot.error is not None:
+                error = error or slot.error
+            else:
+                self._complete(slot, t0)
+            in_flight.pop(slot.idx, None)
+            free.append(slot)
+
+        try:
+            while i < n_req and error is None:
+                reap(False)
+                now = time.perf_counter() - t0
+                if requests[i].arrival > now:
+                    wait = min(requests[i].arrival - now, 0.05)
+                    if in_flight:
+                        reap(True, wait)
+                    else:
+                        time.sleep(wait)
+                    continue
+                if not free:
+                    reap(True)
+                    continue
+                batch: List[Request] = []
+                size = 0
+                busy = sum(in_flight.values())
+                while i < n_req and requests[i].arrival <= now:
+                    k = len(requests[i].updates)
+                    if batch and size + k > self.max_batch:
+                        break
+                    if not self.fits(size + k, busy):
+                        self.refused += 1
+                        if not batch and not in_flight:
+                            raise MemoryError("a single request exceeds the HBM budget")
+                        break
+                    batch.append(requests[i])
+                    size += k
+                    i += 1
+                if not batch:
+                    reap(True)       # wait for a batch in flight to free memory
+                    continue
+                slot = free.pop()
+                overlapped += bool(in_flight)
+                self._stage(slot, batch)
+                in_flight[slot.idx] = size
+                todo.put(slot)
+                batches += 1
+            while in_flight and error is None:
+                reap(True)
+        finally:
+            todo.put(None)
+            dev.join()
+            while True:            # complete what the device thread finished
+                try:
+                    slot = done.get_nowait()
+                except queue.Empty:
                     break
-                if not self.fits(size + k):
-                    self.refused += 1
-                    if not batch:
-                        raise MemoryError("a single request exceeds the HBM budget")
-                    break
-                batch.append(requests[i])
-                size += k
-                i += 1
-            self.run_batch(batch, t0)
-            batches += 1
+                if slot is not None and slot.idx in in_flight:
+                    if slot.error is None:
+                        self._complete(slot, t0)
+                    else:
+                        error = error or slot.error
+                    in_flight.pop(slot.idx, None)
+                    free.append(slot)
+        if error is not None:
+            raise error
         elapsed = time.perf_counter() - t0
         lat = []
         matches = 0
+        partial = 0
         for r in requests:
             lat.extend([r.done - r.arrival] * len(r.updates))
             matches += int(r.counts.sum())
+            partial += int(((r.status & 1) != 0).sum())
         lat_ms = 1e3 * np.asarray(lat)
         n_upd = sum(len(r.updates) for r in requests)
         return ServeStats(requests=n_req, updates=n_upd, batches=batches, elapsed=elapsed,
@@ -162,4 +298,6 @@ class Server:
                           p50_ms=float(np.percentile(lat_ms, 50)),
                           p99_ms=float(np.percentile(lat_ms, 99)),
                           max_ms=float(lat_ms.max()), mean_batch=n_upd / max(batches, 1),
-                          refused_for_memory=self.refused)
+                          refused_for_memory=self.refused, overlapped_uploads=overlapped,
+                          partial_updates=partial,
+                          query_partial_batches=self._qpartial.tolist())

@@ -59,8 +59,10 @@ def _run_device(algo, d, qs, n, batch, **kw):
     dt = time.time() - t0
     cnt = d_cnt.download(np.zeros(n * nq, np.uint64)).reshape(n, nq)
     safe = d_safe.download(np.zeros(n, np.uint8))
+    graphs = m.stats()["graph_batches"]
     m.close()
-    _log(f"{algo}: {n} updates in batches of {batch}: {dt:.2f}s, {int(cnt.sum())} matches")
+    _log(f"{algo}: {n} updates in batches of {batch}: {dt:.2f}s, {int(cnt.sum())} matches, "
+         f"{graphs} batches replayed as graphs")
     return cnt, safe
 
 
@@ -174,10 +176,15 @@ def test_c2_whole_stream_plugins_agree(c2, algo):
 
 @pytest.mark.timeout(600)
 def test_c2_batch_size_invariance(c2):
+    """Batches of 7,919 count what batches of 40,000 count; and batches
+    issued launch by launch (no HIP graphs) count what the graph replays of
+    the whole-stream run counted."""
     d, qs, cnt, _ = c2
     n = 200_000
     got, _ = _run_device("parallel_graphflow", d, qs, n, 7919)
     np.testing.assert_array_equal(got, cnt[:n])
+    eager, _ = _run_device("parallel_graphflow", d, qs, n, 40000, no_graphs=True)
+    np.testing.assert_array_equal(eager, cnt[:n])
 
 
 # ------------------------------------------------------------------ config 5
@@ -196,3 +203,61 @@ def test_c5_matches_oracle_and_plugins_agree(oracle_lib):
     gf, _ = _run_device("parallel_graphflow", d, qs, n, 62500)
     np.testing.assert_array_equal(gf, cnt)
     _log(f"c5: safe fraction {safe.mean():.4f}, matches {int(cnt.sum())}")
+    # the neighbourhood certificate on the same prefix: sound against the
+    # oracle's counts, and certifying updates past the label filter
+    from paracosm_amd.engine import USTAT_CERTIFIED, Matching, make_updates
+    w = d.w
+    m = Matching("parallel_graphflow", qs[0], w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    for q in qs[1:]:
+        m.register_query(q)
+    m.Preprocessing()
+    m.apply_batch(make_updates(w.upd_add[:300], w.upd_v1[:300], w.upd_v2[:300], w.upd_label[:300]))
+    cert = (m.update_status(300) & USTAT_CERTIFIED) != 0
+    m.close()
+    assert (ref[cert].sum(axis=1) == 0).all()
+    _log(f"c5: {int(cert.sum())} of 300 updates certified past the label filter")
+
+
+# ------------------------------------------------------------------ config 3
+@pytest.mark.timeout(900)
+def test_c3_large_queries_capped_prefix(oracle_lib):
+    """Config 3's large queries: random walks of 12, 16 and 24 vertices on the
+    scale-20 graph (no admission rule), insertion stream.  Their exact counts
+    run to 10^9-10^10 per update, so the check uses the per-update result cap
+    (--max-results 1,000, count = min(exact, cap)) on the first 300
+    insertions: every update that both the engine and the oracle finish within
+    their time limits (engine 60 s per call, oracle 1 s per update and 15 s
+    per query) is bit-exact."""
+    from paracosm_amd import workloads
+    from paracosm_amd.engine import USTAT_TIME_LIMIT, Matching, make_updates
+    O = oracle_lib
+    d = workloads.build("c3", seed=1)
+    w = d.w
+    n, cap = 300, 1000
+    ups = make_updates(w.upd_add[:n], w.upd_v1[:n], w.upd_v2[:n], w.upd_label[:n])
+    dg = O.OGraph(w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    compared = 0
+    for k in (12, 16, 24):
+        qs = workloads.candidates(d, k, None, 3, 0)
+        m = Matching("parallel_graphflow", qs[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                     max_results=cap, time_limit_s=60.0)
+        for q in qs[1:]:
+            m.register_query(q)
+        m.Preprocessing()
+        t0 = time.time()
+        pos, _, _, _ = m.apply_batch(ups, partial_ok=True)
+        gpu_done = (m.update_status(n) & USTAT_TIME_LIMIT) == 0
+        m.close()
+        _log(f"c3 k={k}: engine {time.time() - t0:.2f}s, {int(gpu_done.sum())}/{n} finished, "
+             f"capped {int((pos == cap).sum())}")
+        for j, q in enumerate(qs):
+            t0 = time.time()
+            cnt, unf = O.graphflow_limited(dg, O.OGraph.from_query(q), w.upd_add, w.upd_v1,
+                                           w.upd_v2, w.upd_label, limit=n, max_results=cap,
+                                           update_seconds=1.0, total_seconds=15.0)
+            both = gpu_done & ~unf
+            _log(f"c3 k={k} query {j}: oracle {time.time() - t0:.1f}s, {int(both.sum())} "
+                 f"updates compared, {int((cnt[both] == cap).sum())} at the cap")
+            np.testing.assert_array_equal(pos[both, j], cnt[both])
+            compared += int(both.sum())
+    assert compared >= n

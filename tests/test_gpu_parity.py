@@ -38,13 +38,26 @@ def _case(seed, nv=40, p=0.25, n_vl=3, n_el=2, n_upd=120, k=None, extra=None):
 def test_batch_matches_oracle(oracle_lib, algo, seed):
     vl, init, ups, q = _case(seed)
     ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
-    from paracosm_amd.engine import make_updates
+    from paracosm_amd.engine import USTAT_CERTIFIED, make_updates
     with _engine(algo, q, vl, init, want_hash=True) as m:
         pos, neg, safe, hsh = m.apply_batch(make_updates(*ups))
-    print("case", algo, seed, "k", q.k, "total", int(ref.pos.sum() + ref.neg.sum()), flush=True)
+        st = m.update_status(len(ups[0]))
+    print("case", algo, seed, "k", q.k, "total", int(ref.pos.sum() + ref.neg.sum()),
+          "certified", int(((st & USTAT_CERTIFIED) != 0).sum()), flush=True)
     np.testing.assert_array_equal(pos[:, 0], ref.pos)
     np.testing.assert_array_equal(neg[:, 0], ref.neg)
     np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+    # Classify (the label filter) equals the oracle's per update; the
+    # neighbourhood certificate is sound (certified updates change nothing)
+    # and applies only past the label filter
+    add, v1, v2, lab = ups
+    qg = oracle_lib.OGraph.from_query(q)
+    want_safe = [oracle_lib.classify(qg, int(vl[a]), int(vl[b]), int(c))
+                 for a, b, c in zip(v1, v2, lab)]
+    np.testing.assert_array_equal(safe.astype(bool), np.asarray(want_safe))
+    cert = (st & USTAT_CERTIFIED) != 0
+    assert (ref.pos[cert] + ref.neg[cert] == 0).all()
+    assert not (cert & safe.astype(bool)).any()
 
 
 @pytest.mark.parametrize("batch", [1, 5, 33])
@@ -525,3 +538,94 @@ def test_reference_orders_agree(oracle_lib, algo):
             np.testing.assert_array_equal(pos[:, 0], ref.pos)
             np.testing.assert_array_equal(neg[:, 0], ref.neg)
             np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+
+
+@pytest.mark.parametrize("algo,budget", [("parallel_graphflow", 0), ("parallel_symbi", 0),
+                                         ("parallel_graphflow", 2), ("parallel_turboflux", 3)])
+def test_graph_replays_match_eager(oracle_lib, algo, budget):
+    """Steady-state batches replayed as HIP graphs (from the second batch of a
+    size and buffer parity on) count exactly what the oracle and an engine
+    without graphs count, on every replay, with eager batches of another size
+    and host reads between the replays.  A tiny spill budget makes batches
+    need more search rounds than the graph holds (eager continuation)."""
+    from paracosm_amd.engine import make_updates
+    vl, init, ups, q = _case(77, nv=70, p=0.3, n_upd=640, k=5, extra=2)
+    ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+    u = make_updates(*ups)
+    kw = dict(want_hash=True, split_chunk=8, spill_budget=budget) if budget else dict(want_hash=True)
+    with _engine(algo, q, vl, init, **kw) as m, \
+            _engine(algo, q, vl, init, no_graphs=True, **kw) as e:
+        s = 0
+        sizes = [40] * 6 + [17] + [40] * 6 + [33] + [40] * 4
+        for b in sizes:
+            p1, n1, _, h1 = m.apply_batch(u[s:s + b])
+            p2, n2, _, h2 = e.apply_batch(u[s:s + b])
+            np.testing.assert_array_equal(p1[:, 0], ref.pos[s:s + b])
+            np.testing.assert_array_equal(n1[:, 0], ref.neg[s:s + b])
+            np.testing.assert_array_equal(h1[:, 0], ref.hash[s:s + b])
+            np.testing.assert_array_equal(p1, p2)
+            np.testing.assert_array_equal(h1, h2)
+            s += b
+        g = m.stats()["graph_batches"]
+        print("graph replays", algo, budget, g, "rounds", m.stats()["max_rounds"], flush=True)
+        assert g >= 10 and e.stats()["graph_batches"] == 0
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi", "parallel_turboflux"])
+def test_add_vertex_then_edges_matches_oracle(oracle_lib, algo):
+    """Vertex insertions inside a stream ('v' records, Graph::AddVertex,
+    graph.cpp:24-44, through the plugins' AddVertex, parrallel_symbi.cpp:
+    3812-3872) followed by edges to the new vertices: counts equal the oracle
+    run on a graph that held the new vertices, isolated, from the start; the
+    SymBi/TurboFlux index gets rows for them (its fix-point is checked
+    against a from-scratch build afterwards)."""
+    O = oracle_lib
+    from paracosm_amd.engine import make_updates
+    rng = np.random.default_rng(4242)
+    nv0, extra_v = 40, 12
+    vl_all, s, d, el = random_graph(nv0 + extra_v, 0.3, 2, 2, rng)
+    keep = (s < nv0) & (d < nv0)
+    init = (s[keep], d[keep], el[keep])
+    new_edges = [(int(a), int(b), int(c)) for a, b, c in zip(s[~keep], d[~keep], el[~keep])]
+    rng.shuffle(new_edges)
+    q = random_connected_query(4, 1, 2, 2, rng)
+    # stream: each new vertex announced before its first edge
+    recs, announced = [], set()
+    for a, b, c in new_edges:
+        for v in (a, b):
+            if v >= nv0 and v not in announced:
+                announced.add(v)
+                recs.append((True, 1, v, 0, int(vl_all[v])))
+        recs.append((False, 1, a, b, c))
+    for v in range(nv0, nv0 + extra_v):   # vertices without edges, announced last
+        if v not in announced:
+            recs.append((True, 1, v, 0, int(vl_all[v])))
+    vert = np.asarray([r[0] for r in recs])
+    add = np.asarray([r[1] for r in recs], np.uint8)
+    v1 = np.asarray([r[2] for r in recs], np.uint32)
+    v2 = np.asarray([r[3] for r in recs], np.uint32)
+    lab = np.asarray([r[4] for r in recs], np.uint32)
+    e_idx = np.flatnonzero(~vert)
+    ref = _oracle(O, q, vl_all, init, (add[e_idx], v1[e_idx], v2[e_idx], lab[e_idx]))
+    with _engine(algo, q, vl_all[:nv0], init) as m:
+        pos, neg, _, _ = m.apply_batch(make_updates(add, v1, v2, lab, vertex=vert))
+        print("add_vertex", algo, "new vertices", len(announced), "edges", len(e_idx), "total",
+              int(ref.pos.sum()), flush=True)
+        np.testing.assert_array_equal(pos[e_idx, 0], ref.pos)
+        assert (pos[vert, 0] == 0).all() and (neg[:, 0] == 0).all()
+        assert ref.pos.sum() > 0
+        if algo != "parallel_graphflow":
+            nv = nv0 + extra_v
+            final = (np.concatenate([init[0], v1[e_idx]]), np.concatenate([init[1], v2[e_idx]]),
+                     np.concatenate([init[2], lab[e_idx]]))
+            fg = O.OGraph(vl_all, *final)
+            # the index's DAG/tree comes from the initial graph, whose label
+            # frequencies do not count the vertices added later
+            vl0 = vl_all.copy()
+            vl0[nv0:] = 0xFFFFFFFF
+            ig = O.OGraph(vl0, *init)
+            for what in (1, 2):
+                got = m.index_dump(0, what, nv, q.k)
+                want = O.index_dump(fg, O.OGraph.from_query(q), what, nv, q.k,
+                                    tree=algo == "parallel_turboflux", dag_src=ig)
+                np.testing.assert_array_equal(got[:, :nv], want)

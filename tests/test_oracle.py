@@ -293,3 +293,45 @@ def test_turboflux_index_is_the_fixpoint(oracle_lib, seed):
     init, ups = mixed_stream_small(s, d, el, 30, 120, 0.6, 2, rng)
     q = random_connected_query(4 + seed % 3, 1 + seed % 2, 2, 2, rng)
     O.run("turboflux_index", O.OGraph(vl, *init), O.OGraph.from_query(q), *ups, check_index=True)
+
+
+# ------------------------------------------------ result cap and time limits
+@pytest.mark.parametrize("cap", [1, 3, 50])
+def test_oracle_result_cap_is_min_of_exact_and_cap(oracle_lib, cap):
+    """The capped search reports min(exact, cap) per update, the exact count
+    being the uncapped search's (itself pinned by brute force above)."""
+    O = oracle_lib
+    rng = np.random.default_rng(70 + cap)
+    vl, s, d, el = random_graph(40, 0.4, 2, 1, rng)
+    init, ups = mixed_stream_small(s, d, el, 40, 120, 0.7, 1, rng)
+    q = random_connected_query(5, 2, 2, 1, rng)
+    full = O.run("graphflow", O.OGraph(vl, *init), O.OGraph.from_query(q), *ups)
+    capped = O.run("graphflow", O.OGraph(vl, *init), O.OGraph.from_query(q), *ups,
+                   max_results=cap)
+    ex = full.pos + full.neg
+    assert (ex > cap).any()
+    np.testing.assert_array_equal(capped.pos + capped.neg, np.minimum(ex, cap))
+    cnt, unf = O.graphflow_limited(O.OGraph(vl, *init), O.OGraph.from_query(q), *ups,
+                                   limit=len(ex), max_results=cap, update_seconds=60.0)
+    assert not unf.any()
+    np.testing.assert_array_equal(cnt, np.minimum(ex, cap))
+
+
+def test_oracle_time_limit_flags_unfinished_updates(oracle_lib):
+    """Past the run's time budget the remaining updates are applied without a
+    search and flagged; flagged counts are lower bounds, unflagged ones
+    exact; the graph stays exact (a later unlimited run agrees)."""
+    O = oracle_lib
+    rng = np.random.default_rng(81)
+    vl, s, d, el = random_graph(60, 0.35, 1, 1, rng)
+    init, ups = mixed_stream_small(s, d, el, 60, 150, 0.8, 1, rng)
+    q = random_connected_query(6, 3, 1, 1, rng)
+    full = O.run("graphflow", O.OGraph(vl, *init), O.OGraph.from_query(q), *ups)
+    ex = full.pos + full.neg
+    cnt, unf = O.graphflow_limited(O.OGraph(vl, *init), O.OGraph.from_query(q), *ups,
+                                   limit=len(ex), total_seconds=1e-9)
+    assert unf.all() and (cnt <= ex).all()
+    cnt, unf = O.graphflow_limited(O.OGraph(vl, *init), O.OGraph.from_query(q), *ups,
+                                   limit=len(ex), update_seconds=30.0)
+    assert not unf.any()
+    np.testing.assert_array_equal(cnt, ex)
