@@ -112,6 +112,27 @@ typedef struct csm_opts {
 csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph* data,
                        const csm_opts* opts, int* err);
 
+/* csm_create with the data graph given as graph-file records already in HBM
+ * (d_records: n csm_update records of type 'v' / 'e', e.g. from
+ * csm_parse_text_device or csm_parse_text_to_device in CSM_PARSE_GRAPH
+ * mode).  The CSR is built from them on the device, as Graph::LoadFromFile
+ * and AddVertex / AddEdge build it (graph.cpp:23-93, 217-249): vertex ids run
+ * to the largest id of any record, a vertex keeps the label of its first 'v'
+ * record (one without any has no label and matches nothing), edges keep file
+ * order and the first copy of a repeated edge.  Only the per-vertex labels
+ * and the set of edge labels come back to the host. */
+csm_engine* csm_create_from_records(const char* algo, const csm_graph* query,
+                                    const csm_update* d_records, size_t n, const csm_opts* opts,
+                                    int* err);
+
+/* Host text of a graph file parsed on `device` into records that stay in HBM
+ * (allocated by the library; free with csm_free_records): the input of
+ * csm_create_from_records without a host copy of the records.  Same grammar,
+ * errors and *err_line as csm_parse_text.  No reference counterpart. */
+int csm_parse_text_to_device(int device, int mode, const char* text, size_t len,
+                             csm_update** d_records, size_t* n, uint64_t* err_line);
+void csm_free_records(int device, csm_update* d_records);
+
 /* Registers one more standing query on the same data graph (serving); the
  * first query is id 0.  No reference counterpart (the reference runs one
  * query per process). */

@@ -191,15 +191,9 @@ int main(int argc, char** argv) {
 
   auto t0 = std::chrono::steady_clock::now();
   std::cout << "----------- Loading graphs ------------" << std::endl;
-  TextGraph q, d;
+  TextGraph q;
   if (!load_graph(query_path, (int)device, q)) return -1;
   std::cout << "# vertices = " << q.vl.size() << "\n# edges = " << count_unique_edges(q) << std::endl;
-  if (!load_graph(data_path, (int)device, d)) return -1;
-  std::cout << "# vertices = " << d.vl.size() << "\n# edges = " << count_unique_edges(d) << std::endl;
-  std::cout << "Load Graphs: " << ms_since(t0) << "ms" << std::endl;
-
-  std::cout << "------------ Preprocessing ------------" << std::endl;
-  t0 = std::chrono::steady_clock::now();
   csm_opts o;
   memset(&o, 0, sizeof o);
   o.max_results = max_results == UINT32_MAX ? 0 : max_results;
@@ -212,15 +206,46 @@ int main(int argc, char** argv) {
   o.time_limit_s = 0;   // set per phase below
   o.device = device;
   o.reference_orders = reference_orders;
-  csm_graph cq = as_graph(q), cd = as_graph(d);
+  // the data graph: its text parsed on the device into records that stay
+  // there, and the engine's CSR built from them on the device (the
+  // reference's LoadFromFile, graph.cpp:217-249)
+  csm_graph cq = as_graph(q);
+  std::vector<char> text;
+  if (!file_exists(data_path) || !read_file(data_path, text)) {
+    std::cout << "Failed to open: " << data_path << std::endl;
+    return -1;
+  }
+  csm_update* d_rec = nullptr;
+  size_t n_rec = 0;
+  uint64_t bad = 0;
+  int rc = csm_parse_text_to_device((int)device, CSM_PARSE_GRAPH, text.data(), text.size(), &d_rec,
+                                    &n_rec, &bad);
+  if (rc == CSM_E_ARG && bad) {
+    std::cout << "malformed line " << bad << " in " << data_path << std::endl;
+    return -1;
+  }
+  if (rc) {
+    std::cout << "parsing " << data_path << " on the device failed (" << rc << ")" << std::endl;
+    return -1;
+  }
   int err = 0;
-  csm_engine* e = csm_create(algorithm.c_str(), &cq, &cd, &o, &err);
+  csm_engine* e = csm_create_from_records(algorithm.c_str(), &cq, d_rec, n_rec, &o, &err);
+  csm_free_records((int)device, d_rec);
   if (!e) {
     if (err == CSM_E_UNKNOWN_ALGO) std::cout << "Unknown algorithm" << std::endl;
     else std::cout << "engine creation failed (" << err << ")" << std::endl;
     return -1;
   }
-  int rc = csm_preprocess(e);
+  {
+    size_t de = 0, dv = 0;
+    csm_memory(e, &de, &dv);
+    std::cout << "# vertices = " << dv << "\n# edges = " << de << std::endl;
+  }
+  std::cout << "Load Graphs: " << ms_since(t0) << "ms" << std::endl;
+
+  std::cout << "------------ Preprocessing ------------" << std::endl;
+  t0 = std::chrono::steady_clock::now();
+  rc = csm_preprocess(e);
   if (rc) return die(e, rc);
   std::cout << "Preprocessing: " << ms_since(t0) << "ms" << std::endl;
 

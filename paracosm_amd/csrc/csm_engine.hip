@@ -524,27 +524,18 @@ int sort_keys(csm_engine* e, uint64_t* in, uint64_t* out, uint32_t* vin, uint32_
 // ------------------------------------------------------------ graph build
 int dir_init(csm_engine* e, uint64_t cap);
 
-int build_graph(csm_engine* e, const csm_graph* d) {
-  const uint64_t m = d->num_edges;
+// The initial CSR from edge arrays in HBM (m edges; dense edge labels).
+// Self-loops and repeated edges are dropped, the first copy of an edge kept
+// (Graph::AddEdge, graph.cpp:58-93).  The arrays are freed here.
+int build_graph_dev(csm_engine* e, uint32_t* d_src, uint32_t* d_dst, uint32_t* d_el, uint64_t m) {
   const uint64_t n2 = 2 * m;
-  uint32_t *d_src = nullptr, *d_dst = nullptr, *d_el = nullptr;
   uint64_t *k1 = nullptr, *k2 = nullptr;
   uint32_t *v1 = nullptr, *v2 = nullptr, *vend = nullptr;
   int rc;
-  if ((rc = dalloc(e, &d_src, m)) || (rc = dalloc(e, &d_dst, m)) || (rc = dalloc(e, &d_el, m)) ||
-      (rc = dalloc(e, &k1, n2)) || (rc = dalloc(e, &k2, n2)) || (rc = dalloc(e, &v1, n2)) ||
+  if ((rc = dalloc(e, &k1, n2)) || (rc = dalloc(e, &k2, n2)) || (rc = dalloc(e, &v1, n2)) ||
       (rc = dalloc(e, &v2, n2)) || (rc = dalloc(e, &vend, e->nv_cap)))
     return rc;
-  std::vector<uint32_t> del(m);
-  for (uint64_t i = 0; i < m; ++i) {
-    if (d->src[i] >= e->nv || d->dst[i] >= e->nv)
-      return fail(e, CSM_E_ARG, "edge %llu references a vertex >= %u", (unsigned long long)i, e->nv);
-    del[i] = dense_of(e->el_map, d->elabels[i]);
-  }
   if (m) {
-    HIP_TRY(e, hipMemcpyAsync(d_src, d->src, m * 4, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(e, hipMemcpyAsync(d_dst, d->dst, m * 4, hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(e, hipMemcpyAsync(d_el, del.data(), m * 4, hipMemcpyHostToDevice, e->stream));
     k_build_keys<<<grid_for(m, BUILD_BS), BUILD_BS, 0, e->stream>>>(d_src, d_dst, m, k1, v1);
     if ((rc = sort_keys(e, k1, k2, v1, v2, n2))) return rc;
     k_build_pack<<<grid_for(n2, BUILD_BS), BUILD_BS, 0, e->stream>>>(k2, v2, n2, e->d_vlab, d_el,
@@ -577,6 +568,115 @@ int build_graph(csm_engine* e, const csm_graph* d) {
   uint64_t nlong = 0;
   for (auto& r : rec) nlong += (r.y & LEN_MASK) >= e->dir_threshold;
   return dir_init(e, nlong + nlong / 4 + e->nv_cap / 64 + 1024);
+}
+
+// The initial CSR from host edge arrays.
+int build_graph(csm_engine* e, const csm_graph* d) {
+  const uint64_t m = d->num_edges;
+  uint32_t *d_src = nullptr, *d_dst = nullptr, *d_el = nullptr;
+  int rc;
+  if ((rc = dalloc(e, &d_src, m)) || (rc = dalloc(e, &d_dst, m)) || (rc = dalloc(e, &d_el, m)))
+    return rc;
+  std::vector<uint32_t> del(m);
+  for (uint64_t i = 0; i < m; ++i) {
+    if (d->src[i] >= e->nv || d->dst[i] >= e->nv)
+      return fail(e, CSM_E_ARG, "edge %llu references a vertex >= %u", (unsigned long long)i, e->nv);
+    del[i] = dense_of(e->el_map, d->elabels[i]);
+  }
+  if (m) {
+    HIP_TRY(e, hipMemcpyAsync(d_src, d->src, m * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(d_dst, d->dst, m * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(d_el, del.data(), m * 4, hipMemcpyHostToDevice, e->stream));
+  }
+  return build_graph_dev(e, d_src, d_dst, d_el, m);
+}
+
+// The initial CSR from graph-file records in HBM (n records): the records
+// become the build's edge arrays on the device, and the label-triple
+// histogram of the SymBi root choice is counted on the way.
+int build_graph_records(csm_engine* e, const UpdateRec* d_rec, uint64_t n) {
+  uint32_t *d_src = nullptr, *d_dst = nullptr, *d_el = nullptr;
+  unsigned long long* hist = nullptr;
+  const size_t nh = (size_t)e->nlab * e->nlab * e->nel;
+  int rc;
+  if ((rc = dalloc(e, &d_src, n)) || (rc = dalloc(e, &d_dst, n)) || (rc = dalloc(e, &d_el, n)) ||
+      (rc = dalloc(e, &hist, nh)))
+    return rc;
+  HIP_TRY(e, hipMemsetAsync(hist, 0, nh * 8, e->stream));
+  if (n)
+    k_rec_edges<<<grid_for(n, 256), 256, 0, e->stream>>>(d_rec, n, e->d_vlab, e->d_elmap, e->nel,
+                                                          e->nlab, d_src, d_dst, d_el, hist);
+  e->h_triple_freq.assign(nh, 0);
+  HIP_TRY(e, hipMemcpyAsync(e->h_triple_freq.data(), hist, nh * 8, hipMemcpyDeviceToHost,
+                            e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  dfree(e, hist);
+  return build_graph_dev(e, d_src, d_dst, d_el, n);
+}
+
+// Graph-file records in HBM: vertex count (largest id + 1), each vertex's
+// label from its first 'v' record (NO_LABEL without one), the 'e' record
+// count and the set of edge labels, all computed on the device; only the
+// per-vertex labels and the label set come to the host.
+int scan_records(csm_engine* e, const UpdateRec* d_rec, uint64_t n, uint32_t* nv,
+                 std::vector<uint32_t>& raw_vl, uint64_t* ne, std::vector<uint32_t>& els) {
+  int rc;
+  uint32_t* cnt = nullptr;       // [0] vertex count, [2..3] edge count (u64)
+  if ((rc = dalloc(e, &cnt, 4))) return rc;
+  HIP_TRY(e, hipMemsetAsync(cnt, 0, 16, e->stream));
+  if (n)
+    k_rec_maxid<<<grid_for(n, 256), 256, 0, e->stream>>>(
+        d_rec, n, cnt, reinterpret_cast<unsigned long long*>(cnt + 2));
+  uint32_t h[4] = {0, 0, 0, 0};
+  HIP_TRY(e, hipMemcpyAsync(h, cnt, 16, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  dfree(e, cnt);
+  *nv = h[0];
+  *ne = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
+  // labels from the first 'v' record of each vertex
+  unsigned long long* first = nullptr;
+  uint32_t* vl = nullptr;
+  if ((rc = dalloc(e, &first, std::max<uint32_t>(*nv, 1))) ||
+      (rc = dalloc(e, &vl, std::max<uint32_t>(*nv, 1))))
+    return rc;
+  HIP_TRY(e, hipMemsetAsync(first, 0xFF, (size_t)std::max<uint32_t>(*nv, 1) * 8, e->stream));
+  HIP_TRY(e, hipMemsetAsync(vl, 0xFF, (size_t)std::max<uint32_t>(*nv, 1) * 4, e->stream));
+  if (n) {
+    k_rec_first_vertex<<<grid_for(n, 256), 256, 0, e->stream>>>(d_rec, n, first);
+    k_rec_vlabels<<<grid_for(n, 256), 256, 0, e->stream>>>(d_rec, n, first, vl);
+  }
+  raw_vl.assign(*nv, NO_LABEL);
+  if (*nv)
+    HIP_TRY(e, hipMemcpyAsync(raw_vl.data(), vl, (size_t)*nv * 4, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  dfree(e, first);
+  dfree(e, vl);
+  // edge label set: sort the records' labels, keep the distinct ones
+  els.clear();
+  if (!*ne) return CSM_OK;
+  uint32_t *lab = nullptr, *lab2 = nullptr, *uniq = nullptr;
+  size_t* nuniq = nullptr;
+  if ((rc = dalloc(e, &lab, n)) || (rc = dalloc(e, &lab2, n)) || (rc = dalloc(e, &uniq, n)) ||
+      (rc = dalloc(e, &nuniq, 1)))
+    return rc;
+  k_rec_elabels<<<grid_for(n, 256), 256, 0, e->stream>>>(d_rec, n, lab);
+  size_t need = 0, need2 = 0;
+  HIP_TRY(e, rocprim::radix_sort_keys(nullptr, need, lab, lab2, (size_t)n, 0, 32, e->stream));
+  HIP_TRY(e, rocprim::unique(nullptr, need2, lab2, uniq, nuniq, (size_t)n,
+                             rocprim::equal_to<uint32_t>(), e->stream));
+  uint8_t* tmp = nullptr;
+  if ((rc = dalloc(e, &tmp, std::max(need, need2)))) return rc;
+  HIP_TRY(e, rocprim::radix_sort_keys(tmp, need, lab, lab2, (size_t)n, 0, 32, e->stream));
+  HIP_TRY(e, rocprim::unique(tmp, need2, lab2, uniq, nuniq, (size_t)n,
+                             rocprim::equal_to<uint32_t>(), e->stream));
+  size_t k = 0;
+  HIP_TRY(e, hipMemcpyAsync(&k, nuniq, sizeof k, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  els.resize(k);
+  if (k) HIP_TRY(e, hipMemcpy(els.data(), uniq, k * 4, hipMemcpyDeviceToHost));
+  els.erase(std::remove(els.begin(), els.end(), NO_LABEL), els.end());
+  dfree(e, lab); dfree(e, lab2); dfree(e, uniq); dfree(e, nuniq); dfree(e, tmp);
+  return CSM_OK;
 }
 
 // Allocate the key directory pool for `cap` slots and build the directories
@@ -1119,23 +1219,36 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
 }
 
 // ------------------------------------------------------- batches as graphs
-// Steady-state batches (autotuning done, no time limit) run as a HIP graph:
-// k_batch_begin, unpack, the graph and index update, the seeds and a fixed
-// number of search rounds (each followed by k_next_round, so that a round
-// with nothing to do ends at once), replayed with one launch.  The graph
-// reads and writes the engine's own staging buffers (d_upd, d_counts,
-// d_safe, d_hashes), so a caller's buffers are copied in and out around it,
-// and is keyed by everything its launches bake in (batch size, ping-pong
-// buffers, allocations).  Rounds still pending after it continue eagerly.
-// The first batch of a key runs eagerly (sizes and allocations settle), the
-// second is captured.
-constexpr size_t MAX_GRAPHS = 8;
+// Small batches (up to GRAPH_MAX_BATCH updates: the plain per-update path,
+// serving at low and moderate load), where launch and host round-trip
+// latency weigh most, run as a HIP graph: k_batch_begin, unpack, the graph
+// and index update, the seeds and a fixed number of search rounds (each
+// followed by k_next_round, so that a round with nothing to do ends at
+// once), replayed with one launch and one read-back.  The batch is padded
+// with no-op records to a power-of-two bucket, so that batches of varying
+// size share a few graphs.  The graph reads and writes the engine's own
+// staging buffers (d_upd, d_counts, d_safe, d_hashes), so a caller's buffers
+// are copied in and out around it, and is keyed by everything its launches
+// bake in (bucket, ping-pong buffers, allocations).  Rounds still pending
+// after it continue eagerly.  The first batch of a key runs eagerly (sizes
+// and allocations settle), the second is captured.  Larger batches run
+// launch by launch: there the overhead is well under 1 % of a batch, and
+// the search kernel's time is measured per launch with HIP events.
+constexpr size_t MAX_GRAPHS = 16;
+constexpr uint32_t GRAPH_MAX_BATCH = 4096;
+constexpr uint32_t GRAPH_MAX_ROUNDS = 6;
 
 bool graphs_allowed(const csm_engine* e, uint32_t n) {
   static const bool off = getenv("CSM_NO_GRAPHS") != nullptr || getenv("CSM_TRACE_ROUNDS") != nullptr;
-  // batches the autotuner still scores run eagerly; smaller ones never tune
-  const bool tuning = e->tune_active && n >= TUNE_MIN_BATCH;
-  return !off && !e->opts.no_graphs && !tuning && e->deadline <= 0 && e->algo != ALGO_NONE;
+  return !off && !e->opts.no_graphs && n <= GRAPH_MAX_BATCH && e->deadline <= 0 &&
+         e->algo != ALGO_NONE;
+}
+
+// the bucket a batch of n updates is padded to
+uint32_t graph_bucket(const csm_engine* e, uint32_t n) {
+  uint32_t b = 64;
+  while (b < n) b <<= 1;
+  return std::min(b, e->max_batch);
 }
 
 csm_engine::GraphKey graph_key(const csm_engine* e, uint32_t n, bool hash) {
@@ -1175,16 +1288,12 @@ int enqueue_batch_head(csm_engine* e, uint32_t n, bool hash, uint32_t rounds) {
   seeds(e, n, counts, hashes, e->d_safe, true);
   SearchArgs sa = search_args(e, counts, hashes, 0);
   const bool symbi = uses_index(e) && e->d_d2 != nullptr;
-  if (hipEventRecord(e->ev_search[0], e->stream) != hipSuccess)
-    return fail(e, CSM_E_DEVICE, "event record in capture failed");
   for (uint32_t r = 0; r < rounds; ++r) {
     round_buffers(e, sa, r);
     if (hash) { if (symbi) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa); }
     else { if (symbi) launch_search<false, true>(e, sa); else launch_search<false, false>(e, sa); }
     k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl, e->spill_chunk_cap);
   }
-  if (hipEventRecord(e->ev_search[1], e->stream) != hipSuccess)
-    return fail(e, CSM_E_DEVICE, "event record in capture failed");
   return CSM_OK;
 }
 
@@ -1192,7 +1301,7 @@ int enqueue_batch_head(csm_engine* e, uint32_t n, bool hash, uint32_t rounds) {
 // marked bad) if the capture failed, allocated, or holds a memset node.
 bool graph_capture(csm_engine* e, csm_engine::GraphEntry& g, uint32_t n, bool hash) {
   const uint64_t gen0 = e->gen;
-  const uint32_t rounds = std::min<uint32_t>(std::max<uint32_t>(e->max_rounds, 2u), 12u);
+  const uint32_t rounds = std::min<uint32_t>(std::max<uint32_t>(e->max_rounds, 2u), GRAPH_MAX_ROUNDS);
   if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
     g.bad = true;
     return false;
@@ -1228,9 +1337,10 @@ bool graph_capture(csm_engine* e, csm_engine::GraphEntry& g, uint32_t n, bool ha
 int run_batch_graph(csm_engine* e, const csm_update* d_updates, uint32_t n, uint64_t* d_counts,
                     uint8_t* d_safe, uint64_t* d_hash) {
   int rc;
-  if ((rc = batch_prepare(e, n))) return rc;
+  const uint32_t B = graph_bucket(e, n);
+  if ((rc = batch_prepare(e, B))) return rc;
   const bool hash = d_hash != nullptr;
-  const csm_engine::GraphKey key = graph_key(e, n, hash);
+  const csm_engine::GraphKey key = graph_key(e, B, hash);
   csm_engine::GraphEntry* g = nullptr;
   for (auto& x : e->graphs)
     if (x.key == key) g = &x;
@@ -1247,23 +1357,25 @@ int run_batch_graph(csm_engine* e, const csm_update* d_updates, uint32_t n, uint
   }
   if (g->bad) return 1;
   const size_t nq = e->queries.size();
+  if (!g->exec && !graph_capture(e, *g, B, hash)) return 1;
+  // the caller's records, then no-op records (type 0) up to the bucket
   if (reinterpret_cast<const void*>(d_updates) != e->d_upd)
     HIP_TRY(e, hipMemcpyAsync(e->d_upd, d_updates, (size_t)n * sizeof(UpdateRec),
                               hipMemcpyDeviceToDevice, e->stream));
-  if (!g->exec && !graph_capture(e, *g, n, hash)) return 1;
+  if (B > n) {
+    const uint64_t w = (uint64_t)(B - n) * sizeof(UpdateRec) / 4;
+    k_fill_u32<<<grid_for(w, 256), 256, 0, e->stream>>>(reinterpret_cast<uint32_t*>(e->d_upd + n),
+                                                        w, 0u);
+  }
   HIP_TRY(e, hipGraphLaunch(g->exec, e->stream));
   ++e->graph_launches;
-  e->stats[6] += g->rounds;
-  e->ev_pending = true;
   if ((rc = read_ctrl(e))) return rc;
-  collect_search_time(e);
-  e->stats[6] -= 1;   // collect_search_time counted the event pair as one launch
   e->max_rounds = std::max(e->max_rounds, e->h_ctrl->rounds + 1);
   if (e->h_ctrl->nchunks > 0) {
     // more rounds than the graph holds: continue eagerly from round g->rounds
     if ((rc = search(e, true, e->d_counts, hash ? e->d_hashes : nullptr, 0, g->rounds))) return rc;
   }
-  if ((rc = batch_tail(e, n, e->d_counts, true))) return rc;
+  if ((rc = batch_tail(e, B, e->d_counts, true))) return rc;
   if (reinterpret_cast<void*>(d_counts) != e->d_counts)
     HIP_TRY(e, hipMemcpyAsync(d_counts, e->d_counts, (size_t)n * nq * 8, hipMemcpyDeviceToDevice,
                               e->stream));
@@ -1876,12 +1988,41 @@ void tune_finish(csm_engine* e) {
 // ================================================================== C-ABI
 extern "C" {
 
+// Where an engine's data graph comes from: host arrays, or graph-file
+// records in HBM.
+struct DataSrc {
+  const csm_graph* host = nullptr;
+  const UpdateRec* d_rec = nullptr;
+  size_t n_rec = 0;
+};
+
+static csm_engine* create_engine(const char* algo, const csm_graph* query, const DataSrc& data,
+                                 const csm_opts* opts, int* err);
+
 csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph* data,
                        const csm_opts* opts, int* err) {
+  if (!data) { if (err) *err = CSM_E_ARG; return nullptr; }
+  DataSrc src;
+  src.host = data;
+  return create_engine(algo, query, src, opts, err);
+}
+
+csm_engine* csm_create_from_records(const char* algo, const csm_graph* query,
+                                    const csm_update* d_records, size_t n, const csm_opts* opts,
+                                    int* err) {
+  if (n && !d_records) { if (err) *err = CSM_E_ARG; return nullptr; }
+  DataSrc src;
+  src.d_rec = reinterpret_cast<const UpdateRec*>(d_records);
+  src.n_rec = n;
+  return create_engine(algo, query, src, opts, err);
+}
+
+static csm_engine* create_engine(const char* algo, const csm_graph* query, const DataSrc& data,
+                                 const csm_opts* opts, int* err) {
   auto set = [&](int c) { if (err) *err = c; };
   int a;
   if (parse_algo(algo, &a)) { set(CSM_E_UNKNOWN_ALGO); return nullptr; }
-  if (!query || !data) { set(CSM_E_ARG); return nullptr; }
+  if (!query) { set(CSM_E_ARG); return nullptr; }
   std::unique_ptr<csm_engine> e(new csm_engine());
   e->algo = a;
   e->algo_name = algo;
@@ -1911,9 +2052,21 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
 
 
+  // the data graph's vertex labels, edge count and edge label set
+  std::vector<uint32_t> raw_vl, els;
+  uint32_t nv = 0;
+  uint64_t ne = 0;
+  if (data.host) {
+    nv = data.host->num_vertices;
+    ne = data.host->num_edges;
+    raw_vl.assign(data.host->vlabels, data.host->vlabels + nv);
+    els.assign(data.host->elabels, data.host->elabels + ne);
+  } else {
+    const int rc = scan_records(e.get(), data.d_rec, data.n_rec, &nv, raw_vl, &ne, els);
+    if (rc) { set(rc); return nullptr; }
+  }
   // label maps from data and query labels
-  std::vector<uint32_t> vls(data->vlabels, data->vlabels + data->num_vertices);
-  std::vector<uint32_t> els(data->elabels, data->elabels + data->num_edges);
+  std::vector<uint32_t> vls(raw_vl);
   els.insert(els.end(), query->elabels, query->elabels + query->num_edges);
   std::sort(vls.begin(), vls.end());
   vls.erase(std::unique(vls.begin(), vls.end()), vls.end());
@@ -1926,7 +2079,7 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   e->el_map = els;
   e->nlab = (uint32_t)vls.size();
   e->nel = (uint32_t)els.size();
-  e->nv = data->num_vertices;
+  e->nv = nv;
   uint64_t keys = (uint64_t)e->nlab * e->nel;
   uint32_t kbits = bits_for(keys);
   uint64_t want_cap = (uint64_t)e->nv + e->nv / 8 + 1024;
@@ -1946,27 +2099,33 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
   if (e->nv_cap < e->nv) e->nv_cap = e->nv;
   e->d2_words = (e->nv_cap + 31) / 32;
   e->h_vlab.assign(e->nv_cap, NO_LABEL);
-  for (uint32_t v = 0; v < e->nv; ++v) e->h_vlab[v] = dense_of(e->vl_map, data->vlabels[v]);
+  for (uint32_t v = 0; v < e->nv; ++v) e->h_vlab[v] = dense_of(e->vl_map, raw_vl[v]);
 
-  // label histograms of the initial graph for the SymBi root choice
+  // label histograms of the initial graph for the SymBi root choice (the
+  // triple histogram of device records is counted while they are built)
   e->h_triple_freq.assign((size_t)e->nlab * e->nlab * e->nel, 0);
   e->h_vfreq.assign(e->nlab, 0);
   for (uint32_t v = 0; v < e->nv; ++v)
     if (e->h_vlab[v] != NO_LABEL) ++e->h_vfreq[e->h_vlab[v]];
-  for (size_t i = 0; i < data->num_edges; ++i) {
-    uint32_t a = data->src[i], b = data->dst[i];
-    if (a >= e->nv || b >= e->nv || a == b) continue;
-    uint32_t la = e->h_vlab[a], lb = e->h_vlab[b], l = dense_of(e->el_map, data->elabels[i]);
-    if (la == NO_LABEL || lb == NO_LABEL || l == NO_LABEL) continue;
-    ++e->h_triple_freq[(la * e->nlab + lb) * e->nel + l];
-    ++e->h_triple_freq[(lb * e->nlab + la) * e->nel + l];
+  if (data.host) {
+    const csm_graph* d = data.host;
+    for (size_t i = 0; i < d->num_edges; ++i) {
+      uint32_t a = d->src[i], b = d->dst[i];
+      if (a >= e->nv || b >= e->nv || a == b) continue;
+      uint32_t la = e->h_vlab[a], lb = e->h_vlab[b], l = dense_of(e->el_map, d->elabels[i]);
+      if (la == NO_LABEL || lb == NO_LABEL || l == NO_LABEL) continue;
+      ++e->h_triple_freq[(la * e->nlab + lb) * e->nel + l];
+      ++e->h_triple_freq[(lb * e->nlab + la) * e->nel + l];
+    }
   }
 
   int rc = CSM_OK;
   // 8x the initial directed entries: a batch may need room for twice the
   // live entries (every list rewritten by compaction and merge), so a pool
   // of 4x left room for about one batch of hub rewrites between repacks
-  e->pool_cap = std::max<uint64_t>(8 * 2 * data->num_edges + 8ull * e->max_batch, 1u << 20);
+  // (records build from every record, a non-edge one as a dropped self-loop)
+  const uint64_t build_edges = data.host ? ne : data.n_rec;
+  e->pool_cap = std::max<uint64_t>(8 * 2 * build_edges + 8ull * e->max_batch, 1u << 20);
   if (e->pool_cap > 0xFFFFFFF0ull) e->pool_cap = 0xFFFFFFF0ull;
   if ((rc = dalloc(e.get(), &e->d_vrec, e->nv_cap)) || (rc = dalloc(e.get(), &e->d_vlab, e->nv_cap)) ||
       (rc = dalloc(e.get(), &e->d_adj[0], e->pool_cap)) ||
@@ -1988,7 +2147,9 @@ csm_engine* csm_create(const char* algo, const csm_graph* query, const csm_graph
     set(CSM_E_DEVICE);
     return nullptr;
   }
-  if ((rc = build_graph(e.get(), data)) || (rc = alloc_batch(e.get()))) { set(rc); return nullptr; }
+  rc = data.host ? build_graph(e.get(), data.host)
+                 : build_graph_records(e.get(), data.d_rec, data.n_rec);
+  if (rc || (rc = alloc_batch(e.get()))) { set(rc); return nullptr; }
   std::unique_ptr<QueryInfo> qi;
   if ((rc = build_query(e.get(), query, qi))) { set(rc); return nullptr; }
   e->queries.push_back(std::move(qi));
@@ -2480,6 +2641,51 @@ int csm_parse_text_device_static(csm_engine* e, int mode, const char* d_text, si
   HIP_TRY(e, hipSetDevice(e->dev));
   return parse_static(e, mode, reinterpret_cast<const uint8_t*>(d_text), len, d_out, cap, n,
                       err_line);
+}
+
+int csm_parse_text_to_device(int device, int mode, const char* text, size_t len,
+                             csm_update** d_records, size_t* n, uint64_t* err_line) {
+  if (!n || !d_records || (len && !text)) return CSM_E_ARG;
+  *n = 0;
+  *d_records = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return CSM_E_DEVICE;
+  hipStream_t st = nullptr;
+  if (hipStreamCreate(&st) != hipSuccess) return CSM_E_DEVICE;
+  auto alloc = [](size_t b) -> void* { void* p = nullptr; return hipMalloc(&p, b) == hipSuccess ? p : nullptr; };
+  auto release = [](void* p) { (void)hipFree(p); };
+  int rc = CSM_OK;
+  uint8_t* d_text = static_cast<uint8_t*>(len ? alloc(len) : nullptr);
+  if (len && !d_text) rc = CSM_E_DEVICE;
+  if (!rc && len && hipMemcpyAsync(d_text, text, len, hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = CSM_E_DEVICE;
+  std::string msg;
+  if (!rc)
+    rc = parse_text_on(st, mode, d_text, len, n, err_line, alloc, release,
+                       [&](const UpdateRec* recs, size_t cnt) -> int {
+                         void* p = alloc(std::max<size_t>(cnt, 1) * sizeof(UpdateRec));
+                         if (!p) return CSM_E_DEVICE;
+                         if (cnt && hipMemcpyAsync(p, recs, cnt * sizeof(UpdateRec),
+                                                   hipMemcpyDeviceToDevice, st) != hipSuccess) {
+                           release(p);
+                           return CSM_E_DEVICE;
+                         }
+                         *d_records = static_cast<csm_update*>(p);
+                         return CSM_OK;
+                       },
+                       msg);
+  if (d_text) release(d_text);
+  (void)hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
+  if (rc && *d_records) {
+    release(*d_records);
+    *d_records = nullptr;
+  }
+  return rc;
+}
+
+void csm_free_records(int device, csm_update* d_records) {
+  if (!d_records || hipSetDevice(device) != hipSuccess) return;
+  (void)hipFree(d_records);
 }
 
 int csm_parse_text(int device, int mode, const char* text, size_t len, csm_update* out,

@@ -260,6 +260,9 @@ __global__ void k_build_pack(const uint64_t* __restrict__ keys, const uint32_t* 
     bool dup = (k == ~0ull) || (i > 0 && keys[i - 1] == k);
     if (dup) { out[i] = ~0ull; continue; }
     uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+    // an edge to a vertex without a label (never declared by a 'v' record)
+    // takes part in no match and has no segment key: it is left out
+    if (vlab[a] == NO_LABEL || vlab[b] == NO_LABEL) { out[i] = ~0ull; continue; }
     uint32_t key = vlab[b] * nel + elab_dense[vals[i]];
     out[i] = ((uint64_t)a << 32) | ((key << vbits) | b);
   }

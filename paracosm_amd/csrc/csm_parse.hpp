@@ -174,3 +174,91 @@ __global__ void k_parse_records(const uint8_t* __restrict__ text, uint64_t len,
 }
 
 }  // namespace csm
+
+namespace csm {
+
+// ---------------------------------------------------------------------------
+// A data graph straight from graph-file records in HBM (Graph::LoadFromFile,
+// graph.cpp:217-249, with AddVertex / AddEdge, graph.cpp:23-93): vertex ids
+// run to the largest id of any record; a vertex keeps the label of its first
+// 'v' record (later ones are ignored, as AddVertex does) and a vertex without
+// one has no label; every 'e' record is an edge in file order (duplicates are
+// dropped by the CSR build, keeping the first, as AddEdge does).
+
+// largest vertex id of any record (+1 = vertex count); *maxid starts at 0
+__global__ void k_rec_maxid(const UpdateRec* __restrict__ r, uint64_t n, uint32_t* maxid,
+                            unsigned long long* nedges) {
+  uint32_t m = 0;
+  unsigned long long ne = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const UpdateRec u = r[i];
+    const uint32_t hi = u.type == 'e' ? (u.id1 > u.id2 ? u.id1 : u.id2) : u.id1;
+    m = hi + 1 > m ? hi + 1 : m;
+    ne += u.type == 'e';
+  }
+  atomicMax(maxid, m);
+  if (ne) atomicAdd(nedges, ne);
+}
+
+// first 'v' record of each vertex (first[] starts at all ones)
+__global__ void k_rec_first_vertex(const UpdateRec* __restrict__ r, uint64_t n,
+                                   unsigned long long* __restrict__ first) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    if (r[i].type == 'v') atomicMin(&first[r[i].id1], (unsigned long long)i);
+}
+
+// label of each vertex from its first 'v' record (vl[] starts at NO_LABEL)
+__global__ void k_rec_vlabels(const UpdateRec* __restrict__ r, uint64_t n,
+                              const unsigned long long* __restrict__ first,
+                              uint32_t* __restrict__ vl) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    if (r[i].type == 'v' && first[r[i].id1] == i) vl[r[i].id1] = r[i].label;
+}
+
+// edge label of each record (NO_LABEL for the others), for the label set
+__global__ void k_rec_elabels(const UpdateRec* __restrict__ r, uint64_t n, uint32_t* __restrict__ el) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    el[i] = r[i].type == 'e' ? r[i].label : NO_LABEL;
+}
+
+// the records as the CSR build's edge arrays: an 'e' record becomes
+// (id1, id2, dense edge label); any other record a self-loop, which the
+// build drops.  hist (nullable) counts each edge's (L(a), L(b), label)
+// triple in both orientations (the SymBi root choice, symbi.cpp:113-175).
+__global__ void k_rec_edges(const UpdateRec* __restrict__ r, uint64_t n,
+                            const uint32_t* __restrict__ vlab, const uint32_t* __restrict__ elmap,
+                            uint32_t nel, uint32_t nlab, uint32_t* __restrict__ src,
+                            uint32_t* __restrict__ dst, uint32_t* __restrict__ del,
+                            unsigned long long* __restrict__ hist) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const UpdateRec u = r[i];
+    if (u.type != 'e') {
+      src[i] = dst[i] = 0;
+      del[i] = 0;
+      continue;
+    }
+    // dense edge label: position in the sorted label map
+    uint32_t lo = 0, hi = nel;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (elmap[mid] < u.label) lo = mid + 1; else hi = mid;
+    }
+    src[i] = u.id1;
+    dst[i] = u.id2;
+    del[i] = lo;
+    if (hist && u.id1 != u.id2) {
+      const uint32_t la = vlab[u.id1], lb = vlab[u.id2];
+      if (la < nlab && lb < nlab) {
+        atomicAdd(&hist[((uint64_t)la * nlab + lb) * nel + lo], 1ull);
+        atomicAdd(&hist[((uint64_t)lb * nlab + la) * nel + lo], 1ull);
+      }
+    }
+  }
+}
+
+}  // namespace csm

@@ -82,6 +82,11 @@ def lib():
         P = ctypes.POINTER
         sig = {
             "csm_create": (vp, [ctypes.c_char_p, P(_Graph), P(_Graph), P(_Opts), P(ctypes.c_int)]),
+            "csm_create_from_records": (vp, [ctypes.c_char_p, P(_Graph), vp, sz, P(_Opts),
+                                             P(ctypes.c_int)]),
+            "csm_parse_text_to_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, vp, sz,
+                                                        P(vp), P(sz), P(u64)]),
+            "csm_free_records": (None, [ctypes.c_int, vp]),
             "csm_register_query": (ctypes.c_int, [vp, P(_Graph), P(u32)]),
             "csm_num_queries": (u32, [vp]),
             "csm_preprocess": (ctypes.c_int, [vp]),
@@ -134,7 +139,8 @@ def lib():
 
 
 def exported_symbols() -> Sequence[str]:
-    return ("csm_create", "csm_register_query", "csm_num_queries", "csm_preprocess",
+    return ("csm_create", "csm_create_from_records", "csm_parse_text_to_device",
+            "csm_free_records", "csm_register_query", "csm_num_queries", "csm_preprocess",
             "csm_initial", "csm_classify", "csm_add_edge", "csm_remove_edge",
             "csm_add_vertex", "csm_remove_vertex", "csm_apply_batch",
             "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_update_status",
@@ -174,8 +180,54 @@ def make_updates(add, v1, v2, label, vertex=None) -> np.ndarray:
     return u
 
 
+def _opt_values(max_results: int = 0, print_prep: bool = False, print_enum: bool = False,
+                homo: bool = False, threads: int = 0, auto_tuning: int = 1, max_batch: int = 0,
+                want_hash: bool = False, device_budget: int = 0, split_chunk: int = 0,
+                device: int = 0, match_budget: int = 0, spill_budget: int = 0,
+                reference_orders: bool = False, time_limit_s: float = 0.0,
+                want_work: bool = False, no_graphs: bool = False) -> dict:
+    """csm_opts fields from Matching's keyword arguments."""
+    return dict(max_results=max_results, print_prep=int(print_prep), print_enum=int(print_enum),
+                homo=int(homo), threads=threads, auto_tuning=auto_tuning, max_batch=max_batch,
+                want_hash=int(want_hash), time_limit_s=time_limit_s, device_budget=device_budget,
+                split_chunk=split_chunk, device=device, match_budget=match_budget,
+                spill_budget=spill_budget, reference_orders=int(reference_orders),
+                want_work=int(want_work), no_graphs=int(no_graphs))
+
+
 class Matching:
     """One device engine: a data graph plus standing queries."""
+
+    @classmethod
+    def from_text(cls, algo: str, query, text: bytes, device: int = 0, **opts) -> "Matching":
+        """An engine whose data graph is a graph file's text (the reference's
+        Graph::LoadFromFile input), parsed on the device and built into the
+        CSR there (csm_parse_text_to_device + csm_create_from_records)."""
+        L = lib()
+        data = bytes(text)
+        d_rec, n, bad = ctypes.c_void_p(0), ctypes.c_size_t(0), ctypes.c_uint64(0)
+        rc = L.csm_parse_text_to_device(device, PARSE_GRAPH, data if data else None, len(data),
+                                        ctypes.byref(d_rec), ctypes.byref(n), ctypes.byref(bad))
+        if rc == -1 and bad.value:
+            raise ParseError(rc, int(bad.value))
+        if rc:
+            raise CsmError(rc, "csm_parse_text_to_device failed")
+        self = cls.__new__(cls)
+        try:
+            self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
+            o = _Opts(**{k: v for k, v in _opt_values(device=device, **opts).items()})
+            err = ctypes.c_int(0)
+            self.h = L.csm_create_from_records(algo.encode(), ctypes.byref(self._q.g), d_rec,
+                                               n.value, ctypes.byref(o), ctypes.byref(err))
+        finally:
+            L.csm_free_records(device, d_rec)
+        if not self.h:
+            if err.value == CSM_E_UNKNOWN_ALGO:
+                raise CsmError(err.value, "Unknown algorithm")
+            raise CsmError(err.value, f"csm_create_from_records failed for {algo!r}")
+        self.algo = algo
+        self.want_hash = bool(opts.get("want_hash", False))
+        return self
 
     def __init__(self, algo: str, query, vlabels, src, dst, elabels, *,
                  max_results: int = 0, print_prep: bool = False, print_enum: bool = False,

@@ -166,3 +166,46 @@ def test_device_text_feeds_the_engine(oracle_lib):
         cnt = np.zeros(n, np.uint64)
         d_cnt.download(cnt.view(np.uint8))
     np.testing.assert_array_equal(cnt, np.where(add == 1, ref.pos, ref.neg))
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_engine_from_graph_text_equals_host_build(oracle_lib, algo):
+    """A data graph file parsed on the device and built into the CSR there
+    (csm_parse_text_to_device + csm_create_from_records) gives the engine
+    that host arrays give: same counts per update, same SymBi index.  The
+    file has what LoadFromFile tolerates: a header, a vertex declared twice
+    (the first label stays), a vertex never declared (no label), a repeated
+    edge (the first copy stays) and an edge line with another leading
+    char."""
+    from paracosm_amd.engine import Matching, make_updates, records_to_graph, parse_text, PARSE_GRAPH
+    from tests.helpers import mixed_stream_small, random_connected_query, random_graph
+    rng = np.random.default_rng(606)
+    nv = 60
+    vl, s, d, el = random_graph(nv, 0.35, 2, 1, rng)
+    init, ups = mixed_stream_small(s, d, el, nv, 200, 0.7, 1, rng)
+    q = random_connected_query(4, 1, 2, 1, rng)
+    lines = ["t # 0"] + [f"v {v} {vl[v]}" for v in range(nv - 1)]   # vertex nv-1 undeclared
+    lines.append(f"v 3 {(int(vl[3]) + 1) % 2}")                       # ignored re-declaration
+    src, dst, lab = init
+    for i, (a, b, c) in enumerate(zip(src, dst, lab)):
+        lines.append(f"{'x' if i % 7 == 0 else 'e'} {a} {b} {c}")
+    lines.append(f"e {src[0]} {dst[0]} 7")                            # repeated edge, other label
+    text = ("\n".join(lines) + "\n").encode()
+    hv, hs, hd, he = records_to_graph(parse_text(text, PARSE_GRAPH))
+    u = make_updates(*ups)
+    keep = (ups[1] != nv - 1) & (ups[2] != nv - 1)   # updates of the unlabelled vertex are refused
+    u = u[keep]
+    with Matching(algo, q, hv, hs, hd, he) as ref, Matching.from_text(algo, q, text) as m:
+        ref.Preprocessing()
+        m.Preprocessing()
+        assert m.GetMemoryCost() == ref.GetMemoryCost()
+        if algo == "parallel_symbi":
+            for what in (0, 1, 2, 3, 4):
+                rows = 1 if what == 0 else q.k
+                np.testing.assert_array_equal(m.index_dump(0, what, nv, rows),
+                                              ref.index_dump(0, what, nv, rows))
+        p1, n1, _, _ = m.apply_batch(u)
+        p2, n2, _, _ = ref.apply_batch(u)
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_array_equal(n1, n2)
+    assert (p1.sum() + n1.sum()) > 0
