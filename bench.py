@@ -467,10 +467,10 @@ def cpu_baseline(w, queries, seconds):
             r = O.run("parallel_graphflow", dg, qg, *args, threads=cores, limit=s)
             tot += r.seconds
         log(f"cpu baseline: {s} updates x {len(qgs)} queries took {tot:.2f}s")
-        if tot * 3 >= seconds or s >= w.num_updates:
+        if tot * 2 >= seconds or s >= w.num_updates:
             break
-        # at most 3x per step: the cost per update varies along the stream
-        s = int(min(w.num_updates, s * min(3.0, max(1.5, 0.8 * seconds / max(tot, 1e-3)))))
+        # at most 2x per step: the cost per update grows along the stream
+        s = int(min(w.num_updates, s * min(2.0, max(1.25, 0.8 * seconds / max(tot, 1e-3)))))
     model = cpu_model()
     log(f"cpu baseline: {s} updates x {len(qgs)} queries in {tot:.2f}s on {cores} threads "
         f"({model}, {avail} CPUs available)")
