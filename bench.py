@@ -115,7 +115,7 @@ def main() -> int:
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--hbm-budget", type=int, default=0,
                     help="served legs: device bytes the engine plus in-flight batches may "
-                         "hold (0 = 90%% of the free HBM at start)")
+                         "hold (0 = 90%% of the device memory)")
     ap.add_argument("--also", default="parallel_symbi,parallel_turboflux",
                     help="other algorithms timed on the same batches (comma list, '' for none)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -356,7 +356,7 @@ def serve_legs(w, queries, args, batched_rate, tuned, batched_counts, n_batched)
                     "requests": st.requests, "updates": st.updates,
                     "batches": st.batches, "mean_batch": round(st.mean_batch, 1),
                     "matches_per_s": round(st.matches / st.elapsed, 1),
-                    "hbm_budget": args.hbm_budget,
+                    "hbm_budget": srv.hbm_budget,
                     "refused_for_memory": st.refused_for_memory,
                     "counts_equal_batched": exact, "compared_updates": k})
         log(f"served at {load:.0f}/s offered: {out[-1]}")
