@@ -1686,10 +1686,12 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       if (++ticks == 256) {
         ticks = 0;
         if (cnt - flushed > (1ull << 20) || a.qabort || (a.lflags & LIM_CAP)) flush(false);
-        if (poll_abort()) {
-          stop = true;
-          lost = true;
-        }
+      }
+      // every iteration, as one can walk hundreds of leaf windows (the poll
+      // reads the clock, and the host word only every ABORT_POLL_TICKS)
+      if (poll_abort()) {
+        stop = true;
+        lost = true;
       }
       uint32_t cur = S.cur[L], end = S.end[L];
       if (L == L2) {

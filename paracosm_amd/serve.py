@@ -4,18 +4,18 @@ Update requests (each a small batch of edge updates) arrive over time.  The
 server keeps the graph, the indexes and the standing queries on the device
 and overlaps the stages of consecutive batches:
 
-* the admission thread (the caller's) gathers every request that has
-  arrived, in arrival order, into the next batch: up to ``max_batch``
-  updates, and only while the engine's device bytes plus the working sets of
-  the batches in flight and of this one stay inside ``hbm_budget``;
-  requests that do not fit wait for a later batch;
-* it stages the batch in a pinned slot and starts its upload on the
-  engine's transfer stream, while the device is still searching the
-  previous batch;
-* the device thread makes the engine stream wait for that upload, runs the
-  batch (graph update, index update, search), and starts the download of
-  its counts on the transfer stream, where it overlaps the next batch's
-  search;
+* one batch at a time is *open*: the admission thread (the caller's) appends
+  every request that arrives, in arrival order, to it, and starts the
+  upload of those records on the engine's transfer stream at once, while
+  the device is still searching earlier batches.  A request is appended
+  only while the open batch holds at most ``max_batch`` updates and the
+  engine's device bytes plus the working sets of every batch in flight stay
+  inside ``hbm_budget``; one that does not fit waits for a later batch;
+* the device thread closes the open batch the moment it is free (so a batch
+  holds everything that arrived while the one before it ran), makes the
+  engine stream wait for the batch's uploads, runs it (graph update, index
+  update, search), and starts the download of its counts on the transfer
+  stream, where it overlaps the next batch's search;
 * back on the admission thread, each finished batch's download is waited
   for and its requests completed.
 
@@ -95,6 +95,7 @@ class _Slot:
 
     def __init__(self, m: Matching, idx: int, max_batch: int, nq: int):
         self.idx = idx
+        self.cap = max_batch
         self.d_upd = m.device_buffer(max_batch * REC_BYTES)
         self.d_cnt = m.device_buffer(max_batch * nq * 8)
         self.d_safe = m.device_buffer(max_batch)
@@ -102,7 +103,10 @@ class _Slot:
         self.h_cnt = m.host_buffer(max_batch * nq * 8)
         self.h_safe = m.host_buffer(max_batch)
         self.mk_up, self.mk_done, self.mk_dl = 3 * idx, 3 * idx + 1, 3 * idx + 2
-        self.reqs: Sequence[Request] = ()
+        self.reset()
+
+    def reset(self) -> None:
+        self.reqs: List[Request] = []
         self.n = 0
         self.error: Optional[BaseException] = None
         self.ustatus: Optional[np.ndarray] = None
@@ -115,11 +119,12 @@ class _Slot:
 
 class Server:
     """hbm_budget: device bytes the engine plus the batches in flight may
-    hold; 0 = 90 % of the device's memory."""
+    hold; 0 = 90 % of the device's memory.  slots: batches in flight at once
+    (one open, one on the device, one downloading)."""
 
     def __init__(self, engine: Matching, *, max_batch: int = 65534, hbm_budget: int = 0,
-                 slots: int = 2):
-        assert 1 <= slots and 3 * slots <= MARKS
+                 slots: int = 3):
+        assert 2 <= slots and 3 * slots <= MARKS
         self.m = engine
         self.nq = engine.num_queries
         self.max_batch = max_batch
@@ -143,19 +148,33 @@ class Server:
                 <= self.hbm_budget)
 
     # -- the three stages
-    def _stage(self, slot: _Slot, reqs: Sequence[Request]) -> None:
+    def _append(self, slot: _Slot, reqs: Sequence[Request]) -> None:
+        """Stage requests behind the open batch's records and start their
+        upload; the upload marker is recorded again after it, so waiting on
+        it covers every record of the batch."""
         ups = np.concatenate([r.updates for r in reqs]) if len(reqs) > 1 else reqs[0].updates
-        n = len(ups)
-        nb = n * REC_BYTES
-        slot.h_upd.array[:nb] = np.ascontiguousarray(ups, UPDATE_DTYPE).view(np.uint8)
-        slot.reqs, slot.n, slot.error = reqs, n, None
-        self.m.copy_async(slot.d_upd.ptr, slot.h_upd.ptr, nb)
+        o, nb = slot.n * REC_BYTES, len(ups) * REC_BYTES
+        slot.h_upd.array[o:o + nb] = np.ascontiguousarray(ups, UPDATE_DTYPE).view(np.uint8)
+        self.m.copy_async(slot.d_upd.ptr + o, slot.h_upd.ptr + o, nb)
         self.m.mark(STREAM_TRANSFER, slot.mk_up)
+        slot.reqs.extend(reqs)
+        slot.n += len(ups)
 
-    def _device_loop(self, todo: "queue.Queue", done: "queue.Queue") -> None:
+    def _take(self, cond: threading.Condition) -> Optional[_Slot]:
+        """Close the open batch once it holds updates (None: stopping)."""
+        with cond:
+            while not self._stop and (self._open is None or self._open.n == 0):
+                cond.wait()
+            slot = self._open
+            if slot is None or slot.n == 0:
+                return None
+            self._open = None
+            return slot
+
+    def _device_loop(self, cond: threading.Condition, done: "queue.Queue") -> None:
         m = self.m
         while True:
-            slot = todo.get()
+            slot = self._take(cond)
             if slot is None:
                 done.put(None)
                 return
@@ -197,12 +216,18 @@ class Server:
     def serve(self, requests: List[Request]) -> ServeStats:
         """Serve requests in arrival order in real time (requests must be
         sorted by arrival)."""
-        todo: "queue.Queue" = queue.Queue()
+        for r in requests:
+            if len(r.updates) > self.max_batch:
+                raise ValueError(f"a request of {len(r.updates)} updates exceeds "
+                                 f"max_batch={self.max_batch}")
+        cond = threading.Condition()
         done: "queue.Queue" = queue.Queue()
         free = list(self.slots)
         self._qpartial = np.zeros(self.nq, np.int64)
-        in_flight: dict = {}       # slot idx -> updates in flight
-        dev = threading.Thread(target=self._device_loop, args=(todo, done), daemon=True)
+        self._open: Optional[_Slot] = None
+        self._stop = False
+        in_flight: dict = {}       # slot idx -> updates (open, on the device, downloading)
+        dev = threading.Thread(target=self._device_loop, args=(cond, done), daemon=True)
         t0 = time.perf_counter()
         dev.start()
         i = 0
@@ -237,37 +262,49 @@ class Server:
                     else:
                         time.sleep(wait)
                     continue
-                if not free:
-                    reap(True)
-                    continue
-                batch: List[Request] = []
-                size = 0
-                busy = sum(in_flight.values())
-                while i < n_req and requests[i].arrival <= now:
-                    k = len(requests[i].updates)
-                    if batch and size + k > self.max_batch:
-                        break
-                    if not self.fits(size + k, busy):
-                        self.refused += 1
-                        if not batch and not in_flight:
-                            raise MemoryError("a single request exceeds the HBM budget")
-                        break
-                    batch.append(requests[i])
-                    size += k
-                    i += 1
-                if not batch:
-                    reap(True)       # wait for a batch in flight to free memory
-                    continue
-                slot = free.pop()
-                overlapped += bool(in_flight)
-                self._stage(slot, batch)
-                in_flight[slot.idx] = size
-                todo.put(slot)
-                batches += 1
+                added: List[Request] = []
+                with cond:
+                    slot = self._open
+                    if slot is None and free:
+                        slot = free.pop()
+                        slot.reset()
+                        self._open = slot
+                        in_flight[slot.idx] = 0
+                        batches += 1
+                    if slot is not None:
+                        busy = sum(v for k, v in in_flight.items() if k != slot.idx)
+                        size = slot.n
+                        while i < n_req and requests[i].arrival <= now:
+                            k = len(requests[i].updates)
+                            if size + k > self.max_batch:
+                                break
+                            if not self.fits(size + k, busy):
+                                self.refused += 1
+                                if size == 0 and busy == 0:
+                                    raise MemoryError("a single request exceeds the HBM budget")
+                                break
+                            added.append(requests[i])
+                            size += k
+                            i += 1
+                        if added:
+                            self._append(slot, added)
+                            in_flight[slot.idx] = slot.n
+                            overlapped += len(in_flight) > 1
+                            cond.notify()
+                if not added:
+                    # the open batch is full, or memory is short: poll until
+                    # the device takes it or a batch finishes; no slot: wait
+                    # for a batch to finish
+                    reap(True, 0.001 if self._open is not None else 0.05)
+            with cond:
+                self._stop = True
+                cond.notify()
             while in_flight and error is None:
                 reap(True)
         finally:
-            todo.put(None)
+            with cond:
+                self._stop = True
+                cond.notify()
             dev.join()
             while True:            # complete what the device thread finished
                 try:
@@ -281,6 +318,7 @@ class Server:
                         error = error or slot.error
                     in_flight.pop(slot.idx, None)
                     free.append(slot)
+            self._open = None
         if error is not None:
             raise error
         elapsed = time.perf_counter() - t0
