@@ -65,7 +65,10 @@ typedef struct csm_opts {
                                max_num_results_ (parallel_graphflow.cpp:386,
                                2101) and keeps the count it had reached.    */
   uint32_t print_prep;      /* --print-prep                                */
-  uint32_t print_enum;      /* --print-enum (unsupported on device)        */
+  uint32_t print_enum;      /* --print-enum: keep one row per embedding
+                               each call finds (csm_embeddings); searches
+                               then take the hashing path, and batches are
+                               not replayed as HIP graphs                  */
   uint32_t homo;            /* --homo: drop the injectivity test           */
   uint32_t threads;         /* -t (accepted; device sizes itself)          */
   uint32_t auto_tuning;     /* --auto-tuning                               */
@@ -101,6 +104,8 @@ typedef struct csm_opts {
                                its search tree                             */
   uint32_t no_graphs;       /* 1: never replay steady-state batches as HIP
                                graphs (every launch issued one by one)     */
+  uint32_t enum_cap;        /* print_enum: rows kept per call (0 = 2^20);
+                               csm_embeddings reports how many were found  */
 } csm_opts;
 
 /* Creates an engine and uploads the data graph.  `algo` is one of the
@@ -207,6 +212,20 @@ int csm_update_status(const csm_engine* e, uint8_t* status, size_t n);
  * No reference counterpart (the reference's intermediate counters,
  * matching.h:28-33, are not printed). */
 int csm_update_work(const csm_engine* e, uint64_t* work, size_t n);
+
+/* The embeddings found by the last call that searched (csm_initial,
+ * csm_add_edge / csm_remove_edge, csm_apply_batch[_device]), with
+ * csm_opts.print_enum set: the output of --print-enum, which the reference
+ * prints as each search reaches it (SymBi/symbi.cpp:564-569,
+ * Parallel_GraphFlow/parallel_graphflow.cpp:367).  Row r of `rows` holds
+ * *width words: the update's index in the call (0 for csm_initial), the
+ * query id, then the data vertex matched to each query vertex 0..k-1, and
+ * 0xFFFFFFFF past the query's k.  Rows come in no particular order.  Up to
+ * cap_rows rows are copied (*n_rows); *n_found counts every embedding found,
+ * more than the rows kept once csm_opts.enum_cap is reached.  rows = NULL:
+ * only the counts and *width. */
+int csm_embeddings(const csm_engine* e, uint32_t* rows, size_t cap_rows, size_t* n_rows,
+                   uint64_t* n_found, uint32_t* width);
 
 /* Sets csm_opts.time_limit_s for the following calls (the CLI hands each
  * call what is left of the run's --time-limit).  No reference counterpart. */

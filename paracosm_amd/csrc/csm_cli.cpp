@@ -7,6 +7,7 @@
 //       [--homo 0|1] [--report-initial 0|1] [--initial-time-limit S]
 //       [-t N] [--auto-tuning 0|1]
 //       [--compat 0|1] [--batch N] [--device N] [--reference-orders 0|1]
+//       [--enum-cap N]   (rows kept per call with --print-enum; default 2^20)
 //
 // Default mode applies every update of the stream, in order, batched on the
 // device (--batch, default 65534), insertions and deletions alike, and
@@ -152,12 +153,39 @@ int die(csm_engine* e, int rc) {
 
 }  // namespace
 
+// --print-enum: the embeddings of the last call, one line each with the data
+// vertex of every query vertex followed by a space (the reference prints `m`
+// so as each search reaches it, SymBi/symbi.cpp:564-569), grouped by update
+// in stream order and sorted within an update, so that the output does not
+// depend on the device's search order.
+static void print_embeddings(csm_engine* e) {
+  size_t n = 0;
+  uint64_t found = 0;
+  uint32_t w = 0;
+  if (csm_embeddings(e, nullptr, 0, &n, &found, &w) || w == 0) return;
+  std::vector<uint32_t> rows((size_t)n * w);
+  if (n && csm_embeddings(e, rows.data(), n, &n, &found, &w)) return;
+  std::vector<size_t> idx(n);
+  for (size_t r = 0; r < n; ++r) idx[r] = r;
+  std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    return std::lexicographical_compare(&rows[a * w], &rows[a * w] + w, &rows[b * w],
+                                        &rows[b * w] + w);
+  });
+  for (size_t r : idx) {
+    const uint32_t* x = &rows[r * w];
+    for (uint32_t i = 2; i < w && x[i] != 0xFFFFFFFFu; ++i) std::cout << x[i] << " ";
+    std::cout << "\n";
+  }
+  if (found > n) std::cout << found - n << " more embeddings not kept (--enum-cap)\n";
+}
+
 int main(int argc, char** argv) {
   std::string query_path, data_path, stream_path, algorithm = "none";
   uint64_t max_results = 0;
   double time_limit = 0.0, initial_time_limit = 0.0;
   int print_prep = 1, print_enum = 0, homo = 0, report_initial = 1, compat = 0;
   uint32_t threads = 6, auto_tuning = 1, batch = 0, device = 0, reference_orders = 0;
+  uint32_t enum_cap = 0;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto next = [&]() -> std::string {
@@ -172,6 +200,7 @@ int main(int argc, char** argv) {
     else if (a == "--time-limit") time_limit = std::stod(next());
     else if (a == "--print-prep") print_prep = std::stoi(next());
     else if (a == "--print-enum") print_enum = std::stoi(next());
+    else if (a == "--enum-cap") enum_cap = (uint32_t)std::stoul(next());
     else if (a == "--homo") homo = std::stoi(next());
     else if (a == "--report-initial") report_initial = std::stoi(next());
     else if (a == "--initial-time-limit") initial_time_limit = std::stod(next());
@@ -199,6 +228,7 @@ int main(int argc, char** argv) {
   o.max_results = max_results == UINT32_MAX ? 0 : max_results;
   o.print_prep = (uint32_t)print_prep;
   o.print_enum = (uint32_t)print_enum;
+  o.enum_cap = enum_cap;
   o.homo = (uint32_t)homo;
   o.threads = threads;
   o.auto_tuning = auto_tuning;
@@ -258,6 +288,7 @@ int main(int argc, char** argv) {
     const bool timeout = rc == CSM_E_TIME_LIMIT;
     if (timeout) std::cout << "Timeout " << initial_time_limit << "s" << std::endl;
     else if (rc) return die(e, rc);
+    if (print_enum) print_embeddings(e);
     std::cout << "Initial Matching: " << ms_since(t0) << "ms" << std::endl;
     std::cout << n0 << " matches. \n";
     // main.cpp:779-790: a timed-out initial matching ends the run
@@ -312,6 +343,7 @@ int main(int argc, char** argv) {
         if (u.type == 'e' && u.is_add) {
           uint64_t p = 0;
           if ((rc = csm_add_edge(e, u.id1, u.id2, u.label, &p))) return die(e, rc);
+          if (print_enum) print_embeddings(e);
         } else if (u.type == 'v' && !u.is_add) {
           if ((rc = csm_remove_vertex(e, u.id1))) return die(e, rc);
         }
@@ -356,6 +388,7 @@ int main(int argc, char** argv) {
       } else if (rc) {
         return die(e, rc);
       }
+      if (print_enum) print_embeddings(e);
       csm_update_status(e, status.data(), m);
       for (size_t i = 0; i < m; ++i) partial += (status[i] & CSM_USTAT_TIME_LIMIT) != 0;
       for (size_t i = 0; i < m; ++i) {

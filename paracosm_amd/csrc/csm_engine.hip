@@ -35,6 +35,9 @@ namespace {
 enum Algo { ALGO_GRAPHFLOW = 0, ALGO_SYMBI = 1, ALGO_TURBOFLUX = 2, ALGO_NONE = 3 };
 
 constexpr uint32_t DEFAULT_MAX_BATCH = 65534;
+// --print-enum rows: update, query, then the data vertex of each query vertex
+constexpr uint32_t EMB_W = 2 + MAXK;
+constexpr uint64_t EMB_DEFAULT_ROWS = 1u << 20;
 constexpr size_t TUNE_MIN_BATCH = 4096;   // smallest batch the autotuner scores
 constexpr int BUILD_BS = 256;
 constexpr int APPLY_BS = 256;
@@ -196,6 +199,12 @@ struct csm_engine {
   SearchLimits* d_limits = nullptr;      // the search's SearchLimits (device copy)
   unsigned long long* h_uwork_pin = nullptr;
   std::vector<uint64_t> h_uwork;
+  // --print-enum: device rows of the current search, rows of the last call
+  uint32_t* d_emb = nullptr;
+  unsigned long long* d_emb_n = nullptr;
+  uint64_t emb_cap = 0;
+  std::vector<uint32_t> h_emb;           // EMB_W words per row
+  uint64_t emb_found = 0;
   uint32_t* h_ustatus_pin = nullptr;
 
   // steady-state batches replayed as HIP graphs (see run_batch_graph); gen
@@ -812,6 +821,10 @@ int upload_limits(csm_engine* e) {
   l.qtl = e->d_qtl;
   l.uwork = e->d_uwork;
   l.ucap = e->opts.max_results;
+  l.emb = e->d_emb;
+  l.emb_n = e->d_emb_n;
+  l.emb_cap = e->emb_cap;
+  l.emb_w = EMB_W;
   HIP_TRY(e, hipMemcpy(e->d_limits, &l, sizeof l, hipMemcpyHostToDevice));
   return CSM_OK;
 }
@@ -853,6 +866,10 @@ int ensure_results(csm_engine* e) {
   e->h_qtl.assign(e->queries.size(), 0);
   HIP_TRY(e, hipHostMalloc((void**)&e->h_counts, need * 8));
   HIP_TRY(e, hipHostMalloc((void**)&e->h_hashes, need * 8));
+  if (e->opts.print_enum && !e->d_emb) {
+    e->emb_cap = e->opts.enum_cap ? e->opts.enum_cap : EMB_DEFAULT_ROWS;
+    if ((rc = dalloc(e, &e->d_emb, e->emb_cap * EMB_W)) || (rc = dalloc(e, &e->d_emb_n, 1))) return rc;
+  }
   return upload_limits(e);
 }
 
@@ -947,7 +964,7 @@ SearchArgs search_args(csm_engine* e, unsigned long long* counts, unsigned long 
   sa.spill_chunk_cap = e->spill_chunk_cap;
   sa.budget_iters = e->budget_iters;
   sa.lflags = (e->deadline > 0 ? LIM_TIME : 0u) | (e->opts.max_results ? LIM_CAP : 0u) |
-              (e->opts.want_work ? LIM_WORK : 0u);
+              (e->opts.want_work ? LIM_WORK : 0u) | (e->d_emb ? LIM_ENUM : 0u);
   sa.lim = e->d_limits;
   return sa;
 }
@@ -1060,6 +1077,8 @@ void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long 
   a.hashes = hashes;
   a.safe = safe;
   a.ustatus = e->d_ustatus;
+  a.lim = e->d_limits;
+  a.enumerate = (hashes && e->d_emb) ? 1u : 0u;
   k_seeds<<<grid_for((uint64_t)n * WAVE, 256, 8192), 256, 0, e->stream>>>(a);
 }
 
@@ -1117,6 +1136,8 @@ void batch_clear(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned
   if (hashes)
     k_fill_u32<<<grid_for(w, 256), 256, 0, e->stream>>>(reinterpret_cast<uint32_t*>(hashes), w, 0u);
   k_fill_u32<<<grid_for(n, 256), 256, 0, e->stream>>>(e->d_ustatus, n, 0u);
+  if (e->d_emb_n)
+    k_fill_u32<<<1, 64, 0, e->stream>>>(reinterpret_cast<uint32_t*>(e->d_emb_n), 2, 0u);
   if (e->d_uwork)
     k_fill_u32<<<grid_for(2ull * n, 256), 256, 0, e->stream>>>(
         reinterpret_cast<uint32_t*>(e->d_uwork), 2ull * n, 0u);
@@ -1241,7 +1262,7 @@ constexpr uint32_t GRAPH_MAX_ROUNDS = 6;
 bool graphs_allowed(const csm_engine* e, uint32_t n) {
   static const bool off = getenv("CSM_NO_GRAPHS") != nullptr || getenv("CSM_TRACE_ROUNDS") != nullptr;
   return !off && !e->opts.no_graphs && n <= GRAPH_MAX_BATCH && e->deadline <= 0 &&
-         e->algo != ALGO_NONE;
+         e->algo != ALGO_NONE && !e->opts.print_enum;
 }
 
 // the bucket a batch of n updates is padded to
@@ -1661,6 +1682,31 @@ int symbi_add_vertex(csm_engine* e, uint32_t id) {
 // InitialMatching on the device: seeds are all directed data edges matching
 // the first edge of order 0, searched from depth 2 (graphflow.cpp:268-283
 // enumerates from single vertices; both count every embedding once).
+// --print-enum: the rows of the search just finished to the host, their
+// update indices shifted by t_offset, added to the rows already collected
+// (append) or replacing them; at most emb_cap rows are kept per call, and
+// emb_found counts every embedding.
+int collect_embeddings(csm_engine* e, uint32_t t_offset, bool append) {
+  if (!e->d_emb) return CSM_OK;
+  unsigned long long n = 0;
+  HIP_TRY(e, hipMemcpyAsync(&n, e->d_emb_n, 8, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  if (!append) {
+    e->h_emb.clear();
+    e->emb_found = 0;
+  }
+  e->emb_found += n;
+  const uint64_t have = e->h_emb.size() / EMB_W;
+  const uint64_t keep = std::min<uint64_t>(std::min<uint64_t>(n, e->emb_cap),
+                                           e->emb_cap > have ? e->emb_cap - have : 0);
+  if (keep == 0) return CSM_OK;
+  const size_t old = e->h_emb.size();
+  e->h_emb.resize(old + keep * EMB_W);
+  HIP_TRY(e, hipMemcpy(e->h_emb.data() + old, e->d_emb, keep * EMB_W * 4, hipMemcpyDeviceToHost));
+  for (uint64_t r = 0; r < keep; ++r) e->h_emb[old + r * EMB_W] += t_offset;
+  return CSM_OK;
+}
+
 int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
   const PlanRec& P = e->h_plans[q.plan_base];
   if (P.k == 0) { *total = 0; return CSM_OK; }
@@ -1671,6 +1717,12 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
   HIP_TRY(e, hipMemsetAsync(e->d_counts, 0, nq * 8, e->stream));
   k_fill_u32<<<1, 64, 0, e->stream>>>(e->d_ustatus, 1, 0u);
   k_fill_u32<<<1, 256, 0, e->stream>>>(e->d_qtl, nq, 0u);
+  // --print-enum: the rows come from the hashing search
+  unsigned long long* hashes = e->d_emb ? e->d_hashes : nullptr;
+  if (e->d_emb) {
+    k_fill_u32<<<1, 256, 0, e->stream>>>(reinterpret_cast<uint32_t*>(e->d_hashes), 2 * nq, 0u);
+    k_fill_u32<<<1, 64, 0, e->stream>>>(reinterpret_cast<uint32_t*>(e->d_emb_n), 2, 0u);
+  }
   uint32_t v0 = 0, step = e->nv;
   while (v0 < e->nv) {
     uint32_t v1 = (uint32_t)std::min<uint64_t>((uint64_t)v0 + step, e->nv);
@@ -1679,7 +1731,7 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
     k_initial_seeds<<<grid_for(v1 - v0, 256 / WAVE, 8192), 256, 0, e->stream>>>(
         g, e->d_plans, q.plan_base, la, key01, v0, v1, e->opts.split_chunk, e->d_seeds,
         e->seed_cap, e->d_chunk_owner, e->chunk_cap, e->d_ctrl, e->d_d2, e->d2_words,
-        (uses_index(e) && e->d_d2) ? 1u : 0u, e->d_counts);
+        (uses_index(e) && e->d_d2) ? 1u : 0u, e->d_counts, e->d_emb ? e->d_limits : nullptr);
     int rc = read_ctrl(e);
     if (rc) return rc;
     if (e->h_ctrl->err & (ERR_SEED_OVERFLOW | ERR_CHUNK_OVERFLOW)) {
@@ -1688,9 +1740,10 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
       continue;
     }
     // search with the end-of-batch view (position 0xFFFF)
-    if ((rc = search(e, true, e->d_counts, nullptr, 0xFFFEu))) return rc;
+    if ((rc = search(e, true, e->d_counts, hashes, 0xFFFEu))) return rc;
     v0 = v1;
   }
+  if (int rc = collect_embeddings(e, 0, false)) return rc;
   if (e->opts.max_results)
     k_clamp_counts<<<1, 64, 0, e->stream>>>(e->d_counts, 1, (uint32_t)nq, e->opts.max_results,
                                             e->d_ustatus);
@@ -2231,7 +2284,19 @@ static int initial_impl(csm_engine* e, uint64_t* num_results) {
     return CSM_OK;
   }
   if (q.g.ne == 0) {
-    for (uint32_t v = 0; v < e->nv; ++v) total += (e->h_vlab[v] != NO_LABEL && e->h_vlab[v] == q.dlab[0]);
+    e->h_emb.clear();
+    for (uint32_t v = 0; v < e->nv; ++v) {
+      const bool hit = e->h_vlab[v] != NO_LABEL && e->h_vlab[v] == q.dlab[0];
+      total += hit;
+      if (hit && e->d_emb && e->h_emb.size() / EMB_W < e->emb_cap) {
+        e->h_emb.resize(e->h_emb.size() + EMB_W, NONE);
+        uint32_t* row = e->h_emb.data() + e->h_emb.size() - EMB_W;
+        row[0] = 0;
+        row[1] = 0;
+        row[2] = v;
+      }
+    }
+    e->emb_found = e->d_emb ? total : 0;
     *num_results = total;
     e->tot_init = total;
     return CSM_OK;
@@ -2308,14 +2373,24 @@ int csm_remove_vertex(csm_engine* e, uint32_t id) {
   return remove_vertex(e, id);
 }
 
+// emb_offset / emb_append: embedding rows (--print-enum) of this batch are
+// numbered from emb_offset and added to the rows already collected by the
+// same host call (its earlier slices), or replace the last call's rows.
 static int apply_batch_device_impl(csm_engine* e, const csm_update* d_updates, size_t n,
-                                   uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash) {
+                                   uint64_t* d_counts, uint8_t* d_safe, uint64_t* d_hash,
+                                   uint32_t emb_offset = 0, bool emb_append = false) {
   if (!e || (n && (!d_updates || !d_counts))) return CSM_E_ARG;
   if (n > e->max_batch) return fail(e, CSM_E_CAPACITY, "batch of %zu > max_batch %u", n, e->max_batch);
   if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
   if (d_hash && !e->opts.want_hash) d_hash = nullptr;
+  // --print-enum: the rows come from the hashing search
+  if (!d_hash && e->d_emb) d_hash = reinterpret_cast<uint64_t*>(e->d_hashes);
   e->h_ustatus.assign(n, 0);
   e->h_uwork.assign(n, 0);
+  if (!emb_append) {
+    e->h_emb.clear();
+    e->emb_found = 0;
+  }
   if (n == 0) return CSM_OK;
   HIP_TRY(e, hipSetDevice(e->dev));
   int rc = 1;
@@ -2344,6 +2419,7 @@ static int apply_batch_device_impl(csm_engine* e, const csm_update* d_updates, s
   e->h_uwork.assign(e->h_uwork_pin ? e->h_uwork_pin : nullptr,
                     e->h_uwork_pin ? e->h_uwork_pin + n : nullptr);
   accumulate_stats(e);
+  if ((rc = collect_embeddings(e, emb_offset, true))) return rc;
   return check_errors(e);
 }
 
@@ -2364,6 +2440,29 @@ int csm_update_work(const csm_engine* e, uint64_t* work, size_t n) {
   if (!e->opts.want_work) return fail(const_cast<csm_engine*>(e), CSM_E_UNSUPPORTED,
                                       "per-update work needs csm_opts.want_work");
   for (size_t i = 0; i < n; ++i) work[i] = i < e->h_uwork.size() ? e->h_uwork[i] : 0;
+  return CSM_OK;
+}
+
+int csm_embeddings(const csm_engine* e, uint32_t* rows, size_t cap_rows, size_t* n_rows,
+                   uint64_t* n_found, uint32_t* width) {
+  if (!e || (cap_rows && !rows)) return CSM_E_ARG;
+  if (!e->opts.print_enum)
+    return fail(const_cast<csm_engine*>(e), CSM_E_UNSUPPORTED,
+                "embedding rows need csm_opts.print_enum");
+  const uint32_t w = 2 + e->max_k;
+  const size_t have = e->h_emb.size() / EMB_W;
+  const size_t n = rows ? std::min(have, cap_rows) : have;
+  for (size_t r = 0; r < n && rows; ++r) {
+    const uint32_t* src = e->h_emb.data() + r * EMB_W;
+    uint32_t* dst = rows + r * w;
+    dst[0] = src[0];
+    dst[1] = src[1];
+    const uint32_t k = src[1] < e->queries.size() ? e->queries[src[1]]->g.k : 0u;
+    for (uint32_t i = 0; i < e->max_k; ++i) dst[2 + i] = i < k ? src[2 + i] : NONE;
+  }
+  if (n_rows) *n_rows = n;
+  if (n_found) *n_found = e->emb_found;
+  if (width) *width = w;
   return CSM_OK;
 }
 
@@ -2395,6 +2494,8 @@ static int apply_batch_impl(csm_engine* e, const csm_update* updates, size_t n, 
   const size_t nq = e->queries.size();
   size_t done = 0;
   status.assign(n, 0);
+  e->h_emb.clear();       // the slices below add their embedding rows
+  e->emb_found = 0;
   while (done < n) {
     // a vertex deletion is applied on its own, in stream order
     if (updates[done].type == 'v' && !updates[done].is_add) {
@@ -2427,9 +2528,10 @@ static int apply_batch_impl(csm_engine* e, const csm_update* updates, size_t n, 
     HIP_TRY(e, hipMemcpyAsync(e->d_upd, e->h_upd, m * sizeof(UpdateRec), hipMemcpyHostToDevice,
                               e->stream));
     bool hash = emb_hash && e->opts.want_hash;
-    int rc = csm_apply_batch_device(e, reinterpret_cast<const csm_update*>(e->d_upd), m,
-                                    reinterpret_cast<uint64_t*>(e->d_counts), e->d_safe,
-                                    hash ? reinterpret_cast<uint64_t*>(e->d_hashes) : nullptr);
+    int rc = apply_batch_device_impl(e, reinterpret_cast<const csm_update*>(e->d_upd), m,
+                                     reinterpret_cast<uint64_t*>(e->d_counts), e->d_safe,
+                                     hash ? reinterpret_cast<uint64_t*>(e->d_hashes) : nullptr,
+                                     (uint32_t)done, true);
     if (rc) return rc;
     for (size_t i = 0; i < m && i < e->h_ustatus.size(); ++i) status[done + i] = e->h_ustatus[i];
     HIP_TRY(e, hipMemcpyAsync(e->h_counts, e->d_counts, m * nq * 8, hipMemcpyDeviceToHost, e->stream));

@@ -113,8 +113,16 @@ struct SearchLimits {
   uint32_t* qtl;              // per query: 1 once the time limit dropped some of its work
   unsigned long long* uwork;  // per update: candidate entries tested
   unsigned long long ucap;    // per (update, query) result cap
+  // embedding rows (--print-enum): row r at emb + r * emb_w holds the
+  // update, the query and the data vertex of each query vertex; emb_n counts
+  // every embedding found, rows past emb_cap are not written
+  uint32_t* emb;
+  unsigned long long* emb_n;
+  unsigned long long emb_cap;
+  uint32_t emb_w;
 };
-constexpr uint32_t LIM_TIME = 1u, LIM_CAP = 2u, LIM_WORK = 4u;
+constexpr uint32_t LIM_TIME = 1u, LIM_CAP = 2u, LIM_WORK = 4u, LIM_ENUM = 8u;
+
 
 // A search task: update t, plan qo, the first `depth` positions fixed in m[],
 // and the candidate range of position `depth` (absolute pool offsets into the
@@ -699,6 +707,26 @@ constexpr uint32_t USTAT_CAPPED = 2u;      // a count reached the per-update res
 // for every seed, some query neighbour of a seed endpoint has no candidate
 constexpr uint32_t USTAT_CERTIFIED = 4u;
 
+// Reserve one row per lane with `ok` set (one atomic per wave) and write
+// each lane's embedding: vertex_of(i) is the data vertex of query vertex i.
+template <class F>
+__device__ __forceinline__ void emit_rows(const SearchLimits* lim, bool ok, uint32_t t,
+                                          uint32_t q, uint32_t k, F vertex_of) {
+  const uint32_t lane = lane_id();
+  const uint64_t m = ballot(ok);
+  if (m == 0) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(lim->emb_n, (unsigned long long)popc64(m));
+  base = __shfl(base, 0);
+  if (!ok) return;
+  const unsigned long long r = base + popc64(m & ((1ull << lane) - 1ull));
+  if (r >= lim->emb_cap) return;
+  uint32_t* row = lim->emb + r * lim->emb_w;
+  row[0] = t;
+  row[1] = q;
+  for (uint32_t i = 0; i < k; ++i) row[2 + i] = vertex_of(i);
+}
+
 struct SeedArgs {
   GraphView g;
   BatchView b;
@@ -723,6 +751,8 @@ struct SeedArgs {
   unsigned long long* hashes;
   uint8_t* safe;
   uint32_t* ustatus;          // USTAT_CERTIFIED for updates no seed survives (nullable)
+  const SearchLimits* lim;    // embedding rows of 2-vertex queries (enumerate != 0)
+  uint32_t enumerate;
 };
 
 // Emit the depth-2 seeds of the lanes with `have` set (one task each, cut into
@@ -828,6 +858,16 @@ __global__ void k_seeds(SeedArgs a) {
             mq[P->qv[1]] = m1;
             uint64_t h = mix64(mix64((uint64_t)mq[0] + 1) + (uint64_t)mq[1] + 1);
             atomicAdd(&a.hashes[(size_t)i * a.nq + P->q], (unsigned long long)h);
+          }
+          if (a.enumerate) {
+            const unsigned long long r = atomicAdd(a.lim->emb_n, 1ull);
+            if (r < a.lim->emb_cap) {
+              uint32_t* row = a.lim->emb + r * a.lim->emb_w;
+              row[0] = i;
+              row[1] = P->q;
+              row[2 + P->qv[0]] = m0;
+              row[2 + P->qv[1]] = m1;
+            }
           }
           ok = false;
         }
@@ -1597,6 +1637,10 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
             }
           }
           hsum += wave_sum(h);
+          if (a.lflags & LIM_ENUM)
+            emit_rows(a.lim, ok, t, q, k, [&](uint32_t i) {
+              return (i == S.pqv[L1]) ? w : (i == S.pqv[L2]) ? S.cand[L2][p] : S.mq[i];
+            });
         }
       }
       if (jp == NONE) n_bytes += 4ull * bn * S.lfsize;
@@ -1679,6 +1723,9 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
                 h = mix64(h + (uint64_t)mv + 1);
               }
             hsum += wave_sum(h);
+            if (a.lflags & LIM_ENUM)
+              emit_rows(a.lim, ok, t, q, k,
+                        [&](uint32_t i) { return (i == S.pqv[L]) ? v : S.mq[i]; });
           }
         }
         break;
@@ -1884,7 +1931,7 @@ __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, 
                                 uint32_t chunk, Task* seeds, uint32_t seed_cap,
                                 uint32_t* chunk_owner, uint32_t chunk_cap, Ctrl* ctrl,
                                 const uint32_t* __restrict__ d2, uint32_t d2_words, uint32_t symbi,
-                                unsigned long long* counts) {
+                                unsigned long long* counts, const SearchLimits* emb_lim) {
   const uint32_t lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
@@ -1910,6 +1957,8 @@ __global__ void k_initial_seeds(GraphView g, const PlanRec* __restrict__ plans, 
       if (k == 2) {
         const uint32_t n2 = popc64(ballot(ok));
         if (lane == 0 && n2) atomicAdd(&counts[P->q], (unsigned long long)n2);
+        if (emb_lim)
+          emit_rows(emb_lim, ok, 0u, P->q, 2u, [&](uint32_t u) { return u == P->qv[0] ? v : w; });
         continue;
       }
       if (ok) {

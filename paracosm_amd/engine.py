@@ -62,7 +62,7 @@ class _Opts(ctypes.Structure):
                 ("split_chunk", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("match_budget", ctypes.c_uint64), ("spill_budget", ctypes.c_uint32),
                 ("reference_orders", ctypes.c_uint32), ("want_work", ctypes.c_uint32),
-                ("no_graphs", ctypes.c_uint32)]
+                ("no_graphs", ctypes.c_uint32), ("enum_cap", ctypes.c_uint32)]
 
 
 UPDATE_DTYPE = np.dtype([("type", np.uint8), ("is_add", np.uint8), ("pad", np.uint16),
@@ -103,6 +103,7 @@ def lib():
             "csm_update_status": (ctypes.c_int, [vp, vp, sz]),
             "csm_set_time_limit": (ctypes.c_int, [vp, ctypes.c_double]),
             "csm_update_work": (ctypes.c_int, [vp, vp, sz]),
+            "csm_embeddings": (ctypes.c_int, [vp, vp, sz, P(sz), P(u64), P(u32)]),
             "csm_totals": (ctypes.c_int, [vp, P(u64), P(u64), P(u64)]),
             "csm_memory": (ctypes.c_int, [vp, P(sz), P(sz)]),
             "csm_device_bytes": (ctypes.c_int, [vp, P(u64)]),
@@ -144,7 +145,8 @@ def exported_symbols() -> Sequence[str]:
             "csm_initial", "csm_classify", "csm_add_edge", "csm_remove_edge",
             "csm_add_vertex", "csm_remove_vertex", "csm_apply_batch",
             "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_update_status",
-            "csm_set_time_limit", "csm_update_work", "csm_totals", "csm_memory",
+            "csm_set_time_limit", "csm_update_work", "csm_embeddings", "csm_totals",
+            "csm_memory",
             "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
             "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
             "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_copy_async", "csm_mark",
@@ -185,14 +187,14 @@ def _opt_values(max_results: int = 0, print_prep: bool = False, print_enum: bool
                 want_hash: bool = False, device_budget: int = 0, split_chunk: int = 0,
                 device: int = 0, match_budget: int = 0, spill_budget: int = 0,
                 reference_orders: bool = False, time_limit_s: float = 0.0,
-                want_work: bool = False, no_graphs: bool = False) -> dict:
+                want_work: bool = False, no_graphs: bool = False, enum_cap: int = 0) -> dict:
     """csm_opts fields from Matching's keyword arguments."""
     return dict(max_results=max_results, print_prep=int(print_prep), print_enum=int(print_enum),
                 homo=int(homo), threads=threads, auto_tuning=auto_tuning, max_batch=max_batch,
                 want_hash=int(want_hash), time_limit_s=time_limit_s, device_budget=device_budget,
                 split_chunk=split_chunk, device=device, match_budget=match_budget,
                 spill_budget=spill_budget, reference_orders=int(reference_orders),
-                want_work=int(want_work), no_graphs=int(no_graphs))
+                want_work=int(want_work), no_graphs=int(no_graphs), enum_cap=enum_cap)
 
 
 class Matching:
@@ -236,14 +238,14 @@ class Matching:
                  split_chunk: int = 0, device: int = 0, match_budget: int = 0,
                  spill_budget: int = 0, reference_orders: bool = False,
                  time_limit_s: float = 0.0, want_work: bool = False,
-                 no_graphs: bool = False):
+                 no_graphs: bool = False, enum_cap: int = 0):
         L = lib()
         self._q = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         d = _GraphArgs(vlabels, src, dst, elabels)
         o = _Opts(max_results, int(print_prep), int(print_enum), int(homo), threads,
                   auto_tuning, max_batch, int(want_hash), time_limit_s, device_budget, split_chunk,
                   device, match_budget, spill_budget, int(reference_orders), int(want_work),
-                  int(no_graphs))
+                  int(no_graphs), enum_cap)
         err = ctypes.c_int(0)
         self.h = L.csm_create(algo.encode(), ctypes.byref(self._q.g), ctypes.byref(d.g),
                               ctypes.byref(o), ctypes.byref(err))
@@ -350,6 +352,19 @@ class Matching:
         out = np.zeros(max(1, n), np.uint64)
         self._check(lib().csm_update_work(self.h, out.ctypes.data, n))
         return out[:n]
+
+    def embeddings(self):
+        """(rows, n_found) of the last searching call (print_enum=True):
+        rows is [n, 2 + max k] uint32 -- update index, query id, then the data
+        vertex of each query vertex (0xFFFFFFFF past the query's size)."""
+        L = lib()
+        n, found, w = ctypes.c_size_t(0), ctypes.c_uint64(0), ctypes.c_uint32(0)
+        self._check(L.csm_embeddings(self.h, None, 0, ctypes.byref(n), ctypes.byref(found),
+                                     ctypes.byref(w)))
+        rows = np.zeros((max(1, n.value), w.value), np.uint32)
+        self._check(L.csm_embeddings(self.h, rows.ctypes.data, n.value, ctypes.byref(n),
+                                     ctypes.byref(found), ctypes.byref(w)))
+        return rows[:n.value], int(found.value)
 
     def set_time_limit(self, seconds: float) -> None:
         self._check(lib().csm_set_time_limit(self.h, float(seconds)))
