@@ -252,9 +252,12 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes);
  * entry scanned plus 4 B per entry of every partner segment, merge model),
  * [3] batches, [4] pool repacks, [5] search-kernel time in ns (HIP events on
  * the engine stream), [6] search launches timed, [7] most search rounds one
- * batch needed, [8] split_chunk and [9] spill budget in use, [10] 1 while
- * the autotuner (auto_tuning != 0 with neither knob set) is still trying
- * candidates on large batches, [11] batches replayed as HIP graphs. */
+ * batch needed, [8] split_chunk and [9] spill budget in use for the first
+ * registered query's size class, [10] 1 while the autotuner (auto_tuning
+ * != 0 with neither knob set) is still trying candidates on large batches,
+ * [11] batches replayed as HIP graphs, [12 + 2c] split_chunk and [13 + 2c]
+ * spill budget of size class c: queries of up to 8, 12, 16 and 32 vertices
+ * (c = 0..3); the autotuner picks each class's pair separately. */
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
 
 /* Diagnostic read-back of the candidate index of one query (parallel_symbi

@@ -47,6 +47,10 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// Query size classes of the search settings: up to 8, 12, 16 and 32 vertices.
+constexpr int NSIZE_CLASSES = 4;
+inline int size_class(uint32_t k) { return k <= 8 ? 0 : k <= 12 ? 1 : k <= 16 ? 2 : 3; }
+
 struct QueryInfo {
   QueryGraph g;                         // original labels
   std::vector<uint32_t> dlab;           // dense vertex labels
@@ -121,14 +125,18 @@ struct csm_engine {
   Task* d_spill[2] = {nullptr, nullptr};          // ping-pong task lists of later rounds
   uint32_t* d_spill_owner[2] = {nullptr, nullptr};
   uint32_t spill_cap = 0, spill_chunk_cap = 0;
-  uint32_t budget_iters = 1024;
-  // online autotuner of (split_chunk, spill budget) (the reference's
-  // --auto-tuning): after one settling batch, the next large batch runs its
-  // search once per candidate; the least search time is kept
+  // search settings per query size class (size_class): candidates per
+  // chunk when a range is cut, and the spill budget; every plan carries its
+  // class's pair (PlanRec::chunk, PlanRec::budget)
+  uint32_t cls_chunk[NSIZE_CLASSES], cls_budget[NSIZE_CLASSES];
+  // online autotuner of those settings per size class (the reference's
+  // --auto-tuning): after one settling batch, the next large batch runs the
+  // search of each class's queries once per candidate; the least search
+  // time is kept for that class
   bool tune_active = false;
   uint32_t tune_idx = 0;
   std::vector<std::pair<uint32_t, uint32_t>> tune_cands;
-  std::vector<double> tune_cost;
+  bool tune_mask = false;   // the tuning search skips the queries of other classes
   uint32_t max_rounds = 0;                        // stats: deepest round count seen
   unsigned long long* d_counts = nullptr;
   unsigned long long* d_hashes = nullptr;
@@ -211,7 +219,7 @@ struct csm_engine {
   // counts device allocations, whose addresses a captured graph bakes in
   uint64_t gen = 0;
   struct GraphKey {
-    uint32_t n, nq, dcur, cur, hash, nv, chunk, budget, budgeted, capped;
+    uint32_t n, nq, dcur, cur, hash, nv, budgeted, capped;
     uint64_t gen;
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof *this) == 0; }
   };
@@ -470,7 +478,20 @@ void build_seed_table(csm_engine* e) {
   e->h_tab_off[nb] = (uint32_t)e->h_tab_ent.size();
 }
 
+// Every plan takes the search settings of its query's size class.
+void apply_plan_settings(csm_engine* e) {
+  for (auto& qp : e->queries) {
+    const int c = size_class(qp->g.k);
+    for (size_t oi = 0; oi < qp->orders.size(); ++oi) {
+      PlanRec& P = e->h_plans[qp->plan_base + oi];
+      P.chunk = (uint16_t)std::min<uint32_t>(e->cls_chunk[c], 0xFFFFu);
+      P.budget = (uint16_t)std::min<uint32_t>(e->cls_budget[c], 0xFFFFu);
+    }
+  }
+}
+
 int upload_plans(csm_engine* e) {
+  apply_plan_settings(e);
   if (e->h_plans.size() > e->d_plans_cap) {
     dfree(e, e->d_plans);
     size_t cap = std::max<size_t>(64, e->h_plans.size() * 2);
@@ -954,15 +975,13 @@ SearchArgs search_args(csm_engine* e, unsigned long long* counts, unsigned long 
   sa.d2 = e->d_d2;
   sa.d2_words = e->d2_words;
   sa.nq = (uint32_t)e->queries.size();
-  sa.chunk = e->opts.split_chunk;
   sa.homo = e->opts.homo;
   sa.pos_offset = pos_offset;
   sa.qcount = e->opts.match_budget ? e->d_qcount : nullptr;
-  sa.qabort = e->opts.match_budget ? e->d_qabort : nullptr;
+  sa.qabort = (e->opts.match_budget || e->tune_mask) ? e->d_qabort : nullptr;
   sa.budget = e->opts.match_budget;
   sa.spill_cap = e->spill_cap;
   sa.spill_chunk_cap = e->spill_chunk_cap;
-  sa.budget_iters = e->budget_iters;
   sa.lflags = (e->deadline > 0 ? LIM_TIME : 0u) | (e->opts.max_results ? LIM_CAP : 0u) |
               (e->opts.want_work ? LIM_WORK : 0u) | (e->d_emb ? LIM_ENUM : 0u);
   sa.lim = e->d_limits;
@@ -1064,7 +1083,6 @@ void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long 
   a.d2_words = e->d2_words;
   a.nlab = e->nlab;
   a.nq = (uint32_t)e->queries.size();
-  a.chunk = e->opts.split_chunk;
   a.versioned = versioned;
   a.homo = e->opts.homo;
   a.want_hash = hashes != nullptr;
@@ -1111,7 +1129,8 @@ int symbi_add_vertex(csm_engine* e, uint32_t id);
 // d_type/d_add/d_v1/d_v2/d_lab.  With search == false the graph and the SymBi
 // index are updated but no match is searched (vertex deletion).
 int dir_reserve(csm_engine* e, uint32_t n);
-void tune_finish(csm_engine* e);
+int tune_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
+               uint8_t* safe);
 
 // Host decisions before a batch: room in the adjacency pool (repack) and in
 // the key-directory pool.
@@ -1215,21 +1234,7 @@ int run_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned lo
   // 5. seeds + search
   if (search_matches) {
     if (e->tune_active && n >= TUNE_MIN_BATCH && e->tune_idx > 0) {
-      // autotuning: the search alone runs once per candidate setting on this
-      // same batch (identical work and identical counts each time)
-      for (size_t i = 0; i < e->tune_cands.size(); ++i) {
-        if (i) {
-          k_search_reset<<<1, 64, 0, e->stream>>>(e->d_ctrl);
-          batch_clear(e, n, counts, hashes);
-        }
-        e->opts.split_chunk = e->tune_cands[i].first;
-        e->budget_iters = e->tune_cands[i].second;
-        const uint64_t ns0 = e->stats[5];
-        seeds(e, n, counts, hashes, safe, true);
-        if ((rc = search(e, true, counts, hashes))) return rc;
-        e->tune_cost.push_back((double)(e->stats[5] - ns0));
-      }
-      tune_finish(e);
+      if ((rc = tune_batch(e, n, counts, hashes, safe))) return rc;
     } else {
       seeds(e, n, counts, hashes, safe, true);
       if ((rc = search(e, true, counts, hashes))) return rc;
@@ -1281,8 +1286,6 @@ csm_engine::GraphKey graph_key(const csm_engine* e, uint32_t n, bool hash) {
   k.cur = (uint32_t)e->cur;
   k.hash = hash ? 1u : 0u;
   k.nv = e->nv;
-  k.chunk = e->opts.split_chunk;
-  k.budget = e->budget_iters;
   k.budgeted = e->opts.match_budget ? 1u : 0u;
   k.capped = e->opts.max_results ? 1u : 0u;
   k.gen = e->gen;
@@ -1729,7 +1732,7 @@ int initial_matching(csm_engine* e, QueryInfo& q, uint64_t* total) {
     k_batch_begin<<<1, 64, 0, e->stream>>>(e->d_ctrl);
     GraphView g = view(e);
     k_initial_seeds<<<grid_for(v1 - v0, 256 / WAVE, 8192), 256, 0, e->stream>>>(
-        g, e->d_plans, q.plan_base, la, key01, v0, v1, e->opts.split_chunk, e->d_seeds,
+        g, e->d_plans, q.plan_base, la, key01, v0, v1, P.chunk, e->d_seeds,
         e->seed_cap, e->d_chunk_owner, e->chunk_cap, e->d_ctrl, e->d_d2, e->d2_words,
         (uses_index(e) && e->d_d2) ? 1u : 0u, e->d_counts, e->d_emb ? e->d_limits : nullptr);
     int rc = read_ctrl(e);
@@ -2018,22 +2021,80 @@ int parse_static(csm_engine* e, int mode, const uint8_t* d_text, uint64_t len,
   return CSM_OK;
 }
 
-// After the tuning batch: keep the candidate with the least search time for
-// the engine's lifetime.
-void tune_finish(csm_engine* e) {
-  size_t best = 0;
-  for (size_t i = 1; i < e->tune_cost.size(); ++i)
-    if (e->tune_cost[i] < e->tune_cost[best]) best = i;
-  e->opts.split_chunk = e->tune_cands[best].first;
-  e->budget_iters = e->tune_cands[best].second;
-  e->tune_active = false;
-  if (getenv("CSM_TRACE_ROUNDS")) {
-    fprintf(stderr, "[csm] autotune:");
-    for (size_t i = 0; i < e->tune_cost.size(); ++i)
-      fprintf(stderr, " (%u,%u)=%.2fms", e->tune_cands[i].first, e->tune_cands[i].second,
-              e->tune_cost[i] * 1e-6);
-    fprintf(stderr, " -> split %u, budget %u\n", e->opts.split_chunk, e->budget_iters);
+// Autotuning on one batch whose graph update has run.  For each size class
+// with registered queries, the search of that class's queries (the others'
+// chunks are skipped) runs once per candidate setting: the same work each
+// time, so the least search time picks the class's setting for the engine's
+// lifetime.  Then the whole search runs with the chosen settings (with a
+// single class the last candidate's search has counted everything already:
+// the counts do not depend on the settings).
+int tune_batch(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
+               uint8_t* safe) {
+  int rc;
+  std::vector<int> classes;
+  for (auto& qp : e->queries) {
+    const int c = size_class(qp->g.k);
+    if (std::find(classes.begin(), classes.end(), c) == classes.end()) classes.push_back(c);
   }
+  std::sort(classes.begin(), classes.end());
+  const size_t nq = e->queries.size();
+  // skip flags of the other classes' queries (the per-query flags a match
+  // budget uses; with a budget set, the classes are tuned together)
+  const bool mask = classes.size() > 1 && !e->opts.match_budget;
+  static const bool trace = getenv("CSM_TRACE_ROUNDS") != nullptr;
+  bool first = true;
+  for (size_t ci = 0; ci < (mask ? classes.size() : 1); ++ci) {
+    if (mask) {
+      std::vector<uint32_t> skip(nq);
+      for (size_t q = 0; q < nq; ++q) skip[q] = size_class(e->queries[q]->g.k) != classes[ci];
+      HIP_TRY(e, hipMemcpyAsync(e->d_qabort, skip.data(), nq * 4, hipMemcpyHostToDevice, e->stream));
+      HIP_TRY(e, hipStreamSynchronize(e->stream));
+      e->tune_mask = true;
+    }
+    size_t best = 0;
+    double best_ns = 0;
+    if (trace) fprintf(stderr, "[csm] autotune, queries of up to %s vertices:",
+                       mask ? (classes[ci] == 0 ? "8" : classes[ci] == 1 ? "12" : classes[ci] == 2 ? "16" : "32")
+                            : "32");
+    for (size_t i = 0; i < e->tune_cands.size(); ++i) {
+      if (!first) {
+        k_search_reset<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+        batch_clear(e, n, counts, hashes);
+      }
+      first = false;
+      for (int c : classes)
+        if (!mask || c == classes[ci]) {
+          e->cls_chunk[c] = e->tune_cands[i].first;
+          e->cls_budget[c] = e->tune_cands[i].second;
+        }
+      if ((rc = upload_plans(e))) return rc;
+      const uint64_t ns0 = e->stats[5];
+      seeds(e, n, counts, hashes, safe, true);
+      if ((rc = search(e, true, counts, hashes))) return rc;
+      const double ns = (double)(e->stats[5] - ns0);
+      if (i == 0 || ns < best_ns) { best = i; best_ns = ns; }
+      if (trace) fprintf(stderr, " (%u,%u)=%.2fms", e->tune_cands[i].first, e->tune_cands[i].second,
+                         ns * 1e-6);
+    }
+    for (int c : classes)
+      if (!mask || c == classes[ci]) {
+        e->cls_chunk[c] = e->tune_cands[best].first;
+        e->cls_budget[c] = e->tune_cands[best].second;
+      }
+    if (trace) fprintf(stderr, " -> split %u, budget %u\n", e->tune_cands[best].first,
+                       e->tune_cands[best].second);
+  }
+  e->tune_active = false;
+  if ((rc = upload_plans(e))) return rc;
+  if (mask) {
+    e->tune_mask = false;
+    k_fill_u32<<<1, 256, 0, e->stream>>>(e->d_qabort, nq, 0u);
+    k_search_reset<<<1, 64, 0, e->stream>>>(e->d_ctrl);
+    batch_clear(e, n, counts, hashes);
+    seeds(e, n, counts, hashes, safe, true);
+    if ((rc = search(e, true, counts, hashes))) return rc;
+  }
+  return CSM_OK;
 }
 
 }  // namespace
@@ -2084,9 +2145,11 @@ static csm_engine* create_engine(const char* algo, const csm_graph* query, const
   e->tune_active = e->opts.auto_tuning != 0 && e->opts.split_chunk == 0 && e->opts.spill_budget == 0;
   if (e->tune_active)
     e->tune_cands = {{64, 1024}, {32, 1024}, {64, 512}, {64, 2048}, {128, 1024}};
-  if (e->opts.split_chunk == 0) e->opts.split_chunk = 64;
+  for (int c = 0; c < NSIZE_CLASSES; ++c) {
+    e->cls_chunk[c] = e->opts.split_chunk ? e->opts.split_chunk : 64;
+    e->cls_budget[c] = e->opts.spill_budget ? e->opts.spill_budget : 1024;
+  }
   e->max_batch = e->opts.max_batch;
-  if (e->opts.spill_budget) e->budget_iters = e->opts.spill_budget;
   e->dev = (int)e->opts.device;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= e->dev) { set(CSM_E_DEVICE); return nullptr; }
@@ -2591,10 +2654,16 @@ int csm_device_bytes(const csm_engine* e, uint64_t* bytes) {
 int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
   if (!e || !out) return CSM_E_ARG;
   for (size_t i = 0; i < n_out && i < 8; ++i) out[i] = e->stats[i];
-  if (n_out > 8) out[8] = e->opts.split_chunk;
-  if (n_out > 9) out[9] = e->budget_iters;
+  // [8], [9]: the settings of the first registered query's size class
+  const int c0 = e->queries.empty() ? 0 : size_class(e->queries[0]->g.k);
+  if (n_out > 8) out[8] = e->cls_chunk[c0];
+  if (n_out > 9) out[9] = e->cls_budget[c0];
   if (n_out > 10) out[10] = e->tune_active ? 1 : 0;
   if (n_out > 11) out[11] = e->graph_launches;
+  for (int c = 0; c < NSIZE_CLASSES; ++c) {
+    if (n_out > 12 + 2 * (size_t)c) out[12 + 2 * c] = e->cls_chunk[c];
+    if (n_out > 13 + 2 * (size_t)c) out[13 + 2 * c] = e->cls_budget[c];
+  }
   return CSM_OK;
 }
 

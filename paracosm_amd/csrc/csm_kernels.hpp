@@ -96,7 +96,10 @@ struct PlanRec {
   uint32_t k;
   uint32_t q;                    // query id (result column)
   uint32_t symbi;                // 1: filter candidates by the d2 bitmap
-  uint32_t pad;
+  // search settings of this plan's query size (the autotuner's choice):
+  // candidates per chunk when a range is cut, and the work units a task
+  // may use before it spills
+  uint16_t chunk, budget;
   uint32_t qv[MAXK];             // query vertex at position p
   uint32_t nb[MAXK];             // backward neighbours of position p
   uint32_t bw[MAXK][MAXNB];      // (backward position) | (segment key << 8)
@@ -738,7 +741,6 @@ struct SeedArgs {
   uint32_t d2_words;
   uint32_t nlab;
   uint32_t nq;
-  uint32_t chunk;
   uint32_t versioned;
   uint32_t homo;
   uint32_t want_hash;
@@ -884,7 +886,7 @@ __global__ void k_seeds(SeedArgs a) {
           }
           if (best != 0 && best != NONE) {
             have = true;
-            ch = chunking(best, a.chunk);
+            ch = chunking(best, P->chunk);
           }
         }
       }
@@ -908,7 +910,6 @@ struct SearchArgs {
   Task* spill;                // next round's tasks (nullable: never spill)
   uint32_t* spill_owner;      // next round's chunk owners
   uint32_t spill_cap, spill_chunk_cap;
-  uint32_t budget_iters;      // loop iterations before a task spills
   uint32_t xregions;          // this round's chunks sit in per-XCD regions (spilled rounds)
   Ctrl* ctrl;
   unsigned long long* counts;
@@ -916,7 +917,6 @@ struct SearchArgs {
   const uint32_t* __restrict__ d2;
   uint32_t d2_words;
   uint32_t nq;
-  uint32_t chunk;
   uint32_t homo;
   uint32_t pos_offset;        // position of update index 0 minus 1 (0 in a batch)
   unsigned long long* qcount; // per-query running match count (nullable)
@@ -958,6 +958,7 @@ struct WaveState {
   // the current plan's per-position fields (plan index pqo)
   uint32_t pqo;
   uint32_t pnb[MK], pd2[MK], pqv[MK];
+  uint32_t pchunk, pbudget;   // the plan's split chunk and spill budget
   uint32_t pbw[MK][MK - 1];
   // rarely used per-wave state, kept out of registers: the time-limit word
   // as last read and when, and the candidate count at the task's start
@@ -1157,6 +1158,10 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     if (qo != S.pqo) {
       if (lane < MK) {
         S.pnb[lane] = P->nb[lane];
+        if (lane == 0) {
+          S.pchunk = P->chunk;
+          S.pbudget = P->budget;
+        }
         S.pd2[lane] = P->d2row[lane];
         S.pqv[lane] = P->qv[lane];
       }
@@ -1494,7 +1499,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     // the leaves of parent v (candidate range [lo, hi) of the last position's
     // backward neighbour jm) as a task of the next round
     auto spill_leaves = [&](uint32_t v, uint32_t jm, uint32_t lo, uint32_t hi) CSM_AINL -> bool {
-      const uint2 ch = chunking(hi - lo, a.chunk);
+      const uint2 ch = chunking(hi - lo, S.pchunk);
       uint32_t tb, cb;
       if (!reserve_spill(1u, ch.x, tb, cb)) return false;
       emit_task(tb, cb, L1, jm, lo, hi, ch, v);
@@ -1656,7 +1661,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       uint32_t n_t = 0, n_c = 0;
       for (uint32_t L = d0; L <= lv_now; ++L) {
         uint32_t cu = S.cur[L], en = S.end[L];
-        if (cu < en) { ++n_t; n_c += chunking(en - cu, a.chunk).x; }
+        if (cu < en) { ++n_t; n_c += chunking(en - cu, S.pchunk).x; }
         if (L < L2) { uint32_t r = S.bn[L] - S.bi[L]; n_t += r; n_c += r; }
       }
       n_t = uniform(n_t);
@@ -1668,7 +1673,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       for (uint32_t L = d0; L <= lv_now; ++L) {
         uint32_t cu = S.cur[L], en = S.end[L];
         if (cu < en)
-          emit_task(ti, ci, L, S.jmin[L], cu, en, chunking(en - cu, a.chunk), L > 0 ? S.m[L - 1] : 0u);
+          emit_task(ti, ci, L, S.jmin[L], cu, en, chunking(en - cu, S.pchunk), L > 0 ? S.m[L - 1] : 0u);
         if (L < L2)
           for (uint32_t i = S.bi[L]; i < S.bn[L]; ++i)
             emit_task(ti, ci, L + 1, NONE, 0u, NONE, make_uint2(1u, NONE), S.cand[L][i]);
@@ -1698,7 +1703,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     // and a task spilled before any step would be handed on unchanged
     bool moved = false;
     while (live && !stop) {
-      if (may_spill && ++work > a.budget_iters && (uint32_t)lv < L1 && moved) {
+      if (may_spill && ++work > S.pbudget && (uint32_t)lv < L1 && moved) {
         PROF_BEGIN(sp);
         bool ok_sp = spill((uint32_t)lv);
         PROF_END(sp, 7);

@@ -375,12 +375,15 @@ class Matching:
         return int(a.value), int(b.value), int(c.value)
 
     def stats(self):
-        out = (ctypes.c_uint64 * 12)()
-        self._check(lib().csm_stats(self.h, out, 12))
+        out = (ctypes.c_uint64 * 20)()
+        self._check(lib().csm_stats(self.h, out, 20))
         return {"tasks": out[0], "candidates": out[1], "alg_bytes": out[2], "batches": out[3],
                 "repacks": out[4], "search_ns": out[5], "search_launches": out[6],
                 "max_rounds": out[7], "split_chunk": out[8], "spill_budget": out[9],
-                "tuning": out[10], "graph_batches": out[11]}
+                "tuning": out[10], "graph_batches": out[11],
+                # (split_chunk, spill_budget) per query size class: up to 8,
+                # 12, 16 and 32 vertices
+                "size_class_settings": [(out[12 + 2 * c], out[13 + 2 * c]) for c in range(4)]}
 
     def device_bytes(self) -> int:
         b = ctypes.c_uint64(0)
