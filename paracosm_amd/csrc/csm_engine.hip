@@ -186,8 +186,8 @@ struct csm_engine {
   uint64_t tot_init = 0, tot_pos = 0, tot_neg = 0;
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // HIP events around search launches on the engine stream (kernel time)
-  hipEvent_t ev_search[2] = {nullptr, nullptr};
-  bool ev_pending = false;
+  // the start and end of each search launch of a group of rounds
+  hipEvent_t ev_round[ROUND_GROUP_MAX][2] = {};
   uint64_t dev_bytes = 0;
 
   // time limit (csm_opts.time_limit_s per engine call): a word in pinned,
@@ -948,19 +948,17 @@ void launch_search(csm_engine* e, const SearchArgs& sa) {
   else launch_search_mk<MAXK, H, S>(e, sa);
 }
 
-void launch_round(csm_engine* e, SearchArgs& sa, bool versioned, bool h) {
+// One search launch in slot `slot` of a group of rounds, between that slot's
+// two events.
+void launch_round(csm_engine* e, SearchArgs& sa, bool h, uint32_t slot) {
   const bool s = uses_index(e) && e->d_d2 != nullptr;
-  if (e->ev_search[0]) (void)hipEventRecord(e->ev_search[0], e->stream);
-  (void)versioned;  // every list carries version words; clean lists skip them
+  sa.round_slot = slot;
+  (void)hipEventRecord(e->ev_round[slot][0], e->stream);
   if (h) { if (s) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa); }
   else { if (s) launch_search<false, true>(e, sa); else launch_search<false, false>(e, sa); }
-  if (e->ev_search[1]) {
-    (void)hipEventRecord(e->ev_search[1], e->stream);
-    e->ev_pending = true;
-  }
+  (void)hipEventRecord(e->ev_round[slot][1], e->stream);
 }
 
-void collect_search_time(csm_engine* e);
 int read_ctrl(csm_engine* e);
 
 // The search arguments of one batch (round buffers are set per round).
@@ -970,6 +968,7 @@ SearchArgs search_args(csm_engine* e, unsigned long long* counts, unsigned long 
   sa.g = view(e);
   sa.plans = e->d_plans;
   sa.ctrl = e->d_ctrl;
+  sa.round_slot = 0;
   sa.counts = counts;
   sa.hashes = hashes;
   sa.d2 = e->d_d2;
@@ -1010,25 +1009,45 @@ void round_buffers(csm_engine* e, SearchArgs& sa, uint32_t round) {
 // Search rounds from `first` on: a task that exceeds its iteration budget
 // spills the rest of its DFS into the next round's task list, and rounds
 // repeat until nothing spills.  The round's chunk counts are in place on the
-// device (k_seeds for round 0, k_next_round after each round).
+// device (k_seeds for round 0, k_next_round after each round), so rounds are
+// enqueued back to back, as many as the deepest batch so far needed (a round
+// with nothing to do ends at once), with one read-back of Ctrl per group; a
+// batch that needs more continues with another group.
 int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned long long* hashes,
            uint32_t pos_offset = 0, uint32_t first = 0) {
+  (void)versioned;  // every list carries version words; clean lists skip them
   SearchArgs sa = search_args(e, counts, hashes, pos_offset);
   const bool h = hashes != nullptr;
   static const bool trace = getenv("CSM_TRACE_ROUNDS") != nullptr;
-  for (uint32_t round = first;; ++round) {
-    round_buffers(e, sa, round);
-    uint32_t chunks_in = trace ? e->h_ctrl->nspill_chunks : 0;
+  for (uint32_t round = first;;) {
+    const uint32_t want = e->max_rounds > round ? e->max_rounds - round : 1u;
+    const uint32_t group = trace ? 1u : std::min<uint32_t>(want, ROUND_GROUP_MAX);
+    const uint32_t chunks_in = trace ? e->h_ctrl->nchunks : 0;
     check_deadline(e);   // after the limit, rounds only drain their chunks
-    launch_round(e, sa, versioned, h);
+    k_fill_u32<<<1, 64, 0, e->stream>>>(e->d_ctrl->round_busy, ROUND_GROUP_MAX, 0u);
+    for (uint32_t i = 0; i < group; ++i) {
+      round_buffers(e, sa, round + i);
+      launch_round(e, sa, h, i);
+      k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl, e->spill_chunk_cap);
+    }
     int rc = read_ctrl(e);
     if (rc) return rc;
-    uint64_t ns0 = e->stats[5];
-    collect_search_time(e);
+    // the search time of the launches that ran a task
+    uint32_t busy = 0;
+    uint64_t ns_group = 0;
+    for (uint32_t i = 0; i < group; ++i) {
+      if (!e->h_ctrl->round_busy[i]) continue;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e->ev_round[i][0], e->ev_round[i][1]) == hipSuccess) {
+        e->stats[5] += (uint64_t)(ms * 1e6);
+        e->stats[6] += 1;
+        ns_group += (uint64_t)(ms * 1e6);
+      }
+      busy = i + 1;
+    }
     if (trace) {
-      fprintf(stderr, "[csm] round %u: %s %u chunks, %.3f ms, spilled %u tasks / %u chunks\n",
-              round, round ? "spilled" : "seed", round ? chunks_in : e->h_ctrl->nchunks,
-              (e->stats[5] - ns0) / 1e6, e->h_ctrl->nspill, e->h_ctrl->nspill_chunks);
+      fprintf(stderr, "[csm] round %u: %s %u chunks, %.3f ms, spilled %u chunks\n", round,
+              round ? "spilled" : "seed", chunks_in, ns_group / 1e6, e->h_ctrl->nchunks);
       unsigned long long& life0 = e->trace_life0;
       unsigned long long& waves0 = e->trace_waves0;
       const Ctrl* c = e->h_ctrl;
@@ -1041,8 +1060,9 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
         waves0 = c->prof[29];
       }
     }
-    if (e->h_ctrl->nspill == 0) {
-      e->max_rounds = std::max(e->max_rounds, round + 1);
+    round += group;
+    if (e->h_ctrl->nchunks == 0) {   // the group's last round spilled nothing
+      e->max_rounds = std::max(e->max_rounds, round - group + busy);
       if (trace && e->h_ctrl->prof[8]) {
         static const char* names[8] = {"fetch", "enter", "prepare_last", "refill",
                                        "refill_k-2", "leaves_shared", "leaves_own", "spill"};
@@ -1054,20 +1074,8 @@ int search(csm_engine* e, bool versioned, unsigned long long* counts, unsigned l
       }
       return CSM_OK;
     }
-    k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl, e->spill_chunk_cap);
     if (round > 100000) return fail(e, CSM_E_DEVICE, "search did not converge");
   }
-}
-
-// after a stream sync: fold the last search launch's duration into the stats
-void collect_search_time(csm_engine* e) {
-  if (!e->ev_pending) return;
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, e->ev_search[0], e->ev_search[1]) == hipSuccess) {
-    e->stats[5] += (uint64_t)(ms * 1e6);
-    e->stats[6] += 1;
-  }
-  e->ev_pending = false;
 }
 
 void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long long* hashes,
@@ -1314,6 +1322,7 @@ int enqueue_batch_head(csm_engine* e, uint32_t n, bool hash, uint32_t rounds) {
   const bool symbi = uses_index(e) && e->d_d2 != nullptr;
   for (uint32_t r = 0; r < rounds; ++r) {
     round_buffers(e, sa, r);
+    sa.round_slot = r % ROUND_GROUP_MAX;
     if (hash) { if (symbi) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa); }
     else { if (symbi) launch_search<false, true>(e, sa); else launch_search<false, false>(e, sa); }
     k_next_round<<<1, 64, 0, e->stream>>>(e->d_ctrl, e->spill_chunk_cap);
@@ -1412,7 +1421,6 @@ int run_batch_graph(csm_engine* e, const csm_update* d_updates, uint32_t n, uint
 }
 
 void accumulate_stats(csm_engine* e) {
-  collect_search_time(e);
   e->stats[7] = e->max_rounds;
   e->stats[0] = e->h_ctrl->stat_tasks;
   e->stats[1] = e->h_ctrl->stat_cands;
@@ -2159,10 +2167,9 @@ static csm_engine* create_engine(const char* algo, const csm_graph* query, const
   e->num_cus = prop.multiProcessorCount;
   if (e->opts.device_budget == 0) e->opts.device_budget = (uint64_t)(prop.totalGlobalMem * 0.9);
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
-  if (hipEventCreate(&e->ev_search[0]) != hipSuccess || hipEventCreate(&e->ev_search[1]) != hipSuccess) {
-    set(CSM_E_DEVICE);
-    return nullptr;
-  }
+  for (auto& pair : e->ev_round)
+    for (auto& ev : pair)
+      if (hipEventCreate(&ev) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
   if (hipStreamCreateWithFlags(&e->xfer, hipStreamNonBlocking) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
   for (auto& ev : e->marks)
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) { set(CSM_E_DEVICE); return nullptr; }
@@ -2968,8 +2975,9 @@ void csm_destroy(csm_engine* e) {
   if (e->h_uwork_pin) (void)hipHostFree(e->h_uwork_pin);
   if (e->h_abort) (void)hipHostFree(e->h_abort);
   if (e->h_parse) (void)hipHostFree(e->h_parse);
-  for (auto& ev : e->ev_search)
-    if (ev) (void)hipEventDestroy(ev);
+  for (auto& pair : e->ev_round)
+    for (auto& ev : pair)
+      if (ev) (void)hipEventDestroy(ev);
   if (e->xfer) (void)hipStreamSynchronize(e->xfer);
   for (auto& ev : e->marks)
     if (ev) (void)hipEventDestroy(ev);

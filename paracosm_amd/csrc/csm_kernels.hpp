@@ -58,6 +58,7 @@ constexpr uint32_t ERR_POOL_OVERFLOW = 8u;
 constexpr uint32_t ERR_BAD_UPDATE = 16u;
 constexpr uint32_t ERR_VERTEX_UPDATE = 64u;   // a vertex record inside a device batch
 
+constexpr uint32_t ROUND_GROUP_MAX = 16;   // search rounds enqueued per host read-back
 constexpr uint32_t NXCD = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr uint32_t XHEAD_STRIDE = 32;  // u32 words between two XCD chunk heads
 
@@ -89,6 +90,9 @@ struct Ctrl {
   uint32_t xspill[NXCD * XHEAD_STRIDE];
   uint32_t xspill_chunks[NXCD * XHEAD_STRIDE];
   uint32_t xn[NXCD];
+  // rounds enqueued back to back: slot i is set by the i-th launch of the
+  // group once a wave of it ran a task (the host counts only busy launches)
+  uint32_t round_busy[ROUND_GROUP_MAX];
 };
 
 // One search plan per (query, matching order): positions 0..k-1.
@@ -927,6 +931,7 @@ struct SearchArgs {
   // keeps one word of them in registers
   uint32_t lflags;
   const SearchLimits* __restrict__ lim;
+  uint32_t round_slot;        // this launch's slot in Ctrl::round_busy
 };
 
 // real-time counter ticks (100 MHz) between two polls of the host abort word
@@ -1808,6 +1813,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
   if (lane == 0) {
     // a wave of an empty round adds nothing (and skips three contended atomics)
     if (n_tasks) {
+      a.ctrl->round_busy[a.round_slot] = 1u;
       atomicAdd(&a.ctrl->stat_tasks, n_tasks);
       atomicAdd(&a.ctrl->stat_cands, n_cands);
       atomicAdd(&a.ctrl->stat_bytes, n_bytes + 4ull * n_cands);
