@@ -270,6 +270,17 @@ int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out);
  * 6 n2 (per query vertex, per neighbour in id order, [nv] each). */
 int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, size_t n);
 
+/* The matching orders of query qid, one per query edge (the query vertex at
+ * each position; positions 0 and 1 are the edge's endpoints): out receives
+ * *n_orders x k entries (out == NULL: only *n_orders).  For
+ * parallel_turboflux with reference_orders these follow TurboFlux's rule
+ * (Parallel_TurboFlux::GenerateMatchingOrder, parallel_turboflux.cpp:533-700,
+ * which the reference prints with --print-prep), and paths (k entries,
+ * nullable) receives its explicit-path counts per query vertex
+ * (CountDownwards, :469-491); otherwise paths are 0. */
+int csm_query_orders(csm_engine* e, uint32_t qid, uint32_t* out, size_t n, uint32_t* paths,
+                     size_t* n_orders);
+
 /* Device and pinned-host buffers on the engine's device and runtime, for
  * callers that keep inputs resident in HBM without a HIP runtime of their
  * own (one HIP runtime per process).  Device bytes count towards

@@ -39,6 +39,7 @@
 #include <cstring>
 #include <deque>
 #include <string>
+#include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1254,6 +1255,97 @@ uint32_t orc_dag_parents(const orc_graph* data, const orc_graph* query, int tree
   for (uint32_t u = 0; u < k; ++u)
     for (uint32_t j = 0; j < k; ++j) out[u * k + j] = j < sb.tn[u].bw.size() ? sb.tn[u].bw[j] : NOT_EXIST;
   return sb.root;
+}
+
+// TurboFlux's matching orders (parallel_turboflux.cpp:469-836), over the
+// spanning-tree index of the initial graph.
+//   CountDownwards (:469-491): from every data vertex carrying the root's
+//   label (the keys of DCS_[root -> first child], :536-541), walk the DCS
+//   lists down the tree, counting per query vertex the d2-valid children
+//   reached; a (child, parent data vertex) pair is walked once and its count
+//   reused afterwards (num_dp).  Counts are 32-bit and wrap, as `uint` does;
+//   the root's count is UINT_MAX - 1.
+//   initial_order (:544-575): repeatedly take the leaf of the remaining tree
+//   with the fewest paths (ties: the smallest id; the reference takes the
+//   first in an unordered_set's iteration order) and fill the order from the
+//   back; a parent becomes a leaf once its children are all taken.
+//   Per query edge (u < u' in id order, :577-700): a tree edge is oriented
+//   child first; the order is the two endpoints, the tree path from the
+//   second up to the root, then the rest in initial_order.
+// out[i*k + j]: order i (query edges in that enumeration); paths[u]: the
+// path counts.  Returns the number of orders.
+struct TurboFluxOrders {
+  const SymBi& sb;
+  std::vector<uint32_t> paths;
+  std::vector<std::unordered_map<uint32_t, uint32_t>> dp;
+  explicit TurboFluxOrders(const SymBi& s) : sb(s), paths(s.k, 0), dp(s.k) {}
+  void count_downwards(uint32_t u, uint32_t v) {
+    for (uint32_t uc : sb.tn[u].fw) {
+      auto it = dp[uc].find(v);
+      if (it == dp[uc].end()) {
+        sb.dcs_for(u, uc, v, [&](uint32_t vc) {
+          if (sb.d2[uc][vc]) {
+            ++paths[uc];
+            ++dp[uc][v];
+            count_downwards(uc, vc);
+          }
+        });
+      } else {
+        paths[uc] += it->second;
+      }
+    }
+  }
+};
+
+int orc_turboflux_orders(const orc_graph* data, const orc_graph* query, uint32_t* out,
+                         uint32_t* paths_out) {
+  Graph d = data->g;
+  const Graph& q = query->g;
+  SymBi sb(q, d, Opts(), true);
+  const uint32_t k = q.nv(), root = sb.root;
+  TurboFluxOrders t(sb);
+  for (uint32_t v = 0; v < d.nv(); ++v)
+    if (d.label(v) == q.label(root)) t.count_downwards(root, v);
+  t.paths[root] = UINT_MAX - 1;
+  if (paths_out)
+    for (uint32_t u = 0; u < k; ++u) paths_out[u] = t.paths[u];
+  std::vector<uint8_t> leaf(k, 0), taken(k, 0);
+  for (uint32_t u = 0; u < k; ++u) leaf[u] = sb.tn[u].fw.empty();
+  std::vector<uint32_t> initial(k);
+  for (uint32_t i = 0; i < k; ++i) {
+    uint32_t best = NOT_EXIST;
+    for (uint32_t u = 0; u < k; ++u)
+      if (leaf[u] && (best == NOT_EXIST || t.paths[u] < t.paths[best])) best = u;
+    leaf[best] = 0;
+    taken[best] = 1;
+    initial[k - 1 - i] = best;
+    if (best != root) {
+      const uint32_t par = sb.tn[best].bw.front();
+      bool all = true;
+      for (uint32_t c : sb.tn[par].fw) all = all && taken[c];
+      if (all) leaf[par] = 1;
+    }
+  }
+  int n = 0;
+  for (uint32_t a = 0; a < k; ++a)
+    for (uint32_t b : q.nb[a]) {
+      if (a > b) continue;
+      uint32_t u = a, uo = b;
+      if (!sb.tn[uo].bw.empty() && sb.tn[uo].bw.front() == u) std::swap(u, uo);
+      std::vector<uint8_t> vis(k, 0);
+      std::vector<uint32_t> ord{u, uo};
+      vis[u] = vis[uo] = 1;
+      for (uint32_t cur = uo; cur != root;) {
+        cur = sb.tn[cur].bw.front();
+        vis[cur] = 1;
+        ord.push_back(cur);
+      }
+      for (uint32_t x : initial)
+        if (!vis[x]) { vis[x] = 1; ord.push_back(x); }
+      for (uint32_t j = 0; j < k; ++j) out[(size_t)n * k + j] = ord[j];
+      ++n;
+    }
+  return n;
 }
 
 // Matching orders of the GraphFlow plugin (graphflow.cpp:36-160), for

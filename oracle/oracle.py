@@ -62,6 +62,8 @@ def lib():
                                             ctypes.c_double]
         L.orc_graphflow_orders.restype = ctypes.c_int
         L.orc_graphflow_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p]
+        L.orc_turboflux_orders.restype = ctypes.c_int
+        L.orc_turboflux_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p, _u32p]
         _lib = L
     return _lib
 
@@ -159,6 +161,16 @@ def graphflow_orders(data: OGraph, query: OGraph, k: int, ne: int) -> np.ndarray
     out = np.zeros(max(ne, 1) * k, dtype=np.uint32)
     n = lib().orc_graphflow_orders(data.h, query.h, out)
     return out[: n * k].reshape(n, k)
+
+
+def turboflux_orders(data: OGraph, query: OGraph, k: int, ne: int):
+    """TurboFlux's matching orders (one per query edge) and explicit-path
+    counts per query vertex over the spanning-tree index of `data` (see
+    orc_turboflux_orders)."""
+    out = np.zeros(max(ne, 1) * k, dtype=np.uint32)
+    paths = np.zeros(k, dtype=np.uint32)
+    n = lib().orc_turboflux_orders(data.h, query.h, out, paths)
+    return out[: n * k].reshape(n, k), paths
 
 
 def symbi_dump(data: OGraph, query: OGraph, what: int, nv: int, rows: int) -> np.ndarray:

@@ -59,6 +59,9 @@ struct QueryInfo {
   Dag dag;
   uint32_t d2_base = 0;                 // first SymBi row
   bool sx_built = false;
+  // TurboFlux's explicit-path counts per query vertex (its order rule's
+  // ranking; computed for parallel_turboflux with reference_orders)
+  std::vector<uint32_t> tf_paths;
 };
 
 inline uint32_t bits_for(uint64_t n) {  // bits to hold values in [0, n)
@@ -393,14 +396,25 @@ int build_query(csm_engine* e, const csm_graph* q, std::unique_ptr<QueryInfo>& o
 // Dense label maps must cover the query's labels; data labels are fixed at
 // create time, so a query label absent from the data can never match: it is
 // mapped to a label no data vertex carries.
+int turboflux_paths(csm_engine* e, QueryInfo& q, std::vector<uint32_t>& paths);
+
 int plan_query(csm_engine* e, QueryInfo& qi, uint32_t qid) {
   const QueryGraph& g = qi.g;
   qi.dlab.resize(g.k);
+  bool labels_known = true;
   for (uint32_t u = 0; u < g.k; ++u) {
     uint32_t d = dense_of(e->vl_map, g.vl[u]);
     qi.dlab[u] = d;
+    labels_known = labels_known && d != NO_LABEL;
+    for (uint32_t w : g.nb[u]) labels_known = labels_known && dense_of(e->el_map, g.elabel(u, w)) != NO_LABEL;
   }
-  if (!e->opts.reference_orders) {
+  qi.tf_paths.clear();
+  if (e->opts.reference_orders && e->algo == ALGO_TURBOFLUX && qi.sx_built && labels_known) {
+    // TurboFlux's own rule, ranked by the path counts of its index
+    int rc = turboflux_paths(e, qi, qi.tf_paths);
+    if (rc) return rc;
+    qi.orders = turboflux_orders(g, qi.dag, qi.tf_paths);
+  } else if (!e->opts.reference_orders) {
     auto el_of = [&](uint32_t a, uint32_t b) { return dense_of(e->el_map, g.elabel(a, b)); };
     auto triple = [&](uint32_t a, uint32_t b) -> double {
       uint32_t l = el_of(a, b);
@@ -1601,6 +1615,41 @@ int symbi_prepare_query(csm_engine* e, QueryInfo& q) {
   return CSM_OK;
 }
 
+// TurboFlux's explicit-path counts of a query whose spanning-tree index is
+// built (k_tf_paths, one launch per tree edge in BFS order).
+int turboflux_paths(csm_engine* e, QueryInfo& q, std::vector<uint32_t>& paths) {
+  const uint32_t k = q.g.k, nv = e->nv;
+  paths.assign(k, 0);
+  uint32_t* calls = nullptr;
+  unsigned long long* d_paths = nullptr;
+  int rc;
+  if ((rc = dalloc(e, &calls, (size_t)k * nv)) || (rc = dalloc(e, &d_paths, k))) {
+    dfree(e, calls);
+    return rc;
+  }
+  HIP_TRY(e, hipMemsetAsync(calls, 0, (size_t)k * nv * 4, e->stream));
+  HIP_TRY(e, hipMemsetAsync(d_paths, 0, (size_t)k * 8, e->stream));
+  const GraphView g = view(e);
+  const SxView x = sx_view(e);
+  const int grid = grid_for(nv, 256);
+  const uint32_t root = q.dag.root;
+  k_tf_root_calls<<<grid, 256, 0, e->stream>>>(e->d_vlab, q.dlab[root], calls + (size_t)root * nv, nv);
+  for (uint32_t u : q.dag.bfs)
+    for (uint32_t c : q.dag.children[u]) {
+      const uint32_t key = q.dlab[c] * e->nel + dense_of(e->el_map, q.g.elabel(u, c));
+      k_tf_paths<<<grid, 256, 0, e->stream>>>(g, x, key, q.d2_base + c, calls + (size_t)u * nv,
+                                              calls + (size_t)c * nv, d_paths + c);
+    }
+  std::vector<unsigned long long> h(k);
+  HIP_TRY(e, hipMemcpyAsync(h.data(), d_paths, k * 8, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  dfree(e, calls);
+  dfree(e, d_paths);
+  for (uint32_t u = 0; u < k; ++u) paths[u] = (uint32_t)h[u];   // 32-bit, as the reference's uint
+  paths[root] = UINT32_MAX - 1;
+  return CSM_OK;
+}
+
 int sx_buffers(csm_engine* e) {
   if (e->d_sx_mark) return CSM_OK;
   int rc;
@@ -2671,6 +2720,24 @@ int csm_stats(const csm_engine* e, uint64_t* out, size_t n_out) {
     if (n_out > 12 + 2 * (size_t)c) out[12 + 2 * c] = e->cls_chunk[c];
     if (n_out > 13 + 2 * (size_t)c) out[13 + 2 * c] = e->cls_budget[c];
   }
+  return CSM_OK;
+}
+
+int csm_query_orders(csm_engine* e, uint32_t qid, uint32_t* out, size_t n, uint32_t* paths,
+                     size_t* n_orders) {
+  if (!e || !n_orders) return CSM_E_ARG;
+  if (qid >= e->queries.size()) return fail(e, CSM_E_ARG, "no query %u", qid);
+  if (!e->preprocessed) { int rc = csm_preprocess(e); if (rc) return rc; }
+  const QueryInfo& q = *e->queries[qid];
+  const uint32_t k = q.g.k;
+  *n_orders = q.orders.size();
+  if (out) {
+    if (n < q.orders.size() * k) return fail(e, CSM_E_ARG, "order buffer too small");
+    for (size_t i = 0; i < q.orders.size(); ++i)
+      for (uint32_t j = 0; j < k; ++j) out[i * k + j] = q.orders[i][j];
+  }
+  if (paths)
+    for (uint32_t u = 0; u < k; ++u) paths[u] = q.tf_paths.empty() ? 0u : q.tf_paths[u];
   return CSM_OK;
 }
 

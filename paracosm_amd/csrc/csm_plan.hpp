@@ -265,4 +265,57 @@ Dag turboflux_tree(const QueryGraph& q, EdgeFreq edge_freq, VertexFreq vertex_fr
   return d;
 }
 
+// TurboFlux's matching orders (ParaCOSM/CSM/matching/Parallel_TurboFlux/
+// parallel_turboflux.cpp:533-700), one per query edge, from its spanning tree
+// and its explicit-path counts per query vertex (`paths`, the root's being
+// UINT32_MAX - 1):
+//   * initial order: repeatedly take the leaf of the remaining tree with the
+//     fewest paths (ties: the smallest id -- the reference takes the first in
+//     an unordered_set's iteration order) and fill the order from the back; a
+//     parent becomes a leaf once its children are all taken;
+//   * per query edge (u < u' in id order; a tree edge oriented child first):
+//     the two endpoints, the tree path from the second up to the root, then
+//     the rest in the initial order.
+inline std::vector<std::vector<uint32_t>> turboflux_orders(const QueryGraph& q, const Dag& t,
+                                                           const std::vector<uint32_t>& paths) {
+  const uint32_t k = q.k;
+  auto parent = [&](uint32_t u) { return t.parents[u].empty() ? UINT32_MAX : t.parents[u][0]; };
+  std::vector<uint8_t> leaf(k, 0), taken(k, 0);
+  for (uint32_t u = 0; u < k; ++u) leaf[u] = t.children[u].empty();
+  std::vector<uint32_t> initial(k);
+  for (uint32_t i = 0; i < k; ++i) {
+    uint32_t best = UINT32_MAX;
+    for (uint32_t u = 0; u < k; ++u)
+      if (leaf[u] && (best == UINT32_MAX || paths[u] < paths[best])) best = u;
+    leaf[best] = 0;
+    taken[best] = 1;
+    initial[k - 1 - i] = best;
+    if (best != t.root) {
+      const uint32_t par = parent(best);
+      bool all = true;
+      for (uint32_t c : t.children[par]) all = all && taken[c];
+      if (all) leaf[par] = 1;
+    }
+  }
+  std::vector<std::vector<uint32_t>> orders;
+  for (uint32_t a = 0; a < k; ++a)
+    for (uint32_t b : q.nb[a]) {
+      if (a > b) continue;
+      uint32_t u = a, uo = b;
+      if (parent(uo) == u) std::swap(u, uo);
+      std::vector<uint8_t> vis(k, 0);
+      std::vector<uint32_t> ord{u, uo};
+      vis[u] = vis[uo] = 1;
+      for (uint32_t cur = uo; cur != t.root;) {
+        cur = parent(cur);
+        vis[cur] = 1;
+        ord.push_back(cur);
+      }
+      for (uint32_t x : initial)
+        if (!vis[x]) { vis[x] = 1; ord.push_back(x); }
+      orders.push_back(ord);
+    }
+  return orders;
+}
+
 }  // namespace csm

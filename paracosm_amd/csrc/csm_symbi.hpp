@@ -143,6 +143,45 @@ __global__ void k_sx_build(GraphView g, SxView x, uint32_t r, int mode) {
   }
 }
 
+// TurboFlux's explicit-path counts (CountDownwards, parallel_turboflux.cpp:
+// 469-491), in closed form over one tree edge u -> c at a time, parents before
+// children.  calls_u[v] is the number of times CountDownwards(u, v) runs: 1
+// for every data vertex with the root's label (GenerateMatchingOrder calls it
+// once per key of DCS_[root -> first child], :536-541), and for a child the
+// number of distinct parent vertices whose walk reached it (the walk of a
+// (c, parent vertex) pair happens on its first run only; later runs add its
+// memoised count).  Each run of (u, v) adds to paths[c] the number of
+// d2-valid children w in S(v, L(c), l(u,c)); each first walk adds 1 to
+// calls_c[w].
+__global__ void k_tf_root_calls(const uint32_t* __restrict__ vlab, uint32_t lab,
+                                uint32_t* __restrict__ calls, uint32_t nv) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x)
+    calls[v] = vlab[v] == lab ? 1u : 0u;
+}
+
+__global__ void k_tf_paths(GraphView g, SxView x, uint32_t key, uint32_t crow,
+                           const uint32_t* __restrict__ calls_u, uint32_t* __restrict__ calls_c,
+                           unsigned long long* __restrict__ paths) {
+  unsigned long long acc = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < g.nv; v += gridDim.x * blockDim.x) {
+    const uint32_t runs = calls_u[v];
+    if (runs == 0) continue;
+    const uint2 rec = g.vrec[v];
+    const bool dirty = (rec.y & DIRTY) != 0;
+    const uint2 sg = seg_at(g, v, rec.x, rec.y & LEN_MASK, key);
+    uint32_t n = 0;
+    for (uint32_t i = sg.x; i < sg.y; ++i) {
+      if (!present(g, i, sg.x, dirty, true)) continue;
+      const uint32_t w = g.adj[i] & g.vmask;
+      if (!bit_get(x.d2, x.words, crow, w)) continue;
+      ++n;
+      atomicAdd(&calls_c[w], 1u);
+    }
+    acc += (unsigned long long)runs * n;
+  }
+  if (acc) atomicAdd(paths, acc);
+}
+
 // a new vertex: root rows get d1, rows without children also d2
 __global__ void k_sx_add_vertex(SxView x, uint32_t nrows, uint32_t v, uint32_t lab) {
   for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {

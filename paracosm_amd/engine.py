@@ -109,6 +109,7 @@ def lib():
             "csm_device_bytes": (ctypes.c_int, [vp, P(u64)]),
             "csm_stats": (ctypes.c_int, [vp, P(u64), sz]),
             "csm_index_dump": (ctypes.c_int, [vp, u32, u32, vp, sz]),
+            "csm_query_orders": (ctypes.c_int, [vp, u32, vp, sz, vp, ctypes.POINTER(ctypes.c_size_t)]),
             "csm_device_alloc": (ctypes.c_int, [vp, sz, P(vp)]),
             "csm_device_free": (ctypes.c_int, [vp, vp]),
             "csm_host_alloc": (ctypes.c_int, [vp, sz, P(vp)]),
@@ -147,7 +148,8 @@ def exported_symbols() -> Sequence[str]:
             "csm_apply_batch_device", "csm_sync", "csm_query_status", "csm_update_status",
             "csm_set_time_limit", "csm_update_work", "csm_embeddings", "csm_totals",
             "csm_memory",
-            "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_device_alloc",
+            "csm_device_bytes", "csm_stats", "csm_index_dump", "csm_query_orders",
+            "csm_device_alloc",
             "csm_device_free", "csm_host_alloc", "csm_host_free", "csm_copy_to_device",
             "csm_copy_to_host", "csm_copy_device", "csm_stream", "csm_copy_async", "csm_mark",
             "csm_wait_mark", "csm_sync_mark", "csm_mem_info", "csm_parse_text",
@@ -255,6 +257,7 @@ class Matching:
             raise CsmError(err.value, f"csm_create failed for {algo!r}")
         self.algo = algo
         self.want_hash = want_hash
+        self._query_k = [len(query.vlabels)]
 
     # -- helpers
     def _check(self, rc: int) -> None:
@@ -327,6 +330,7 @@ class Matching:
         g = _GraphArgs(query.vlabels, query.src, query.dst, query.elabels)
         qid = ctypes.c_uint32(0)
         self._check(lib().csm_register_query(self.h, ctypes.byref(g.g), ctypes.byref(qid)))
+        self._query_k.append(len(query.vlabels))
         return int(qid.value)
 
     @property
@@ -395,6 +399,20 @@ class Matching:
         out = np.zeros(max(1, rows * nv), np.uint32)
         self._check(lib().csm_index_dump(self.h, qid, what, out.ctypes.data, out.size))
         return out if what == 0 else out.reshape(rows, nv)
+
+    def query_orders(self, qid: int = 0):
+        """(orders [n, k], paths [k]) of query `qid`: its matching orders, one
+        per query edge, and TurboFlux's explicit-path counts (see
+        csm_query_orders; zeros unless parallel_turboflux with
+        reference_orders)."""
+        n = ctypes.c_size_t(0)
+        self._check(lib().csm_query_orders(self.h, qid, None, 0, None, ctypes.byref(n)))
+        k = self._query_k[qid]
+        out = np.zeros(max(1, n.value * k), np.uint32)
+        paths = np.zeros(k, np.uint32)
+        self._check(lib().csm_query_orders(self.h, qid, out.ctypes.data, out.size,
+                                           paths.ctypes.data, ctypes.byref(n)))
+        return out[: n.value * k].reshape(n.value, k), paths
 
     # -- device buffers on the engine's own HIP runtime (no second runtime)
     def device_buffer(self, nbytes: int) -> "DeviceBuffer":

@@ -7,7 +7,11 @@ the set definitions, with no incremental rule and no code shared with the
 oracle, and compared element by element with orc_index_dump / orc_dag_parents
 on small random graphs.  The GraphFlow matching orders
 (Graphflow::GenerateMatchingOrder, graphflow.cpp:36-160) are checked on a
-hand-worked query whose orders depend on both tie rules.
+hand-worked query whose orders depend on both tie rules.  TurboFlux's
+explicit-path counts and matching orders (Parallel_TurboFlux::CountDownwards
+and GenerateMatchingOrder, parallel_turboflux.cpp:469-700) are checked against
+a closed form of the memoised count and the order definition on random cases,
+and on two hand-worked ones.
 """
 import numpy as np
 import pytest
@@ -200,3 +204,175 @@ def test_graphflow_orders_tie_rules(oracle_lib):
     got = O.graphflow_orders(g, O.OGraph.from_query(q), 5, 5).tolist()
     assert got == [[2, 0, 1, 3, 4], [1, 0, 2, 3, 4], [3, 1, 2, 0, 4], [3, 2, 0, 1, 4],
                    [4, 2, 0, 1, 3]]
+
+
+# ------------------------------------------------ TurboFlux's matching orders
+U32 = 1 << 32
+
+
+def turboflux_paths_from_definition(q, vl, adj, root, parents, d2):
+    """TurboFlux's explicit-path counts (Parallel_TurboFlux::CountDownwards,
+    parallel_turboflux.cpp:469-491, started once per data vertex of the root's
+    label, :536-541) in closed form, tree edge by tree edge:
+
+    C(c, v)      = {w in S(v, u->c) : d2[c][w]} for the tree edge u -> c
+    calls(r, v)  = 1 for every v with L(v) = L(r), r the root
+    calls(c, w)  = #{v : calls(u, v) > 0 and w in C(c, v)}
+    paths[c]     = sum_v calls(u, v) * |C(c, v)|  (mod 2^32)
+    paths[r]     = 2^32 - 2.
+
+    calls(c, w) counts the distinct parent vertices that reach w, not the
+    paths into w: the memo of a (c, parent vertex) pair replays its count on
+    later runs without walking again."""
+    k, nv = q.k, len(vl)
+    qa = _qadj(q)
+    lab = {(u, x): l for u in range(k) for x, l in qa[u]}
+    children = [[c for c in range(k) if parents[c] == [u]] for u in range(k)]
+    calls = {root: [1 if vl[v] == q.vlabels[root] else 0 for v in range(nv)]}
+    paths = [0] * k
+    frontier = [root]
+    while frontier:
+        u = frontier.pop(0)
+        for c in children[u]:
+            calls[c] = [0] * nv
+            for v in range(nv):
+                if not calls[u][v]:
+                    continue
+                cand = [w for w, l in adj[v] if vl[w] == q.vlabels[c] and l == lab[(u, c)]
+                        and d2[c][w]]
+                paths[c] += calls[u][v] * len(cand)
+                for w in cand:
+                    calls[c][w] += 1
+            frontier.append(c)
+    paths = [p % U32 for p in paths]
+    paths[root] = U32 - 2
+    return paths
+
+
+def turboflux_orders_from_definition(q, root, parents, paths):
+    """GenerateMatchingOrder (parallel_turboflux.cpp:533-700): the initial
+    order lists the tree so that the leaf with the fewest paths (ties: the
+    smallest id) goes last, then the next leaf of what remains, a parent
+    becoming a leaf when its children are gone; the order of a query edge
+    {a, b} (a < b, a tree edge written child first) starts with its two
+    endpoints and the second one's ancestors up to the root, followed by the
+    other vertices in initial order.  Edges go by a, then b, ascending."""
+    k = q.k
+    par = [p[0] if p else None for p in parents]
+    remaining = set(range(k))
+    initial = []
+    while remaining:
+        leaves = [u for u in remaining if not any(par[c] == u for c in remaining)]
+        u = min(leaves, key=lambda x: (paths[x], x))
+        initial.insert(0, u)
+        remaining.remove(u)
+    edges = sorted({(min(a, b), max(a, b)) for a, b in zip(q.src.tolist(), q.dst.tolist())})
+    orders = []
+    for a, b in edges:
+        first, second = (b, a) if par[b] == a else (a, b)
+        o = [first, second]
+        x = second
+        while x != root:
+            x = par[x]
+            o.append(x)
+        o += [x for x in initial if x not in o]
+        orders.append(o)
+    return orders
+
+
+def test_turboflux_paths_hand_worked(oracle_lib):
+    """Query path 0-1-2-3 (labels 0, 1, 2, 3).  Data: two label-0 vertices
+    0 and 1 on one label-1 vertex 2, which has the label-2 vertex 5, which has
+    the label-3 vertices 7 and 8; vertices 3, 4, 6, 9, 10 (labels 1, 1, 2, 2,
+    3) make label 0 the rarest, and 3-5 keeps the three query edges at two
+    matching data edges each, so the root is 0 and the tree is the path.
+    calls(1, 2) = 2 (both roots reach 2), calls(2, 5) = 1 (one parent vertex),
+    so paths = [2^32 - 2, 2, 2, 2], where the true numbers of root-to-vertex
+    paths are 2, 2 and 4: the memo's count, not the path count."""
+    O = oracle_lib
+    q = Query(vlabels=np.asarray([0, 1, 2, 3], np.uint32), src=np.asarray([0, 1, 2], np.uint32),
+              dst=np.asarray([1, 2, 3], np.uint32), elabels=np.zeros(3, np.uint32))
+    vl = np.asarray([0, 0, 1, 1, 1, 2, 2, 3, 3, 2, 3], np.uint32)
+    e = [(0, 2), (1, 2), (2, 5), (3, 5), (5, 7), (5, 8)]
+    dg = O.OGraph(vl, [a for a, _ in e], [b for _, b in e], [0] * len(e))
+    qg = O.OGraph.from_query(q)
+    root, par = O.dag_parents(dg, qg, 4, tree=True)
+    assert (root, par) == (0, [[], [0], [1], [2]])
+    orders, paths = O.turboflux_orders(dg, qg, 4, 3)
+    assert paths.tolist() == [U32 - 2, 2, 2, 2]
+    assert orders.tolist() == [[1, 0, 2, 3], [2, 1, 0, 3], [3, 2, 1, 0]]
+
+
+def test_turboflux_initial_order_hand_worked(oracle_lib):
+    """Query star 0-{1, 2, 3} (labels 1, 2, 3 on the leaves) with 2-4 below
+    leaf 2 (label 4).  Data: the root vertex 0 has three label-1, one label-2
+    and two label-3 neighbours, and the label-2 one has one label-4
+    neighbour: paths [2^32 - 2, 3, 1, 2, 1].  Filling the initial order from
+    the back: 4 (1 path; 2 becomes a leaf), 2 (1), 3 (2), 1 (3), the root:
+    0, 1, 3, 2, 4."""
+    O = oracle_lib
+    q = Query(vlabels=np.asarray([0, 1, 2, 3, 4], np.uint32),
+              src=np.asarray([0, 0, 0, 2], np.uint32), dst=np.asarray([1, 2, 3, 4], np.uint32),
+              elabels=np.zeros(4, np.uint32))
+    vl = np.asarray([0, 1, 1, 1, 2, 3, 3, 4], np.uint32)
+    e = [(0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (0, 6), (4, 7)]
+    dg = O.OGraph(vl, [a for a, _ in e], [b for _, b in e], [0] * len(e))
+    qg = O.OGraph.from_query(q)
+    root, par = O.dag_parents(dg, qg, 5, tree=True)
+    assert (root, par) == (0, [[], [0], [0], [0], [2]])
+    orders, paths = O.turboflux_orders(dg, qg, 5, 4)
+    assert paths.tolist() == [U32 - 2, 3, 1, 2, 1]
+    # edge orders: endpoints (child first), the ancestors, then 0, 1, 3, 2, 4
+    assert orders.tolist() == [[1, 0, 3, 2, 4], [2, 0, 1, 3, 4], [3, 0, 1, 2, 4],
+                               [4, 2, 0, 1, 3]]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_turboflux_orders_from_definition(oracle_lib, seed):
+    """The oracle's recursive, memoised CountDownwards and its order rule
+    equal the closed form and the order definition above on random cases
+    (tree and d2 from the oracle; both are pinned by the tests above)."""
+    O = oracle_lib
+    vl, s, d, el, q = _case(seed)
+    dg, qg = O.OGraph(vl, s, d, el), O.OGraph.from_query(q)
+    root, par = O.dag_parents(dg, qg, q.k, tree=True)
+    d2 = index_from_definition(q, vl, _adj(len(vl), s, d, el), par)[2]
+    paths = turboflux_paths_from_definition(q, vl, _adj(len(vl), s, d, el), root, par, d2)
+    ne = len({(min(a, b), max(a, b)) for a, b in zip(q.src.tolist(), q.dst.tolist())})
+    got_orders, got_paths = O.turboflux_orders(dg, qg, q.k, ne)
+    assert got_paths.tolist() == paths
+    assert got_orders.tolist() == turboflux_orders_from_definition(q, root, par, paths)
+
+
+def test_turboflux_order_cases_are_not_trivial(oracle_lib):
+    """Across the random cases the memo count differs from the plain
+    root-to-vertex path count somewhere, and some order's tail is not the
+    tree's BFS order, so a dropped memo or a wrong ranking changes a result."""
+    memo_differs = order_differs = 0
+    for seed in range(12):
+        vl, s, d, el, q = _case(seed)
+        adj = _adj(len(vl), s, d, el)
+        root, par = oracle_lib.dag_parents(oracle_lib.OGraph(vl, s, d, el),
+                                           oracle_lib.OGraph.from_query(q), q.k, tree=True)
+        d2 = index_from_definition(q, vl, adj, par)[2]
+        paths = turboflux_paths_from_definition(q, vl, adj, root, par, d2)
+        # plain path counts: number of root-to-vertex walks through d2 vertices
+        qa = _qadj(q)
+        lab = {(u, x): l for u in range(q.k) for x, l in qa[u]}
+        cnt = {root: [1 if vl[v] == q.vlabels[root] else 0 for v in range(len(vl))]}
+        order = [root]
+        for u in order:
+            for c in range(q.k):
+                if par[c] == [u]:
+                    cnt[c] = [0] * len(vl)
+                    for v in range(len(vl)):
+                        for w, l in adj[v]:
+                            if cnt[u][v] and vl[w] == q.vlabels[c] and l == lab[(u, c)] and d2[c][w]:
+                                cnt[c][w] += cnt[u][v]
+                    order.append(c)
+        true_paths = [sum(cnt[u]) for u in range(q.k)]
+        memo_differs += any(true_paths[u] != paths[u] for u in range(q.k) if u != root)
+        o = turboflux_orders_from_definition(q, root, par, paths)
+        bfs = [root] + [x for x in order if x != root]
+        order_differs += any(oo[2:] != [x for x in bfs if x not in oo[:2]] for oo in o)
+    assert memo_differs > 0 and order_differs > 0
