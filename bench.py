@@ -117,7 +117,8 @@ def main() -> int:
                     help="served legs: device bytes the engine plus in-flight batches may "
                          "hold (0 = 90%% of the device memory)")
     ap.add_argument("--also", default="parallel_symbi,parallel_turboflux",
-                    help="other algorithms timed on the same batches (comma list, '' for none)")
+                    help="other algorithms timed on the same batches (comma list, '' for none; "
+                         "name:ref runs the plugin with the reference's matching orders)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy first (HBM counter calibration)")
     args = ap.parse_args()
@@ -243,10 +244,12 @@ def main() -> int:
     # counts equal to this run's
     others = {}
     for algo in [a for a in args.also.split(",") if a and a != args.algo]:
-        v, c = batched_only(algo, w, queries, ups, args)
+        v, c, cpu_ = batched_only(algo, w, queries, ups, args)
         same = bool(np.array_equal(c, counts))
-        others[algo] = {"updates_per_s": round(v, 1), "counts_equal": same}
-        log(f"{algo}: {v:.0f} updates/s, counts equal to {args.algo}: {same}")
+        others[algo] = {"updates_per_s": round(v, 1), "counts_equal": same,
+                        "candidates_per_update": round(cpu_, 1)}
+        log(f"{algo}: {v:.0f} updates/s, {cpu_:.0f} candidates per update, "
+            f"counts equal to {args.algo}: {same}")
 
     # roofline of the search kernel: algorithmic bytes per launch over its
     # average launch duration (HIP events on the engine stream)
@@ -265,6 +268,7 @@ def main() -> int:
             "kernel": "k_search", "avg_launch_us": round(avg_ns / 1e3, 2),
             "alg_bytes_per_launch": int(per_launch_bytes), "launches": int(launches),
             "candidates_per_launch": int(cands / max(launches, 1)),
+            "candidates_per_update": round(cands / max(n_done, 1), 1),
             "traffic_over_alg_bytes": (round(traffic / per_launch_bytes, 4)
                                        if traffic and per_launch_bytes else None),
             "l2": l2,
@@ -363,15 +367,19 @@ def serve_legs(w, queries, args, batched_rate, tuned, batched_counts, n_batched)
     return out
 
 
-def batched_only(algo, w, queries, ups, args):
-    """Batched rate of another plugin on the same warm-up and timed batches;
-    returns (updates/s, counts of every batch)."""
+def batched_only(spec, w, queries, ups, args):
+    """Batched rate of another plugin on the same warm-up and timed batches
+    (spec "name" or "name:ref", the latter with the reference's matching
+    orders: for parallel_symbi its search-time order rule); returns
+    (updates/s, counts of every batch, candidate entries tested per timed
+    update)."""
     from paracosm_amd.engine import Matching
+    algo, _, mode = spec.partition(":")
     nq = len(queries)
     n_total = len(ups)
     m = Matching(algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
                  max_batch=args.batch, split_chunk=args.split_chunk,
-                 spill_budget=args.spill_budget)
+                 spill_budget=args.spill_budget, reference_orders=(mode == "ref"))
     for q in queries[1:]:
         m.register_query(q)
     m.Preprocessing()
@@ -387,15 +395,19 @@ def batched_only(algo, w, queries, ups, args):
     for i in range(args.warmup):
         step(i)
     m.sync()
+    c0 = m.stats()["candidates"]
     t = time.perf_counter()
     done = 0
     for i in range(args.warmup, args.warmup + args.steps):
         done += step(i)
+        if i % 5 == 0:
+            log(f"{spec}: step {i}")
     m.sync()
     elapsed = time.perf_counter() - t
+    cands = m.stats()["candidates"] - c0
     counts = d_counts.download(np.zeros(n_total * nq, np.uint64)).reshape(n_total, nq)
     m.close()
-    return done / elapsed, counts
+    return done / elapsed, counts, cands / max(done, 1)
 
 
 def ingest_rate(m, w, reps=5):

@@ -93,8 +93,17 @@ typedef struct csm_opts {
   uint32_t spill_budget;    /* DFS loop iterations before a search task
                                hands the rest of its subtree to the next
                                round (0 = default)                         */
-  uint32_t reference_orders; /* 1: search in the GraphFlow plugin's matching
-                               orders (graphflow.cpp:36-160); 0 (default):
+  uint32_t reference_orders; /* 1: search in the reference plugin's own
+                               matching orders: parallel_graphflow the
+                               GraphFlow rule (graphflow.cpp:36-160),
+                               parallel_turboflux TurboFlux's path-count rule
+                               (parallel_turboflux.cpp:533-700), and
+                               parallel_symbi SymBi's search-time rule (the
+                               next vertex picked at every step by least
+                               n2 count, non-isolated first,
+                               parrallel_symbi.cpp:4231-4262; its seeds skip
+                               the neighbourhood certificate and its batches
+                               are not replayed as HIP graphs).  0 (default):
                                the same seeds with positions 2..k-1 chosen by
                                estimated candidate counts from the data
                                graph's label statistics.  Counts are the same

@@ -23,6 +23,9 @@
 // graphs (pos = |Emb(G+e)| - |Emb(G)|), closed forms (triangles, 2-paths,
 // stars), the telescoping identity against InitialMatching, GraphFlow == SymBi
 // per update, and the SymBi index fix-point (incremental == from scratch).
+// SymBi's search-time order rule (symbi.cpp:474-506) and its per-update work
+// (orc_symbi_work) are pinned by hand-worked cases in
+// tests/test_oracle_order_rule.py.
 //
 // The DCS lists of SymBi are not stored: DCS[e(u->u')][v] is exactly the set of
 // neighbours w of v with label L(u') reached over an edge labelled l(u,u')
@@ -475,6 +478,9 @@ struct SymBi {
   std::vector<std::vector<uint32_t>> n1, n2, np1, nc2;
   std::deque<std::pair<uint32_t, uint32_t>> Q1, Q2;
   std::vector<uint8_t> visited;
+  // search work: DCS entries the searches walked (one per candidate of the
+  // vertex the ExtendableVertex rule picked, at every node of the search tree)
+  uint64_t work = 0;
   // updated edge and the (directed query edge, from-vertex) pairs whose DCS
   // lists currently hold it (insert) / no longer hold it (delete)
   uint32_t pe_a = NOT_EXIST, pe_b = NOT_EXIST;
@@ -796,6 +802,7 @@ struct SymBi {
     ext[u] = Ext();
     std::vector<uint32_t> cands;
     dcs_for(u_min, u, m[u_min], [&](uint32_t w) { cands.push_back(w); });
+    work += cands.size();
     for (uint32_t v : cands) {
       if (d2[u][v] == 0) continue;
       bool ok = true;
@@ -1163,6 +1170,30 @@ int orc_run(const char* algo, const orc_graph* data, const orc_graph* query, siz
     return finish(0);
   }
   return -1;
+}
+
+// Per-update search work of sequential SymBi (symbi.cpp:471-587, the
+// ExtendableVertex order rule of :474-506): the DCS entries its search walks
+// for each update (the entries of DCS[u_min -> u][m(u_min)] at every node of
+// the search tree, valid or not), and the per-update match counts.
+// Returns 0, -2 deletion of an absent edge (at *fail_at).
+int orc_symbi_work(const orc_graph* data, const orc_graph* query, size_t n, const uint8_t* add,
+                   const uint32_t* v1, const uint32_t* v2, const uint32_t* lab, int homo,
+                   uint64_t* work, uint64_t* count, size_t* fail_at) {
+  Graph d = data->g;
+  const Graph& q = query->g;
+  Opts o;
+  o.homo = homo != 0;
+  SymBi sb(q, d, o);
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t c = 0, h = 0;
+    sb.work = 0;
+    int rc = add[i] ? sb.add_edge(v1[i], v2[i], lab[i], c, h) : sb.remove_edge(v1[i], v2[i], c, h);
+    if (rc) { if (fail_at) *fail_at = i; return rc; }
+    work[i] = sb.work;
+    count[i] = c;
+  }
+  return 0;
 }
 
 // Sequential GraphFlow over a stream with a per-update result cap and time

@@ -64,6 +64,10 @@ def lib():
         L.orc_graphflow_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p]
         L.orc_turboflux_orders.restype = ctypes.c_int
         L.orc_turboflux_orders.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u32p, _u32p]
+        L.orc_symbi_work.restype = ctypes.c_int
+        L.orc_symbi_work.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _u8p,
+                                     _u32p, _u32p, _u32p, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
 
@@ -155,6 +159,25 @@ def graphflow_limited(data: OGraph, query: OGraph, add, v1, v2, lab, *, limit: i
     if rc == -2:
         raise RuntimeError("deletion error")
     return cnt, unf.astype(bool)
+
+
+def symbi_work(data: OGraph, query: OGraph, add, v1, v2, lab, *, homo: bool = False):
+    """Sequential SymBi over the stream: (work, counts) per update, where work
+    is the number of DCS entries its searches walked under the
+    ExtendableVertex order rule (see orc_symbi_work)."""
+    add = np.ascontiguousarray(np.asarray(add, dtype=np.uint8))
+    v1, v2, lab = _u32(v1), _u32(v2), _u32(lab)
+    n = len(v1)
+    work = np.zeros(n, np.uint64)
+    cnt = np.zeros(n, np.uint64)
+    fail = ctypes.c_size_t(0)
+    rc = lib().orc_symbi_work(data.h, query.h, n, add, v1, v2, lab, int(homo), work.ctypes.data,
+                              cnt.ctypes.data, ctypes.byref(fail))
+    if rc == -2:
+        raise RuntimeError(f"deletion error at update {fail.value}")
+    if rc != 0:
+        raise RuntimeError(f"oracle returned {rc}")
+    return work, cnt
 
 
 def graphflow_orders(data: OGraph, query: OGraph, k: int, ne: int) -> np.ndarray:

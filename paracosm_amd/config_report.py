@@ -28,6 +28,17 @@ fraction of batches it completed within the limit.
 
     python -m paracosm_amd.config_report --config c4 --mode serve --time-limit 0.2
 
+--mode orders: the same admitted queries and stream sample under three
+matching-order rules: the engine's estimated-cost orders (parallel_symbi),
+the GraphFlow plugin's static orders (parallel_graphflow with reference
+orders, graphflow.cpp:36-160) and SymBi's search-time rule (parallel_symbi
+with reference orders, parrallel_symbi.cpp:4231-4262).  Batches of --batch
+updates, each under --time-limit seconds.  Reported per rule: updates/s,
+updates complete, and candidate entries tested per update over the updates
+every rule completed (whose counts must agree).
+
+    python -m paracosm_amd.config_report --config c2 --mode orders --sample 4000 --batch 1000 --time-limit 20
+
 Prints one JSON line; progress goes to stderr.
 """
 from __future__ import annotations
@@ -57,7 +68,7 @@ def main(argv=None) -> int:
     ap.add_argument("--sample", type=int, default=1_000_000, help="stream updates run")
     ap.add_argument("--batch", type=int, default=62500)
     ap.add_argument("--scale", type=int, default=None)
-    ap.add_argument("--mode", default="rate", choices=("rate", "large", "serve"))
+    ap.add_argument("--mode", default="rate", choices=("rate", "large", "serve", "orders"))
     ap.add_argument("--loads", default="1e4,1e5,1e6,1e7",
                     help="serve: offered loads in updates/s")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
@@ -72,6 +83,8 @@ def main(argv=None) -> int:
         return large_queries(args)
     if args.mode == "serve":
         return serve_loads(args)
+    if args.mode == "orders":
+        return order_rules(args)
 
     t0 = time.time()
     d = workloads.build(args.config, seed=args.seed, scale=args.scale, progress=log)
@@ -216,6 +229,73 @@ def large_queries(args) -> int:
            "updates": n, "batch": args.batch, "time_limit_s_per_query": args.time_limit,
            "work_unit": "candidate entries tested by the search of one update",
            "sizes": sizes}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+ORDER_RULES = (
+    ("estimated_cost", "parallel_symbi", False),
+    ("graphflow_static", "parallel_graphflow", True),
+    ("symbi_search_time", "parallel_symbi", True),
+)
+
+
+def order_rules(args) -> int:
+    """Search effort and rate of the three order rules on one workload."""
+    from . import workloads
+    from .engine import USTAT_TIME_LIMIT, Matching, make_updates
+    t0 = time.time()
+    d = workloads.build(args.config, seed=args.seed, scale=args.scale, progress=log)
+    w = d.w
+    n = min(args.sample, w.num_updates)
+    log(f"workload {args.config}: {w.nv} vertices, {len(w.init_src)} initial edges; running {n}")
+    queries = workloads.admit(d, log=log)
+    log(f"admitted {len(queries)} queries in {time.time() - t0:.1f}s")
+    ups = make_updates(w.upd_add[:n], w.upd_v1[:n], w.upd_v2[:n], w.upd_label[:n])
+    res = {}
+    for name, algo, ref in ORDER_RULES:
+        m = Matching(algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                     max_batch=args.batch, want_work=True, reference_orders=ref)
+        for q in queries[1:]:
+            m.register_query(q)
+        m.Preprocessing()
+        work, done, cnts = [], [], []
+        t_all = 0.0
+        for s0 in range(0, n, args.batch):
+            kk = min(args.batch, n - s0)
+            m.set_time_limit(args.time_limit)
+            t = time.perf_counter()
+            pos, neg, _, _ = m.apply_batch(ups[s0:s0 + kk], partial_ok=True)
+            t_all += time.perf_counter() - t
+            st = m.update_status(kk)
+            work.append(m.update_work(kk))
+            done.append((st & USTAT_TIME_LIMIT) == 0)
+            cnts.append((pos + neg).sum(axis=1))
+            log(f"{name}: batch at {s0}: {time.perf_counter() - t:.3f}s, "
+                f"{int(done[-1].sum())}/{kk} complete")
+        m.close()
+        res[name] = {"algo": algo, "reference_orders": ref, "seconds": t_all,
+                     "work": np.concatenate(work), "done": np.concatenate(done),
+                     "counts": np.concatenate(cnts)}
+    all_done = np.logical_and.reduce([r["done"] for r in res.values()])
+    base = res[ORDER_RULES[0][0]]["counts"][all_done]
+    rules = {}
+    for name, r in res.items():
+        wk = r["work"][all_done]
+        rules[name] = {
+            "algo": r["algo"], "reference_orders": r["reference_orders"],
+            "updates_per_s": round(n / r["seconds"], 1) if r["seconds"] else None,
+            "updates_complete": int(r["done"].sum()),
+            "candidates_per_update": round(float(wk.mean()), 1) if wk.size else None,
+            "max_update_candidates": int(wk.max()) if wk.size else 0,
+            "counts_equal_estimated_cost": bool(np.array_equal(r["counts"][all_done], base)),
+        }
+    out = {"config": d.cfg.description, "mode": "orders", "seed": args.seed, "updates": n,
+           "batch": args.batch, "time_limit_s_per_batch": args.time_limit,
+           "queries": len(queries), "query_fingerprint": workloads.query_fingerprint(queries),
+           "updates_complete_in_all": int(all_done.sum()),
+           "work_unit": "candidate entries tested by the search of one update (all queries)",
+           "rules": rules}
     print(json.dumps(out), flush=True)
     return 0
 

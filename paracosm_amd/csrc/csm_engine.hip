@@ -26,6 +26,7 @@
 #include "csm_kernels.hpp"
 #include "csm_plan.hpp"
 #include "csm_symbi.hpp"
+#include "csm_dyn.hpp"
 #include "csm_parse.hpp"
 
 using namespace csm;
@@ -62,6 +63,9 @@ struct QueryInfo {
   // TurboFlux's explicit-path counts per query vertex (its order rule's
   // ranking; computed for parallel_turboflux with reference_orders)
   std::vector<uint32_t> tf_paths;
+  // n2 counter row of each directed query edge w -> u, at [w * k + u]
+  // (NONE: not an edge)
+  std::vector<uint32_t> n2row;
 };
 
 inline uint32_t bits_for(uint64_t n) {  // bits to hold values in [0, n)
@@ -165,6 +169,11 @@ struct csm_engine {
   uint32_t* d_tab_ent = nullptr;
   size_t d_tab_ent_cap = 0;
   bool preprocessed = false;
+  // parallel_symbi with reference orders: the next query vertex is chosen
+  // during the search (k_search_dyn), from these per-query tables
+  bool dyn = false;
+  DynQuery* d_dynq = nullptr;
+  size_t d_dynq_cap = 0;
 
   // SymBi index: rows of all queries; d_d2 aliases the d2 bitmap for the search
   uint32_t* d_d2 = nullptr;
@@ -504,8 +513,49 @@ void apply_plan_settings(csm_engine* e) {
   }
 }
 
+// Per-query tables of the search-time order rule (k_search_dyn).
+int upload_dyn(csm_engine* e) {
+  std::vector<DynQuery> h(e->queries.size());
+  for (size_t qi = 0; qi < e->queries.size(); ++qi) {
+    const QueryInfo& q = *e->queries[qi];
+    DynQuery& D = h[qi];
+    std::memset(&D, 0, sizeof D);
+    const uint32_t k = q.g.k;
+    D.k = k;
+    const bool have_rows = q.n2row.size() == (size_t)k * k && q.dag.parents.size() == k;
+    for (uint32_t u = 0; u < k; ++u) {
+      D.deg[u] = (uint32_t)q.g.nb[u].size();
+      D.d2row[u] = q.d2_base + u;
+      for (uint32_t w : q.g.nb[u]) {
+        D.nbm[u] |= 1u << w;
+        const uint32_t el = dense_of(e->el_map, q.g.elabel(w, u));
+        const uint32_t lu = q.dlab[u];
+        D.key[w][u] = (lu == NO_LABEL ? 0 : lu) * e->nel + (el == NO_LABEL ? 0 : el);
+        D.n2[w][u] = have_rows ? q.n2row[(size_t)w * k + u] : 0u;
+      }
+      if (have_rows)
+        for (uint32_t p : q.dag.parents[u]) D.parm[u] |= 1u << p;
+    }
+  }
+  if (!e->d_dynq || h.size() > e->d_dynq_cap) {
+    dfree(e, e->d_dynq);
+    const size_t cap = std::max<size_t>(8, h.size() * 2);
+    int rc = dalloc(e, &e->d_dynq, cap);
+    if (rc) return rc;
+    e->d_dynq_cap = cap;
+  }
+  if (!h.empty())
+    HIP_TRY(e, hipMemcpyAsync(e->d_dynq, h.data(), h.size() * sizeof(DynQuery),
+                              hipMemcpyHostToDevice, e->stream));
+  return CSM_OK;
+}
+
 int upload_plans(csm_engine* e) {
   apply_plan_settings(e);
+  if (e->dyn) {
+    int rc = upload_dyn(e);
+    if (rc) return rc;
+  }
   if (e->h_plans.size() > e->d_plans_cap) {
     dfree(e, e->d_plans);
     size_t cap = std::max<size_t>(64, e->h_plans.size() * 2);
@@ -964,12 +1014,42 @@ void launch_search(csm_engine* e, const SearchArgs& sa) {
 
 // One search launch in slot `slot` of a group of rounds, between that slot's
 // two events.
+template <int MK, bool H>
+void launch_dyn_mk(csm_engine* e, const DynArgs& da) {
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search_dyn<MK, H>,
+                                                     search_waves<MK>() * WAVE, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+  }
+  k_search_dyn<MK, H><<<e->num_cus * per_cu, search_waves<MK>() * WAVE, 0, e->stream>>>(da);
+}
+
+// the search-time order rule (parallel_symbi with reference orders)
+template <bool H>
+void launch_dyn(csm_engine* e, const SearchArgs& sa) {
+  DynArgs da;
+  da.s = sa;
+  da.dq = e->d_dynq;
+  da.n2 = e->d_sx_n2;
+  da.n2_stride = e->nv_cap;
+  if (e->max_k <= 8) launch_dyn_mk<8, H>(e, da);
+  else if (e->max_k <= 16) launch_dyn_mk<16, H>(e, da);
+  else launch_dyn_mk<MAXK, H>(e, da);
+}
+
 void launch_round(csm_engine* e, SearchArgs& sa, bool h, uint32_t slot) {
   const bool s = uses_index(e) && e->d_d2 != nullptr;
   sa.round_slot = slot;
   (void)hipEventRecord(e->ev_round[slot][0], e->stream);
-  if (h) { if (s) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa); }
-  else { if (s) launch_search<false, true>(e, sa); else launch_search<false, false>(e, sa); }
+  if (e->dyn && s) {
+    if (h) launch_dyn<true>(e, sa); else launch_dyn<false>(e, sa);
+  } else if (h) {
+    if (s) launch_search<true, true>(e, sa); else launch_search<true, false>(e, sa);
+  } else {
+    if (s) launch_search<false, true>(e, sa); else launch_search<false, false>(e, sa);
+  }
   (void)hipEventRecord(e->ev_round[slot][1], e->stream);
 }
 
@@ -1119,6 +1199,7 @@ void seeds(csm_engine* e, uint32_t n, unsigned long long* counts, unsigned long 
   a.ustatus = e->d_ustatus;
   a.lim = e->d_limits;
   a.enumerate = (hashes && e->d_emb) ? 1u : 0u;
+  a.dyn = e->dyn ? 1u : 0u;
   k_seeds<<<grid_for((uint64_t)n * WAVE, 256, 8192), 256, 0, e->stream>>>(a);
 }
 
@@ -1289,7 +1370,7 @@ constexpr uint32_t GRAPH_MAX_ROUNDS = 6;
 bool graphs_allowed(const csm_engine* e, uint32_t n) {
   static const bool off = getenv("CSM_NO_GRAPHS") != nullptr || getenv("CSM_TRACE_ROUNDS") != nullptr;
   return !off && !e->opts.no_graphs && n <= GRAPH_MAX_BATCH && e->deadline <= 0 &&
-         e->algo != ALGO_NONE && !e->opts.print_enum;
+         e->algo != ALGO_NONE && !e->opts.print_enum && !e->dyn;
 }
 
 // the bucket a batch of n updates is padded to
@@ -1558,6 +1639,9 @@ int symbi_prepare_query(csm_engine* e, QueryInfo& q) {
   uint32_t n2next = e->sx_n2rows;
   for (uint32_t u = 0; u < k; ++u)
     for (uint32_t w : g.nb[u]) n2id[u][w] = n2next++;
+  q.n2row.assign((size_t)k * k, NONE);
+  for (uint32_t u = 0; u < k; ++u)
+    for (uint32_t w = 0; w < k; ++w) q.n2row[(size_t)u * k + w] = n2id[u][w];
   std::vector<std::vector<uint32_t>> n1id(k, std::vector<uint32_t>(k, NONE));
   uint32_t n1next = e->sx_n1rows;
   for (uint32_t u = 0; u < k; ++u)
@@ -2199,7 +2283,10 @@ static csm_engine* create_engine(const char* algo, const csm_graph* query, const
   e->algo_name = algo;
   if (opts) e->opts = *opts;
   if (e->opts.max_batch == 0 || e->opts.max_batch > DEFAULT_MAX_BATCH) e->opts.max_batch = DEFAULT_MAX_BATCH;
-  e->tune_active = e->opts.auto_tuning != 0 && e->opts.split_chunk == 0 && e->opts.spill_budget == 0;
+  e->dyn = a == ALGO_SYMBI && e->opts.reference_orders != 0;
+  // the search-time order has no split or spill settings to tune
+  e->tune_active = e->opts.auto_tuning != 0 && e->opts.split_chunk == 0 &&
+                   e->opts.spill_budget == 0 && !e->dyn;
   if (e->tune_active)
     e->tune_cands = {{64, 1024}, {32, 1024}, {64, 512}, {64, 2048}, {128, 1024}};
   for (int c = 0; c < NSIZE_CLASSES; ++c) {

@@ -759,6 +759,10 @@ struct SeedArgs {
   uint32_t* ustatus;          // USTAT_CERTIFIED for updates no seed survives (nullable)
   const SearchLimits* lim;    // embedding rows of 2-vertex queries (enumerate != 0)
   uint32_t enumerate;
+  // seeds of the search-time order rule: every seed that passes d2 becomes a
+  // task (no neighbourhood certificate, and a task even when the static
+  // depth-2 segment is empty), so that the search tree is the reference's
+  uint32_t dyn;
 };
 
 // Emit the depth-2 seeds of the lanes with `have` set (one task each, cut into
@@ -847,7 +851,7 @@ __global__ void k_seeds(SeedArgs a) {
         // seed endpoints needs a candidate among their neighbours with its
         // label and edge label; an empty segment (in the pool, so at every
         // position of the batch) rules the seed out
-        for (uint32_t p = 2; ok && p < P->k; ++p)
+        for (uint32_t p = 2; ok && !a.dyn && p < P->k; ++p)
           for (uint32_t jj = 0; ok && jj < P->nb[p]; ++jj) {
             const uint32_t w = P->bw[p][jj];
             if ((w & 0xFFu) > 1u) continue;
@@ -891,6 +895,9 @@ __global__ void k_seeds(SeedArgs a) {
           if (best != 0 && best != NONE) {
             have = true;
             ch = chunking(best, P->chunk);
+          } else if (a.dyn) {
+            have = true;
+            ch = chunking(0u, P->chunk);
           }
         }
       }
