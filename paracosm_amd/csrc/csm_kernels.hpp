@@ -1602,20 +1602,36 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       PROF_ADD(6, uniform(popc64(ballot(lane < bn)) ? (uint32_t)wave_sum((unsigned long long)(lane < bn ? (ohi - olo) : 0u)) : 0u));
       S.pref[lane] = incl - size;
       if (lane == 0) S.pref[WAVE] = total;
+      // parent of each window entry: the parents whose leaves start inside
+      // the window mark their start in pmap (tagged with the window number,
+      // so a slot needs no clearing between windows), and each entry takes
+      // the last mark at or
+      // before it, or the parent that ran on from the previous window.  The
+      // last position's candidate buffer is free: it is never descended into.
+      uint32_t* pmap = S.cand[L1];
+      pmap[lane] = NONE;
       wave_sync();
       const uint32_t fsum_lo = jf != NONE ? S.plo[L1][jf] : 0u;
-      for (uint32_t b = 0; b < total; b += WAVE) {
+      const uint32_t pstart = incl - size;
+      const uint64_t upto_mask = lane == WAVE - 1 ? ~0ull : (2ull << lane) - 1ull;
+      uint32_t pcarry = 0;
+      for (uint32_t b = 0, wid = 0; b < total; b += WAVE, ++wid) {
         const uint32_t g = b + lane;
-        bool ok = g < total;
-        uint32_t w = 0, p = 0;
+        bool ok = g < total, need = false;
+        uint32_t w = 0, p;
+        // a mark is 0x800000 | (window mod 2^20) above the parent index
+        // (never NONE >> 8); the slots are cleared every 2^20 windows
+        const uint32_t tag = 0x800000u | (wid & 0xFFFFFu);
+        if ((wid & 0xFFFFFu) == 0 && wid) pmap[lane] = NONE;
+        if (lane < bn && size > 0 && pstart >= b && pstart - b < WAVE)
+          pmap[pstart - b] = (tag << 8) | lane;
+        wave_sync();
+        {
+          const uint64_t marks = ballot((pmap[lane] >> 8) == tag) & upto_mask;
+          p = marks ? (pmap[63 - __builtin_clzll(marks)] & 0xFFu) : pcarry;
+          pcarry = uniform(__shfl(p, WAVE - 1));
+        }
         if (ok) {
-          // parent p: last index with pref[p] <= g (pref is non-decreasing;
-          // the halvings depend on bn only)
-          for (uint32_t m = bn; m > 1;) {
-            const uint32_t half = m >> 1;
-            p = S.pref[p + half] <= g ? p + half : p;
-            m -= half;
-          }
           const uint32_t sp = S.src[p];
           const uint32_t off = g - S.pref[p];
           if ((sp & 1u) || !useF) {
@@ -1637,9 +1653,21 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
             if (ok && !a.homo) ok = w != mk3;
             if (ok && (sp & 1u)) ok = lds_contains(S.F, nF, w);
           }
-          if (ok && jp != NONE && !(sp & 1u)) {
-            uint32_t x = ((S.pbw[L1][jp] >> 8) << vbits) | w;
-            ok = contains(adj, ats, S.olo[p], S.ohi[p], x, VERSIONED && ((sp >> 1) & 1u), pos);
+          need = ok && jp != NONE && !(sp & 1u);
+        }
+        // leaves drawn from F or a fixed segment: join with the parent's own
+        // segment; a window whose joins all fall under one parent searches
+        // that segment with wave-uniform halvings
+        const uint64_t nm = ballot(need);
+        if (nm) {
+          const uint32_t x = ((S.pbw[L1][jp] >> 8) << vbits) | w;
+          const uint32_t p0 = uniform(__shfl(p, (int)__builtin_ctzll(nm)));
+          if (ballot(need && p != p0) == 0) {
+            const uint32_t lo0 = uniform(S.olo[p0]), hi0 = uniform(S.ohi[p0]);
+            const bool d0 = VERSIONED && ((uniform(S.src[p0]) >> 1) & 1u);
+            if (need) ok = contains_u(adj, ats, lo0, hi0 - lo0, x, d0, pos);
+          } else if (need) {
+            ok = contains(adj, ats, S.olo[p], S.ohi[p], x, VERSIONED && ((S.src[p] >> 1) & 1u), pos);
           }
         }
         cnt += popc64(ballot(ok));
