@@ -305,7 +305,8 @@ __global__ __launch_bounds__(search_waves<MK>() * WAVE) void k_search_dyn(DynArg
           jm &= jm - 1u;
           if (!ok) continue;
           const uint32_t x = (DQ->key[w][u] << vbits) | v;
-          ok = contains_u(adj, ats, S.jlo[d][w], S.jhi[d][w] - S.jlo[d][w], x, (jd >> w) & 1u, pos);
+          const uint32_t slo = uniform(S.jlo[d][w]);
+          ok = contains_u(adj, ats, slo, uniform(S.jhi[d][w]) - slo, x, (jd >> w) & 1u, pos);
         }
         const uint64_t m = ballot(ok);
         if (d == L1) {

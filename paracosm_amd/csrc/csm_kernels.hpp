@@ -1233,23 +1233,26 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     auto test = [&](uint32_t lv, uint32_t e, uint32_t end, uint32_t& vout) CSM_AINL -> bool {
       bool ok = e < end;
       uint32_t v = 0;
+      // the plan fields and cached segment bounds read below are the same
+      // for every lane: held in scalar registers
       if (ok) {
         v = adj[e] & vmask;
-        if (VERSIONED && S.udirty[lv]) ok = ts_visible(ats[e], pos);
+        if (VERSIONED && uniform(S.udirty[lv])) ok = ts_visible(ats[e], pos);
       }
       if (ok && !a.homo) {
         for (uint32_t i = 0; i < lv; ++i) ok = ok && (S.m[i] != v);
       }
-      if (SYMBI && ok) ok = d2_bit(a.d2, S.pd2[lv], a.d2_words, v);
+      if (SYMBI && ok) ok = d2_bit(a.d2, uniform(S.pd2[lv]), a.d2_words, v);
       const uint32_t nb = uniform(S.pnb[lv]);
-      const uint32_t jm = S.jmin[lv];
+      const uint32_t jm = uniform(S.jmin[lv]);
+      const uint32_t pdm = VERSIONED ? uniform(S.pdirty[lv]) : 0u;
       for (uint32_t j = 0; j < nb; ++j) {
         if (j == jm) continue;
+        const uint32_t w = uniform(S.pbw[lv][j]);
+        const uint32_t slo = uniform(S.plo[lv][j]), shi = uniform(S.phi[lv][j]);
         if (!ok) continue;
-        uint32_t w = S.pbw[lv][j];
-        uint32_t x = ((w >> 8) << vbits) | v;
-        bool vd = VERSIONED && ((S.pdirty[lv] >> j) & 1u);
-        ok = contains_u(adj, ats, S.plo[lv][j], S.phi[lv][j] - S.plo[lv][j], x, vd, pos);
+        const uint32_t x = ((w >> 8) << vbits) | v;
+        ok = contains_u(adj, ats, slo, shi - slo, x, (pdm >> j) & 1u, pos);
       }
       vout = v;
       return ok;
@@ -1268,7 +1271,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         uint32_t bw = S.pbw[L1][j];
         uint32_t x = ((bw >> 8) << vbits) | w;
         bool vd = VERSIONED && ((S.pdirty[L1] >> j) & 1u);
-        ok = contains_u(adj, ats, S.plo[L1][j], S.phi[L1][j] - S.plo[L1][j], x, vd, pos);
+        const uint32_t slo = uniform(S.plo[L1][j]);
+        ok = contains_u(adj, ats, slo, uniform(S.phi[L1][j]) - slo, x, vd, pos);
       }
       return ok;
     };
@@ -1339,7 +1343,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         if (kept) return lds_contains(S.F, C, mk3) ? C - 1u : C;
         const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
         const uint32_t x = ((S.pbw[L1][jf] >> 8) << vbits) | mk3;
-        const bool in = contains_u(adj, ats, S.plo[L1][jf], S.phi[L1][jf] - S.plo[L1][jf], x, fd,
+        const uint32_t slo = uniform(S.plo[L1][jf]);
+        const bool in = contains_u(adj, ats, slo, uniform(S.phi[L1][jf]) - slo, x, fd,
                                    pos) &&
                         last_fixed_ok(mk3, jf, L2 - 1);
         return in ? C - 1u : C;
@@ -1521,8 +1526,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     // all leaves under the bn parents now in cand[k-2]
     auto finish_parents = [&](uint32_t bn) CSM_AINL {
       PROF_BEGIN(fp);
-      const uint32_t jp = S.ljp, jf = S.ljf;
-      const uint32_t nF = S.lF;          // F in LDS (NONE: not built)
+      const uint32_t jp = uniform(S.ljp), jf = uniform(S.ljf);
+      const uint32_t nF = uniform(S.lF);   // F in LDS (NONE: not built)
       const bool useF = nF != NONE;
       const uint32_t mk3 = S.m[L2 - 1];
       uint32_t v = lane < bn ? S.cand[L2][lane] : 0u;
@@ -1536,7 +1541,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
             const bool fd = VERSIONED && ((S.pdirty[L1] >> jf) & 1u);
             uint32_t bw = S.pbw[L1][jf];
             uint32_t x = ((bw >> 8) << vbits) | v;
-            self = contains_u(adj, ats, S.plo[L1][jf], S.phi[L1][jf] - S.plo[L1][jf], x, fd, pos) &&
+            const uint32_t slo = uniform(S.plo[L1][jf]);
+            self = contains_u(adj, ats, slo, uniform(S.phi[L1][jf]) - slo, x, fd, pos) &&
                    last_fixed_ok(v, jf, L2);
           }
         }
