@@ -286,7 +286,10 @@ int csm_index_dump(csm_engine* e, uint32_t qid, uint32_t what, uint32_t* out, si
  * (Parallel_TurboFlux::GenerateMatchingOrder, parallel_turboflux.cpp:533-700,
  * which the reference prints with --print-prep), and paths (k entries,
  * nullable) receives its explicit-path counts per query vertex
- * (CountDownwards, :469-491); otherwise paths are 0. */
+ * (CountDownwards, :469-491); otherwise paths are 0.  For parallel_symbi
+ * with reference_orders only positions 0 and 1 (the seed edge) are used:
+ * the search picks every later vertex as it goes (SymBi's rule,
+ * parrallel_symbi.cpp:4231-4262). */
 int csm_query_orders(csm_engine* e, uint32_t qid, uint32_t* out, size_t n, uint32_t* paths,
                      size_t* n_orders);
 
