@@ -1060,6 +1060,9 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
   const uint32_t* __restrict__ adj = a.g.adj;
   const uint32_t* __restrict__ ats = a.g.ats;
   const uint32_t vbits = a.g.vbits, vmask = a.g.vmask;
+  // n_tasks and n_cands are the same in every lane; n_bytes (algorithmic
+  // bytes of the segments a DFS node reads) is kept per lane and summed once
+  // at the end
   unsigned long long n_tasks = 0, n_cands = 0, n_bytes = 0;
 #ifdef CSM_PHASE_PROF
   unsigned long long prof_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1321,7 +1324,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       uint32_t jf = (uint32_t)(mn >> 32) == 0xFFFFFFFFu ? NONE : uniform((uint32_t)mn);
       uint32_t fsize = uniform((uint32_t)(mn >> 32));
       const uint32_t dmask = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
-      n_bytes += 4ull * wave_sum((unsigned long long)(my_size == 0xFFFFFFFFu ? 0u : my_size));
+      n_bytes += 4ull * (my_size == 0xFFFFFFFFu ? 0u : my_size);
       wave_sync();
       if (lane == 0) {
         S.ljp = jp;
@@ -1427,8 +1430,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       uint32_t jm = uniform((uint32_t)mn);
       uint32_t msz = uniform((uint32_t)(mn >> 32));
       uint32_t dmask = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
-      unsigned long long seg_total = wave_sum((unsigned long long)(lane < nb ? my_size : 0u));
-      n_bytes += 4ull * seg_total;
+      n_bytes += 4ull * (lane < nb ? my_size : 0u);
       // the cached slots' dirty bits and F's validity, before any early return
       S.pdirty[lv] = dmask;
       if (lv < L2) S.fvalid = 0;   // the prefix that F excludes has changed
@@ -1547,7 +1549,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
           }
         }
         cnt += (unsigned long long)bn * S.lCeff - popc64(ballot(self));
-        n_bytes += 4ull * bn * S.lfsize;
+        if (lane == 0) n_bytes += 4ull * bn * S.lfsize;
         PROF_END(fp, 5);
         return;
       }
@@ -1694,7 +1696,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
             });
         }
       }
-      if (jp == NONE) n_bytes += 4ull * bn * S.lfsize;
+      if (jp == NONE && lane == 0) n_bytes += 4ull * bn * S.lfsize;
 #ifdef CSM_PHASE_PROF
       PROF_ADD(useF ? 7 : 4, __builtin_readcyclecounter() - prof_t0_fp);
 #endif
@@ -1851,6 +1853,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
     if (lane == 0 && (a.lflags & LIM_WORK) && n_cands > S.cands0)
       atomicAdd(&a.lim->uwork[t], n_cands - S.cands0);
   }
+  n_bytes = wave_sum(n_bytes);
   if (lane == 0) {
     // a wave of an empty round adds nothing (and skips three contended atomics)
     if (n_tasks) {
