@@ -113,6 +113,9 @@ def main() -> int:
     ap.add_argument("--split-chunk", type=int, default=0)
     ap.add_argument("--spill-budget", type=int, default=0)
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--serve-budget", type=int, default=0,
+                    help="served legs: spill budget of the served engines (0 = the one the "
+                         "autotuner chose for the batched run)")
     ap.add_argument("--hbm-budget", type=int, default=0,
                     help="served legs: device bytes the engine plus in-flight batches may "
                          "hold (0 = 90%% of the device memory)")
@@ -337,7 +340,8 @@ def serve_legs(w, queries, args, batched_rate, tuned, batched_counts, n_batched)
         if n_srv < 1:
             continue
         m = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
-                     max_batch=args.batch, split_chunk=tuned[0], spill_budget=tuned[1])
+                     max_batch=args.batch, split_chunk=tuned[0],
+                     spill_budget=args.serve_budget or tuned[1])
         for q in queries[1:]:
             m.register_query(q)
         m.Preprocessing()

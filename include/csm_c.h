@@ -90,9 +90,13 @@ typedef struct csm_opts {
   uint64_t match_budget;    /* per query and batch: stop searching a query
                                once its matches in the batch exceed this
                                (0 = unbounded); see csm_query_status       */
-  uint32_t spill_budget;    /* DFS loop iterations before a search task
-                               hands the rest of its subtree to the next
-                               round (0 = default)                         */
+  uint32_t spill_budget;    /* work units (loop iterations, 64-entry
+                               walks) before a search task hands the rest
+                               of its subtree to the next round (0 =
+                               autotuned).  A budget of 256 or more is
+                               quartered for batches of up to 2,048
+                               updates and halved up to 32,768 (not below
+                               64): shorter rounds for small batches     */
   uint32_t reference_orders; /* 1: search in the reference plugin's own
                                matching orders: parallel_graphflow the
                                GraphFlow rule (graphflow.cpp:36-160),

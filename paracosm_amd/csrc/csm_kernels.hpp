@@ -939,6 +939,10 @@ struct SearchArgs {
   uint32_t lflags;
   const SearchLimits* __restrict__ lim;
   uint32_t round_slot;        // this launch's slot in Ctrl::round_busy
+  // small batches leave the GPU partly idle, so their rounds are kept short:
+  // plan budgets of at least 256 units are divided by 2^budget_shift (not
+  // below 64)
+  uint32_t budget_shift;
 };
 
 // real-time counter ticks (100 MHz) between two polls of the host abort word
@@ -1175,7 +1179,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         S.pnb[lane] = P->nb[lane];
         if (lane == 0) {
           S.pchunk = P->chunk;
-          S.pbudget = P->budget;
+          const uint32_t b = P->budget;
+          S.pbudget = b >= 256u ? max(64u, b >> a.budget_shift) : b;
         }
         S.pd2[lane] = P->d2row[lane];
         S.pqv[lane] = P->qv[lane];
