@@ -240,6 +240,19 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) {
   return x;
 }
 
+// (least x, its lane) over lanes [0, n) (ties to the lower lane; value
+// 0xFFFFFFFF and lane 0 when every value is 0xFFFFFFFF or n == 0): the same
+// as wave_min_u64((x << 32) | lane) when the lanes from n on hold
+// 0xFFFFFFFF, with n scalar reads instead of a 64-lane reduction
+__device__ __forceinline__ uint64_t lanes_min_u32(uint32_t x, uint32_t n) {
+  uint32_t best = 0xFFFFFFFFu, bl = 0;
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t v = __builtin_amdgcn_readlane(x, j);
+    if (v < best) { best = v; bl = j; }
+  }
+  return ((uint64_t)best << 32) | bl;
+}
+
 __device__ __forceinline__ bool d2_bit(const uint32_t* __restrict__ d2, uint32_t row,
                                        uint32_t words_per_row, uint32_t v) {
   return (d2[(size_t)row * words_per_row + (v >> 5)] >> (v & 31)) & 1u;
@@ -1325,7 +1338,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       const bool reuse = S.fvalid != 0 && ballot(miss != 0) == 0;
       uint64_t pm = ballot(is_par != 0);
       uint32_t jp = pm ? (uint32_t)__builtin_ctzll(pm) : NONE;
-      uint64_t mn = wave_min_u64(((uint64_t)my_size << 32) | lane);
+      uint64_t mn = lanes_min_u32(my_size, nb);
       uint32_t jf = (uint32_t)(mn >> 32) == 0xFFFFFFFFu ? NONE : uniform((uint32_t)mn);
       uint32_t fsize = uniform((uint32_t)(mn >> 32));
       const uint32_t dmask = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
@@ -1429,9 +1442,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         }
         my_size = my_hi - my_lo;
       }
-      uint64_t mk = ((uint64_t)my_size << 32) | lane;
       uint64_t mn = (forced_j >= 0) ? (((uint64_t)__shfl(my_size, forced_j) << 32) | (uint32_t)forced_j)
-                                    : wave_min_u64(mk);
+                                    : lanes_min_u32(my_size, nb);
       uint32_t jm = uniform((uint32_t)mn);
       uint32_t msz = uniform((uint32_t)(mn >> 32));
       uint32_t dmask = VERSIONED ? (uint32_t)ballot(my_dirty != 0) : 0u;
