@@ -1113,7 +1113,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       S.last_poll = now;
       S.aborted = v;
     }
-    return uniform(__shfl(v, 0)) != 0;
+    return __builtin_amdgcn_readlane(v, 0) != 0;
   };
   while (true) {
     PROF_BEGIN(fetch);
@@ -1130,14 +1130,14 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         uint32_t lo, hi;
         if (XCD_REGIONS && a.xregions) {
           lo = y * (a.chunk_cap / NXCD);
-          hi = lo + uniform(__shfl(xn_lane, y));
+          hi = lo + __builtin_amdgcn_readlane(xn_lane, y);
         } else {
           lo = (uint32_t)((uint64_t)nch * y / NXCD);
           hi = (uint32_t)((uint64_t)nch * (y + 1) / NXCD);
         }
         uint32_t g = 0;
         if (lane == 0 && lo < hi) g = atomicAdd(&a.ctrl->xhead[y * XHEAD_STRIDE], grab);
-        g = uniform(__shfl(g, 0));
+        g = __builtin_amdgcn_readlane(g, 0);
         if (lo < hi && g < hi - lo) {
           c_next = lo + g;
           c_end = hi - c_next < grab ? hi : c_next + grab;
@@ -1150,7 +1150,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
 #else
       uint32_t c0 = 0;
       if (lane == 0) c0 = atomicAdd(&a.ctrl->chunk_head, grab);
-      c_next = uniform(__shfl(c0, 0));
+      c_next = __builtin_amdgcn_readlane(c0, 0);
       c_end = c_next + grab;
 #endif
     }
@@ -1183,7 +1183,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       if (lane == 0)
         full = __hip_atomic_load(&a.counts[(size_t)t * a.nq + q], __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT) >= a.lim->ucap;
-      if (uniform(__shfl(full, 0))) continue;
+      if (__builtin_amdgcn_readlane(full, 0)) continue;
     }
     ++n_tasks;
     if ((a.lflags & LIM_WORK) && lane == 0) S.cands0 = n_cands;
@@ -1242,7 +1242,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
                 : __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           capped = tot_u >= a.lim->ucap;
         }
-        if (!final_ && uniform(__shfl(capped, 0))) stop = true;
+        if (!final_ && __builtin_amdgcn_readlane(capped, 0)) stop = true;
       }
       flushed = cnt;
       if (!final_ && a.qabort)
@@ -1384,7 +1384,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         // C, the number of valid leaves shared by every parent, both without
         // the injectivity test against position k-3, so that they hold for
         // every entry of position k-2 under the same fixed vertices
-        const uint32_t lo = uniform(__shfl(my_lo, jf)), hi = uniform(__shfl(my_hi, jf));
+        const uint32_t lo = __builtin_amdgcn_readlane(my_lo, jf), hi = __builtin_amdgcn_readlane(my_hi, jf);
         const bool fd = VERSIONED && ((dmask >> jf) & 1u);
         const bool keep = fsize <= FCAP;
         const uint32_t ninj = L2 - 1;
@@ -1442,7 +1442,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         }
         my_size = my_hi - my_lo;
       }
-      uint64_t mn = (forced_j >= 0) ? (((uint64_t)__shfl(my_size, forced_j) << 32) | (uint32_t)forced_j)
+      uint64_t mn = (forced_j >= 0) ? (((uint64_t)__builtin_amdgcn_readlane(my_size, forced_j) << 32) | (uint32_t)forced_j)
                                     : lanes_min_u32(my_size, nb);
       uint32_t jm = uniform((uint32_t)mn);
       uint32_t msz = uniform((uint32_t)(mn >> 32));
@@ -1453,7 +1453,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
       if (lv < L2) S.fvalid = 0;   // the prefix that F excludes has changed
       wave_sync();
       if (msz == 0) return false;
-      uint32_t lo = uniform(__shfl(my_lo, jm)), hi = uniform(__shfl(my_hi, jm));
+      uint32_t lo = __builtin_amdgcn_readlane(my_lo, jm), hi = __builtin_amdgcn_readlane(my_hi, jm);
       lo = lo > rlo ? lo : rlo;
       hi = hi < rhi ? hi : rhi;
       S.cur[lv] = lo;
@@ -1481,8 +1481,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
           cb = atomicAdd(&a.ctrl->xspill_chunks[xcd * XHEAD_STRIDE], n_c);
           atomicAdd(&a.ctrl->nspill, n_t);
         }
-        tb = uniform(__shfl(tb, 0));
-        cb = uniform(__shfl(cb, 0));
+        tb = __builtin_amdgcn_readlane(tb, 0);
+        cb = __builtin_amdgcn_readlane(cb, 0);
         const uint32_t tb0 = xcd * tcap, cb0 = xcd * ccap;
         if ((uint64_t)tb + n_t <= tcap && (uint64_t)cb + n_c <= ccap) {
           tb += tb0;
@@ -1497,8 +1497,8 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         tb = atomicAdd(&a.ctrl->nspill, n_t);
         cb = atomicAdd(&a.ctrl->nspill_chunks, n_c);
       }
-      tb = uniform(__shfl(tb, 0));
-      cb = uniform(__shfl(cb, 0));
+      tb = __builtin_amdgcn_readlane(tb, 0);
+      cb = __builtin_amdgcn_readlane(cb, 0);
       if ((uint64_t)tb + n_t <= a.spill_cap && (uint64_t)cb + n_c <= a.spill_chunk_cap) return true;
       for (uint64_t id = (uint64_t)cb + lane; id < (uint64_t)cb + n_c && id < a.spill_chunk_cap; id += WAVE)
         a.spill_owner[id] = NONE;
@@ -1601,9 +1601,9 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         while (big) {
           const uint32_t p = (uint32_t)__builtin_ctzll(big);
           big &= big - 1;
-          const bool own = (uniform(__shfl(src, p)) & 1u) != 0;
-          const uint32_t lo = own ? uniform(__shfl(olo, p)) : S.plo[L1][jf];
-          const uint32_t hi = own ? uniform(__shfl(ohi, p)) : S.phi[L1][jf];
+          const bool own = (__builtin_amdgcn_readlane(src, p) & 1u) != 0;
+          const uint32_t lo = own ? __builtin_amdgcn_readlane(olo, p) : S.plo[L1][jf];
+          const uint32_t hi = own ? __builtin_amdgcn_readlane(ohi, p) : S.phi[L1][jf];
           if (!spill_leaves(S.cand[L2][p], own ? jp : jf, lo, hi)) {
             may_spill = false;
             break;
@@ -1618,7 +1618,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         uint32_t y = __shfl_up(incl, o);
         if ((int)lane >= o) incl += y;
       }
-      const uint32_t total = uniform(__shfl(incl, WAVE - 1));
+      const uint32_t total = __builtin_amdgcn_readlane(incl, WAVE - 1);
       work += total / WAVE;
       PROF_ADD(0, total);
       PROF_ADD(1, bn);
@@ -1654,7 +1654,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         {
           const uint64_t marks = ballot((pmap[lane] >> 8) == tag) & upto_mask;
           p = marks ? (pmap[63 - __builtin_clzll(marks)] & 0xFFu) : pcarry;
-          pcarry = uniform(__shfl(p, WAVE - 1));
+          pcarry = __builtin_amdgcn_readlane(p, WAVE - 1);
         }
         if (ok) {
           const uint32_t sp = S.src[p];
@@ -1686,7 +1686,7 @@ __attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 
         const uint64_t nm = ballot(need);
         if (nm) {
           const uint32_t x = ((S.pbw[L1][jp] >> 8) << vbits) | w;
-          const uint32_t p0 = uniform(__shfl(p, (int)__builtin_ctzll(nm)));
+          const uint32_t p0 = __builtin_amdgcn_readlane(p, (int)__builtin_ctzll(nm));
           if (ballot(need && p != p0) == 0) {
             const uint32_t lo0 = uniform(S.olo[p0]), hi0 = uniform(S.ohi[p0]);
             const bool d0 = VERSIONED && ((uniform(S.src[p0]) >> 1) & 1u);
