@@ -78,6 +78,8 @@ def main(argv=None) -> int:
                     help="large: seconds per query for the whole stream sample")
     ap.add_argument("--per-size", type=int, default=0,
                     help="large: queries per size (0 = the config's)")
+    ap.add_argument("--sizes", default="",
+                    help="large: query sizes to run (comma list; default: the config's)")
     args = ap.parse_args(argv)
     if args.mode == "large":
         return large_queries(args)
@@ -175,7 +177,8 @@ def large_queries(args) -> int:
     ups = make_updates(w.upd_add[:n], w.upd_v1[:n], w.upd_v2[:n], w.upd_label[:n])
     per = args.per_size or d.cfg.per_size
     sizes = {}
-    for k in d.cfg.query_sizes:
+    run_sizes = [int(x) for x in args.sizes.split(",") if x] or list(d.cfg.query_sizes)
+    for k in run_sizes:
         qs = workloads.candidates(d, k, None, per, 0)
         rows = []
         work_all = []

@@ -1011,6 +1011,7 @@ void launch_search(csm_engine* e, const SearchArgs& sa) {
   else if (e->max_k <= 10) launch_search_mk<10, H, S>(e, sa);
   else if (e->max_k <= 12) launch_search_mk<12, H, S>(e, sa);
   else if (e->max_k <= 16) launch_search_mk<16, H, S>(e, sa);
+  else if (e->max_k <= 24) launch_search_mk<24, H, S>(e, sa);
   else launch_search_mk<MAXK, H, S>(e, sa);
 }
 

@@ -1061,7 +1061,7 @@ constexpr bool XCD_REGIONS = CSM_XCD_REGIONS && CSM_XCD_SPLIT;
 #endif
 template <int MK, bool VERSIONED, bool HASH, bool SYMBI>
 __global__ __launch_bounds__(search_waves<MK>() * WAVE)
-__attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 ? 5 : MK <= 12 ? 4 : MK <= 16 ? 3 : 1, 8))) void k_search(SearchArgs a) {
+__attribute__((amdgpu_waves_per_eu(MK <= 8 ? CSM_SEARCH_WAVES_PER_EU : MK <= 10 ? 5 : MK <= 12 ? 4 : MK <= 16 ? 3 : MK <= 24 ? 2 : 1, 8))) void k_search(SearchArgs a) {
   __shared__ WaveState<MK> S_all[search_waves<MK>()];
   const uint32_t lane = lane_id();
   const uint64_t lt_mask = (1ull << lane) - 1ull;

@@ -387,12 +387,12 @@ def test_fixed_segment_beyond_the_leaf_filter(oracle_lib, budget, algo):
     np.testing.assert_array_equal(neg[:, 0], ref.neg)
 
 
-@pytest.mark.parametrize("k", [10, 12, 14, 17, 24])
+@pytest.mark.parametrize("k", [10, 12, 14, 17, 24, 28])
 @pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
 def test_large_queries(oracle_lib, k, algo):
     """Queries beyond 8 vertices run the larger LDS layouts of the search
-    (10, 12, 16 and 32 vertices); a small spill budget also cuts their deep
-    prefixes into tasks."""
+    (10, 12, 16, 24 and 32 vertices); a small spill budget also cuts their
+    deep prefixes into tasks."""
     from paracosm_amd.engine import make_updates
     for seed in (0, 1):
         vl, init, ups, q = walk_case(700 + 10 * k + seed, k)
