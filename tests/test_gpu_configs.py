@@ -261,3 +261,79 @@ def test_c3_large_queries_capped_prefix(oracle_lib):
             np.testing.assert_array_equal(pos[both, j], cnt[both])
             compared += int(both.sum())
     assert compared >= n
+
+
+@pytest.mark.timeout(900)
+def test_c4_standing_queries_capped_prefix_and_served(oracle_lib):
+    """Config 4's standing queries (scale-21 graph, 70/30 mixed stream, random
+    walks of 6 to 16 vertices, no admission rule).  Every other one of the 64
+    queries (all sizes) on one engine, with the per-update result cap (10^5,
+    count = min(exact, cap)), over the first 200 updates in calls of 20 with a
+    3 s time limit each: bit-exact against the capped oracle on every update
+    that both finish within their limits (oracle 1 s per update, 15 s per
+    query).  Some updates of these queries need far longer than the limits
+    (search trees of 10^10 entries and more), so a call that holds one is cut
+    and its updates are left out.  The same updates are then served as
+    Poisson requests through the continuous-batching server, and every
+    served count that both runs finished equals the batched one."""
+    from paracosm_amd import workloads
+    from paracosm_amd.engine import USTAT_TIME_LIMIT, Matching, make_updates
+    from paracosm_amd.serve import Server, poisson_requests
+    O = oracle_lib
+    t0 = time.time()
+    d = workloads.build("c4", seed=1)
+    w = d.w
+    qs = []
+    for k in d.cfg.query_sizes:
+        qs += workloads.candidates(d, k, None, d.cfg.per_size, 0)
+    qs = qs[:64][::2]
+    _log(f"c4: built in {time.time() - t0:.1f}s, {len(qs)} queries, k = {[q.k for q in qs]}")
+    n, cap, part, limit = 200, 100_000, 20, 3.0
+    ups = make_updates(w.upd_add[:n], w.upd_v1[:n], w.upd_v2[:n], w.upd_label[:n])
+
+    def engine():
+        m = Matching("parallel_graphflow", qs[0], w.vlabels, w.init_src, w.init_dst,
+                     w.init_elab, max_results=cap, time_limit_s=limit)
+        for q in qs[1:]:
+            m.register_query(q)
+        m.Preprocessing()
+        return m
+
+    m = engine()
+    t0 = time.time()
+    counts = np.zeros((n, len(qs)), np.uint64)
+    done = np.zeros(n, bool)
+    for s0 in range(0, n, part):
+        pos, neg, _, _ = m.apply_batch(ups[s0:s0 + part], partial_ok=True)
+        counts[s0:s0 + part] = pos + neg
+        done[s0:s0 + part] = (m.update_status(part) & USTAT_TIME_LIMIT) == 0
+    m.close()
+    _log(f"c4: engine {time.time() - t0:.2f}s, {int(done.sum())}/{n} finished, "
+         f"{int((counts == cap).sum())} (update, query) pairs at the cap")
+    dg = O.OGraph(w.vlabels, w.init_src, w.init_dst, w.init_elab)
+    compared = 0
+    for j, q in enumerate(qs):
+        t0 = time.time()
+        cnt, unf = O.graphflow_limited(dg, O.OGraph.from_query(q), w.upd_add, w.upd_v1,
+                                       w.upd_v2, w.upd_label, limit=n, max_results=cap,
+                                       update_seconds=1.0, total_seconds=15.0)
+        both = done & ~unf
+        _log(f"c4 query {j} (k={q.k}): oracle {time.time() - t0:.1f}s, {int(both.sum())} "
+             f"updates compared, {int((cnt[both] > 0).sum())} with matches")
+        np.testing.assert_array_equal(counts[both, j], cnt[both])
+        compared += int(both.sum())
+    assert compared >= n * len(qs) // 2
+
+    m = engine()
+    reqs = poisson_requests(ups, 1e4, seed=1, max_request=part)
+    srv = Server(m)
+    st = srv.serve(reqs)
+    srv.close()
+    m.close()
+    got = np.concatenate([r.counts for r in reqs])
+    status = np.concatenate([r.status for r in reqs])
+    ok = ((status & USTAT_TIME_LIMIT) == 0) & done
+    _log(f"c4 served: {st.requests} requests in {st.batches} batches, p50 {st.p50_ms:.1f} ms, "
+         f"{int(ok.sum())}/{n} updates complete in both runs")
+    assert st.batches > 1 and ok.sum() >= n // 2
+    np.testing.assert_array_equal(got[ok], counts[ok])
