@@ -113,6 +113,8 @@ def main() -> int:
     ap.add_argument("--split-chunk", type=int, default=0)
     ap.add_argument("--spill-budget", type=int, default=0)
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) batched leg")
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="served legs: spill budget of the served engines (0 = the one the "
                          "autotuner chose for the batched run)")
@@ -254,6 +256,10 @@ def main() -> int:
         log(f"{algo}: {v:.0f} updates/s, {cpu_:.0f} candidates per update, "
             f"counts equal to {args.algo}: {same}")
 
+    host = None if args.no_host else host_buffers_rate(w, queries, ups, args, counts)
+    if host:
+        log(f"host buffers (csm_apply_batch): {host}")
+
     # roofline of the search kernel: algorithmic bytes per launch over its
     # average launch duration (HIP events on the engine stream)
     launches = st1["search_launches"] - st0["search_launches"]
@@ -317,6 +323,7 @@ def main() -> int:
         "exact_counts_plain_vs_batched": exact,
         "roofline": roof,
         "other_algorithms": others,
+        "host_buffers": host,
         "ingest": ingest,
         "cpu_baseline": cpu,
     }
@@ -369,6 +376,38 @@ def serve_legs(w, queries, args, batched_rate, tuned, batched_counts, n_batched)
                     "counts_equal_batched": exact, "compared_updates": k})
         log(f"served at {load:.0f}/s offered: {out[-1]}")
     return out
+
+
+def host_buffers_rate(w, queries, ups, args, batched_counts):
+    """The batched path through csm_apply_batch with host arrays (records
+    copied in, counts copied out on every call): the PCIe-inclusive rate on
+    the same warm-up and timed batches, on a fresh engine with the batched
+    run's search settings."""
+    from paracosm_amd.engine import Matching
+    nq = len(queries)
+    m = Matching(args.algo, queries[0], w.vlabels, w.init_src, w.init_dst, w.init_elab,
+                 max_batch=args.batch, split_chunk=args.split_chunk,
+                 spill_budget=args.spill_budget)
+    for q in queries[1:]:
+        m.register_query(q)
+    m.Preprocessing()
+    counts = np.zeros((len(ups), nq), np.uint64)
+    for i in range(args.warmup):
+        s = i * args.batch
+        pos, neg, _, _ = m.apply_batch(ups[s:s + args.batch])
+        counts[s:s + args.batch] = pos + neg
+    t = time.perf_counter()
+    done = 0
+    for i in range(args.warmup, args.warmup + args.steps):
+        s = i * args.batch
+        pos, neg, _, _ = m.apply_batch(ups[s:s + args.batch])
+        counts[s:s + args.batch] = pos + neg
+        done += args.batch
+    elapsed = time.perf_counter() - t
+    m.close()
+    return {"updates_per_s": round(done / elapsed, 1),
+            "counts_equal": bool(np.array_equal(counts, batched_counts)),
+            "note": "csm_apply_batch: host update records in, host counts out, every batch"}
 
 
 def batched_only(spec, w, queries, ups, args):

@@ -43,11 +43,14 @@ def _num(out, what):
     return int(m.group(1))
 
 
-@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
-def test_cli_default_mode_totals(tmp_path, oracle_lib, algo):
+@pytest.mark.parametrize("algo,ref_orders", [("parallel_graphflow", 0), ("parallel_symbi", 0),
+                                             ("parallel_symbi", 1)])
+def test_cli_default_mode_totals(tmp_path, oracle_lib, algo, ref_orders):
+    """Totals and report lines; parallel_symbi with --reference-orders 1 runs
+    SymBi's search-time order rule."""
     O = oracle_lib
     w, q = _files(tmp_path, mixed=True)
-    rc, out = _run(tmp_path, "-a", algo)
+    rc, out = _run(tmp_path, "-a", algo, "--reference-orders", str(ref_orders))
     assert rc == 0, out
     ref = O.run("graphflow", O.OGraph(w.vlabels, w.init_src, w.init_dst, w.init_elab),
                 O.OGraph.from_query(q), w.upd_add, w.upd_v1, w.upd_v2, w.upd_label,
