@@ -620,6 +620,13 @@ __global__ void k_dir_build(const uint2* __restrict__ vrec, const uint32_t* __re
       if (s == NONE) continue;
     }
     uint32_t* __restrict__ D = dir + (size_t)s * dstride;
+    if (len >= 16u * dstride) {
+      // a long list (a hub): one binary search per key instead of a walk of
+      // every entry; D[b] is the first entry whose key is at least b
+      for (uint32_t b = lane; b <= nk; b += WAVE)
+        D[b] = b == nk ? len : lower_bound(adj, off, off + len, b << vbits) - off;
+      continue;
+    }
     // entry i opens the keys (key(i-1), key(i)]
     for (uint32_t i = lane; i < len; i += WAVE) {
       const uint32_t kc = adj[off + i] >> vbits;

@@ -629,3 +629,28 @@ def test_add_vertex_then_edges_matches_oracle(oracle_lib, algo):
                 want = O.index_dump(fg, O.OGraph.from_query(q), what, nv, q.k,
                                     tree=algo == "parallel_turboflux", dag_src=ig)
                 np.testing.assert_array_equal(got[:, :nv], want)
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_hub_lists(oracle_lib, algo):
+    """Lists long enough for their key directories to be built by one binary
+    search per key (at least 16 entries per directory slot): a hub joined to
+    every other vertex, a mixed stream around it, several batch sizes."""
+    from paracosm_amd.engine import make_updates
+    rng = np.random.default_rng(4242)
+    nv = 600
+    vl, s, d, el = random_graph(nv, 0.01, 1, 2, rng)
+    keep = s != 0                      # the hub (vertex 0) gets every other vertex
+    s = np.concatenate([s[keep], np.zeros(nv - 1, np.uint32)])
+    d = np.concatenate([d[keep], np.arange(1, nv, dtype=np.uint32)])
+    el = rng.integers(0, 2, size=len(s)).astype(np.uint32)
+    init, ups = mixed_stream_small(s, d, el, nv, 400, 0.6, 2, rng)
+    q = random_connected_query(4, 1, 1, 2, rng)
+    ref = _oracle(oracle_lib, q, vl, init, ups, want_hash=True)
+    assert ref.pos.sum() > 1000
+    for batch in (0, 37):
+        with _engine(algo, q, vl, init, want_hash=True, max_batch=batch) as m:
+            pos, neg, _, hsh = m.apply_batch(make_updates(*ups))
+        np.testing.assert_array_equal(pos[:, 0], ref.pos)
+        np.testing.assert_array_equal(neg[:, 0], ref.neg)
+        np.testing.assert_array_equal(hsh[:, 0], ref.hash)
