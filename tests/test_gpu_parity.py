@@ -654,3 +654,40 @@ def test_hub_lists(oracle_lib, algo):
         np.testing.assert_array_equal(pos[:, 0], ref.pos)
         np.testing.assert_array_equal(neg[:, 0], ref.neg)
         np.testing.assert_array_equal(hsh[:, 0], ref.hash)
+
+
+@pytest.mark.parametrize("algo", ["parallel_graphflow", "parallel_symbi"])
+def test_hub_list_merges(oracle_lib, algo):
+    """A hub list of 6,000-12,000 entries merged with thousands of new
+    entries per batch (and its key directory rebuilt by binary searches):
+    6,000 initial hub edges and 6,000 more inserted, among insertions and
+    deletions of 40,000 random edges, in batches of 4,000; triangles."""
+    from paracosm_amd.engine import make_updates
+    from paracosm_amd.gen import Query
+    rng = np.random.default_rng(99)
+    nv = 12000
+    a = rng.integers(1, nv, size=60000)
+    b = rng.integers(1, nv, size=60000)
+    rand = sorted({(int(min(x, y)), int(max(x, y))) for x, y in zip(a, b) if x != y})[:40000]
+    order = rng.permutation(len(rand))
+    held = [rand[i] for i in order[:5000]]                 # inserted by the stream
+    dels = [rand[i] for i in order[5000:6000]]             # deleted by the stream
+    init_pairs = [rand[i] for i in order[5000:]] + [(0, v) for v in range(1, 6001)]
+    ups = ([(1, 0, v) for v in range(6001, nv)] + [(1, x, y) for x, y in held] +
+           [(0, x, y) for x, y in dels])
+    ups = [ups[i] for i in rng.permutation(len(ups))]
+    init = (np.array([p[0] for p in init_pairs], np.uint32),
+            np.array([p[1] for p in init_pairs], np.uint32),
+            np.zeros(len(init_pairs), np.uint32))
+    stream = (np.array([u[0] for u in ups], np.uint8), np.array([u[1] for u in ups], np.uint32),
+              np.array([u[2] for u in ups], np.uint32), np.zeros(len(ups), np.uint32))
+    vl = np.zeros(nv, np.uint32)
+    q = Query(vlabels=np.zeros(3, np.uint32), src=np.array([0, 0, 1], np.uint32),
+              dst=np.array([1, 2, 2], np.uint32), elabels=np.zeros(3, np.uint32))
+    ref = _oracle(oracle_lib, q, vl, init, stream, want_hash=True)
+    assert ref.pos.sum() > 0 and ref.neg.sum() > 0
+    with _engine(algo, q, vl, init, want_hash=True, max_batch=4000) as m:
+        pos, neg, _, hsh = m.apply_batch(make_updates(*stream))
+    np.testing.assert_array_equal(pos[:, 0], ref.pos)
+    np.testing.assert_array_equal(neg[:, 0], ref.neg)
+    np.testing.assert_array_equal(hsh[:, 0], ref.hash)
