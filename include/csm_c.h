@@ -95,7 +95,7 @@ typedef struct csm_opts {
                                of its subtree to the next round (0 =
                                autotuned).  A budget of 256 or more is
                                quartered for batches of up to 2,048
-                               updates and halved up to 32,768 (not below
+                               updates and halved up to 8,192 (not below
                                64): shorter rounds for small batches     */
   uint32_t reference_orders; /* 1: search in the reference plugin's own
                                matching orders: parallel_graphflow the

@@ -1082,9 +1082,9 @@ SearchArgs search_args(csm_engine* e, unsigned long long* counts, unsigned long 
               (e->opts.want_work ? LIM_WORK : 0u) | (e->d_emb ? LIM_ENUM : 0u);
   sa.lim = e->d_limits;
   // spill budget for the batch's size: a quarter up to 2,048 updates, half
-  // up to 32,768 (served batches: shorter rounds, lower latency; measured on
+  // up to 8,192 (served batches: shorter rounds, lower latency; measured on
   // the config 2 workload)
-  sa.budget_shift = e->batch_n <= 2048 ? 2u : e->batch_n <= 32768 ? 1u : 0u;
+  sa.budget_shift = e->batch_n <= 2048 ? 2u : e->batch_n <= 8192 ? 1u : 0u;
   return sa;
 }
 
@@ -2296,7 +2296,7 @@ static csm_engine* create_engine(const char* algo, const csm_graph* query, const
   e->tune_active = e->opts.auto_tuning != 0 && e->opts.split_chunk == 0 &&
                    e->opts.spill_budget == 0 && !e->dyn;
   if (e->tune_active)
-    e->tune_cands = {{64, 1024}, {32, 1024}, {64, 512}, {64, 2048}, {128, 1024}};
+    e->tune_cands = {{64, 1024}, {64, 2048}, {64, 512}, {64, 384}, {64, 320}, {64, 256}, {32, 320}};
   for (int c = 0; c < NSIZE_CLASSES; ++c) {
     e->cls_chunk[c] = e->opts.split_chunk ? e->opts.split_chunk : 64;
     e->cls_budget[c] = e->opts.spill_budget ? e->opts.spill_budget : 1024;

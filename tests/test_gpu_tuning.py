@@ -11,7 +11,7 @@ from tests.helpers import walk_query
 
 pytestmark = pytest.mark.gpu
 
-CANDIDATES = {(64, 1024), (32, 1024), (64, 512), (64, 2048), (128, 1024)}
+CANDIDATES = {(64, 1024), (64, 2048), (64, 512), (64, 384), (64, 320), (64, 256), (32, 320)}
 BATCH = 4096   # the smallest batch the autotuner scores
 
 
